@@ -1,0 +1,172 @@
+/*
+ * diffphd.h — C-ABI of the MI355X-native DiffPhD projective-dynamics path.
+ *
+ * Drop-in boundary for the reference solver plugin SolverDiffPhD
+ * (src/newton_solver_diff_phd/_src/solvers/diff_phd/solver.py:47). The
+ * reference binds nothing through an FFI (it is Python + Warp); these entry
+ * points are what its Python class would call through ctypes in place of its
+ * Warp launches. Every pointer is either a host array (setup calls) or a raw
+ * device pointer (step calls: torch.Tensor.data_ptr()); streams are
+ * hipStream_t passed as void*. All calls return DPHD_OK or an error code;
+ * dphd_last_error() describes the last failure on the calling thread.
+ * Numerical failures never abort a step: they set status bits (see below),
+ * read with dphd_read_status.
+ */
+#ifndef DIFFPHD_H
+#define DIFFPHD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPHD_ABI_VERSION 1
+
+enum {
+  DPHD_OK = 0,
+  DPHD_EINVAL = 1,   /* invalid argument (reference: ValueError/TypeError) */
+  DPHD_ESTATE = 2,   /* call out of order (reference: RuntimeError) */
+  DPHD_EHIP = 3,     /* HIP runtime failure */
+  DPHD_ENOTIMPL = 4, /* unsupported option (reference: NotImplementedError) */
+  DPHD_ENOMEM = 5
+};
+
+enum { DPHD_TET4 = 0, DPHD_HEX8 = 1 };
+enum { DPHD_PD = 0, DPHD_NEO_HOOKEAN = 1 };
+
+/* Status bits, identical to DP/kernels.py:29-31 and DP/adjoint.py:64-65. */
+enum {
+  DPHD_STATUS_NOT_CONVERGED = 1,
+  DPHD_STATUS_SINGULAR = 2,
+  DPHD_STATUS_ACTIVATION = 4,
+  DPHD_STATUS_ADJOINT_NOT_CONVERGED = 8,
+  DPHD_STATUS_ADJOINT_SAFEGUARD = 16
+};
+
+typedef struct dphd_factor dphd_factor;
+typedef struct dphd_solver dphd_solver;
+
+const char* dphd_last_error(void);
+int dphd_version(void);
+
+/* ---- Exact sparse inverse factor A^{-1} = S^T S --------------------------
+ * Replaces SparseInverse.__init__ (DP/sparse.py:291-320): nested dissection,
+ * LDL^T, inverse rows, CSR of S and S^T. Input: full symmetric CSR (both
+ * halves, ascending columns). Runs on the host. */
+int dphd_factor_create(int n, const int* offsets, const int* columns, const double* values, dphd_factor** out);
+int dphd_factor_sizes(const dphd_factor* f, int64_t* nnz, int64_t* factorization_nnz, int64_t* matrix_nnz);
+int dphd_factor_export(const dphd_factor* f, int* permutation, int* s_offsets, int* s_columns, double* s_values,
+                       int* t_offsets, int* t_columns, double* t_values);
+void dphd_factor_destroy(dphd_factor* f);
+
+/* ---- Topology and geometry helpers ----------------------------------------
+ * Vertex incidence sorted by (element, corner) (DP/kernels.py:61-72,
+ * DP/hex.py:83-92); entries = element * corners + corner. */
+int dphd_build_incidence(int vertices, int elements, int corners, const int* indices, int* offsets, int* entries);
+/* Tet4 gradients [E,4,3] and volumes from float32 inverse rest edge matrices
+ * (DP/solver.py:214-223). */
+int dphd_tet_geometry(int elements, const float* poses, double* gradients, double* volumes);
+/* Hex8 shared Gauss gradients [8 samples, 8 corners, 3] (DP/hex.py:63-80). */
+int dphd_hex_gradients(double dx, double* out);
+/* Constant matrix of one body (DP/solver.py:521-545). Writes nnz; copies the
+ * CSR when capacity >= nnz. */
+int dphd_assemble(int vertices, int elements, int corners, int samples, const int* indices, const double* grads,
+                  int shared_grads, const double* weights, const double* fibre_dir, const double* fibre_w,
+                  const unsigned char* free, double inertia, int plane_count, const int* plane_vertices,
+                  const double* plane_stiffness, int* nnz, int* offsets, int* columns, double* values, int capacity);
+
+/* ---- Solver ----------------------------------------------------------------
+ * One solver holds B >= 1 independent bodies packed by vertex and element
+ * offsets (B = 1 is the reference's single-model solver). Constructor
+ * options mirror SolverDiffPhD.__init__ (DP/solver.py:157-179). */
+typedef struct {
+  int element_kind;      /* DPHD_TET4 or DPHD_HEX8 */
+  int material;          /* DPHD_PD or DPHD_NEO_HOOKEAN */
+  int anderson_window;   /* 0..8, reference default 5 */
+  int max_iterations;    /* >= 1 */
+  double abs_tol;        /* >= 0 */
+  double rel_tol;        /* >= 0 */
+  int factor_precision;  /* 64: FP64 S values; 32: FP32 S values with an FP64 defect correction */
+  int integrate_pinned_velocities;
+  int jacobian_mode;     /* 0: reference mean weights in K; 1: per-element weights (exact derivative) */
+  int iteration_chunk;   /* iterations per captured graph launch; 0 picks the default */
+} dphd_config;
+
+typedef struct {
+  int bodies, vertices, elements, samples_per_element, corners;
+  int64_t factor_nnz;       /* nnz(S) summed over bodies (S^T has the same) */
+  int64_t matrix_nnz;       /* nnz(A) per coordinate summed over bodies */
+  int64_t jacobian_bytes;   /* bytes of the stored adjoint Jacobian */
+  double setup_seconds;     /* host time of the last dphd_build */
+} dphd_info;
+
+int dphd_create(const dphd_config* config, int device, dphd_solver** out);
+void dphd_destroy(dphd_solver* s);
+
+/* Packed bodies: vertex_offsets[B+1], element_offsets[B+1]. */
+int dphd_set_bodies(dphd_solver* s, int bodies, const int* vertex_offsets, const int* element_offsets);
+/* Tet4: indices [E,4] global vertex ids, poses [E,3,3] float32 Dm^{-1}
+ * (Newton model.tet_indices / model.tet_poses). */
+int dphd_set_tet4(dphd_solver* s, const int* indices, const float* poses);
+/* Hex8: indices [E,8] binary corner order, rest vertices [N,3] (DP/hex.py:27-92). */
+int dphd_set_hex8(dphd_solver* s, const int* indices, const double* rest_vertices);
+/* inverse_mass [N] (0 = Dirichlet), gravity [B,3], density [B]. */
+int dphd_set_particles(dphd_solver* s, const float* inverse_mass, const double* gravity, const double* density);
+/* materials [E,2]: (mu, lambda) per element (model.tet_materials[:, :2]). */
+int dphd_set_materials(dphd_solver* s, const double* materials);
+/* Optional fibres (DP/solver.py:429-462): directions [E,3], stiffness [E]. */
+int dphd_set_muscles(dphd_solver* s, const double* directions, const double* stiffness);
+/* Mutable between steps (DP/solver.py:206-209). */
+int dphd_set_tolerances(dphd_solver* s, int max_iterations, double abs_tol, double rel_tol);
+/* Element weights, assembly and factor per body (DP/solver.py:491-552). */
+int dphd_build(dphd_solver* s, double dt);
+int dphd_get_info(const dphd_solver* s, dphd_info* info);
+
+/* One forward step of all bodies (DP/solver.py:689-786). q/qd/f are [N,3]
+ * float32 device arrays; dirichlet is [N,3] float32 or NULL
+ * (DP/solver.py:755-761); muscle_targets [E] float64 device or NULL (target
+ * fibre lengths a = 1 - u); checkpoint [N,3] float64 device or NULL
+ * receives the converged iterate for the adjoint (StepRecord.x). */
+int dphd_step(dphd_solver* s, const float* q_in, const float* qd_in, const float* f_in, const float* dirichlet,
+              const double* muscle_targets, float* q_out, float* qd_out, double* checkpoint, void* stream);
+
+/* Backward of one recorded step (Adjoint.backward, DP/adjoint.py:1088-1259).
+ * Seeds from gq_out/gqd_out (NULL = zero); accumulates (+=) into gq_in,
+ * gqd_in, gf_in [N,3] float32, gmaterials [E,2] float64 (mu, lambda; NULL
+ * skips material gradients) and gdirichlet [N,3] float32 (non-NULL when the
+ * step used explicit Dirichlet targets). */
+int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle_targets, const float* gq_out,
+                  const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
+                  float* gdirichlet, void* stream);
+
+/* Status word (DP/solver.py:380-390): OR of all bodies since the last clear. */
+int dphd_read_status(dphd_solver* s, int clear, int* bits);
+/* Iterations of the last forward (adjoint = 0) or backward (adjoint = 1)
+ * solve per body; host copy, synchronizes. */
+int dphd_iterations(dphd_solver* s, int adjoint, int* per_body);
+int dphd_synchronize(dphd_solver* s);
+
+/* Device-side application of A^{-1} = S^T S of every body to [N,3] float64
+ * device arrays (SparseInverse.apply, DP/sparse.py:350-352). */
+int dphd_apply_inverse(dphd_solver* s, const double* rhs, double* out, void* stream);
+
+/* Evaluate the pointwise projections of `count` float64 3x3 matrices on the
+ * device (the reference's _evaluate_projections test kernel,
+ * tests/test_diff_phd_solver.py:34-56): sigma [count,3], rotation, volume and
+ * neo-Hookean projections [count,3,3], fibre projection [count,3] with
+ * target length fibre_target. All device pointers. */
+int dphd_eval_projections(int count, const double* f, const double* directions, double mu, double lame,
+                          double penalty, double fibre_target, double* sigma, double* reconstruction,
+                          double* rotation, double* volume, double* neo_hookean, double* fibre, void* stream);
+/* Directional derivatives of the same projections along df (the reference's
+ * _jvps kernel, tests/test_diff_phd_adjoint.py:54-73). */
+int dphd_eval_jvps(int count, const double* f, const double* df, const double* directions, double mu, double lame,
+                   double fibre_target, double* rotation, double* volume, double* neo_hookean, double* fibre,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIFFPHD_H */

@@ -1,0 +1,148 @@
+// Device data layout and kernel launchers of the DiffPhD path on gfx950.
+//
+// Layout in HBM (all bodies packed; B = 1 is the single-model solver):
+//   vertices   [N]      double3 AoS vectors (x, targets, boundary, p, rhs, ...)
+//   elements   [E]      indices [E,K] int32, Tet4 poses [E,9] float32 (exact
+//                       source of the gradients), Hex8 gradient table per body
+//                       [B,8,8,3], per-element weights float64
+//   factor     S, S^T   CSR with global row/column ids, values float64 or float32
+//   history    [W+1,N]  Anderson ring: g = x_raw - x_prev and raw = x_prev + g
+//   chunks     256-vertex / 256-thread element chunks never straddle bodies;
+//              per-chunk partial sums are reduced per body in chunk order, so
+//              every reduction is deterministic and float-atomic free.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dphd {
+
+constexpr int kBlock = 256;
+constexpr int kMaxWindow = 8;
+constexpr int kGramEntries = kMaxWindow * (kMaxWindow + 1) / 2 + kMaxWindow;  // upper triangle + rhs
+
+// One workgroup of a CSR product: rows row_list[first, first + count) with
+// `lanes` lanes per row (1..64, or 256 for one row per workgroup).
+struct SpmvTask {
+  int first;
+  int count;
+  int lanes;
+};
+
+struct Chunk {
+  int body;
+  int begin;
+  int end;
+};
+
+// Pointers of one solver's device state (all owned by the solver).
+struct DeviceState {
+  // topology
+  int B = 0, N = 0, E = 0, K = 4, Q = 1;
+  int* idx = nullptr;          // [E*K]
+  float* poses = nullptr;      // Tet4 [E*9]
+  double* hexg = nullptr;      // Hex8 [B*192]
+  uint8_t* pinmask = nullptr;  // [E] bit k: corner k is a Dirichlet vertex
+  int* ebody = nullptr;        // [E]
+  int* vbody = nullptr;        // [N]
+  uint8_t* free = nullptr;     // [N]
+  int* inc_off = nullptr;      // [N+1]
+  int* inc_ent = nullptr;      // [E*K] entry = e*K + k
+  // weights
+  double *wc = nullptr, *wv = nullptr, *wn = nullptr;     // [E] per sample
+  double *jwc = nullptr, *jwv = nullptr, *jwn = nullptr;  // [E] Jacobian weights
+  double* mdir = nullptr;                                 // [E*3] or null
+  double* mw = nullptr;                                   // [E] or null
+  // per body
+  double* inertia = nullptr;  // [B]
+  double* mass = nullptr;     // [B]
+  double* gravity = nullptr;  // [B*3]
+  double* nh = nullptr;       // [B*3] mean mu, mean lambda, penalty
+  double* svol = nullptr;     // [B] sample volume
+  // factor
+  int *s_off = nullptr, *s_col = nullptr, *t_off = nullptr, *t_col = nullptr;
+  void *s_val = nullptr, *t_val = nullptr;  // double or float
+  int* s_rows = nullptr;                    // row lists of the tasks
+  int* t_rows = nullptr;
+  SpmvTask* s_tasks = nullptr;
+  SpmvTask* t_tasks = nullptr;
+  int s_ntasks = 0, t_ntasks = 0;
+  bool f32 = false;
+  // constant matrix for the defect correction (per coordinate)
+  int *a_off = nullptr, *a_col = nullptr;
+  double* a_val = nullptr;
+  // vectors [N*3]
+  double *q0 = nullptr, *qt = nullptr, *xd = nullptr, *x = nullptr, *xr = nullptr, *p = nullptr, *rhs = nullptr,
+         *y = nullptr;
+  double* contrib = nullptr;  // [E*K*3]
+  // Anderson
+  int W = 0;
+  double *hg = nullptr, *hr = nullptr;  // [(W+1)*N*3]
+  int* aa = nullptr;                    // [B*4] count, start, saved count, saved start
+  int* accept = nullptr;                // [B]
+  double* gamma = nullptr;              // [B*8]
+  // chunks
+  Chunk* vchunks = nullptr;
+  int nvc = 0;
+  int* vcb = nullptr;  // [B+1] body chunk ranges
+  Chunk* echunks = nullptr;
+  int nec = 0;
+  double* gram_part = nullptr;  // [nvc*kGramEntries]
+  double* conv_part = nullptr;  // [nvc*4]
+  double* clip_part = nullptr;  // [nvc]
+  // iteration control
+  int* iter = nullptr;      // [B]
+  int* active = nullptr;    // [B]
+  int* status = nullptr;    // [1]
+  int* any_active = nullptr;  // mapped host int
+  // adjoint
+  double* jac = nullptr;     // [E*Q*81]
+  double *arhs = nullptr, *sol = nullptr;  // [N*3]
+  int* aiter = nullptr;      // [B]
+  int* aactive = nullptr;    // [B]
+  double* ascale = nullptr;  // [B*2] clip scales
+  double* route = nullptr;   // [B*2] routing scale, diverged flag
+  double* gmat_tmp = nullptr;  // [E*2]
+  int* valid = nullptr;        // [B]
+  int* ones = nullptr;         // [B] all ones (every body active)
+  int* eoff = nullptr;         // [B+1] element offsets
+  // options
+  int material = 0;
+  int has_muscles = 0;
+  int integrate = 0;
+  int defect = 0;  // FP32 factor: x_raw = x + S^T S (b - A x)
+};
+
+struct StepScalars {
+  double dt;
+  double abs_tol, rel_tol;
+  int max_iterations;
+};
+
+// Launchers (all asynchronous on `stream`).
+void launch_begin_step(const DeviceState& d, const StepScalars& s, const float* q, const float* qd, const float* f,
+                       const float* dirichlet, hipStream_t stream);
+void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const double* targets, hipStream_t stream);
+void launch_any_active(const DeviceState& d, const int* active, hipStream_t stream);
+void launch_write_state(const DeviceState& d, const StepScalars& s, float* q, float* qd, double* checkpoint,
+                        hipStream_t stream);
+
+void launch_adjoint_setup(const DeviceState& d, const StepScalars& s, const double* x, const double* targets,
+                          const float* gq_out, const float* gqd_out, hipStream_t stream);
+void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStream_t stream);
+void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const double* x, const float* gqd_out,
+                            float* gq_in, float* gqd_in, float* gf_in, double* gmat, float* gdir,
+                            bool any_pinned, hipStream_t stream);
+
+void launch_material_finish(const DeviceState& d, double* gmat, hipStream_t stream);
+void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, hipStream_t stream);
+
+void launch_eval_projections(int count, const double* f, const double* dirs, double mu, double lame, double penalty,
+                             double target, double* sigma, double* recon, double* rotation, double* volume,
+                             double* nh, double* fibre, hipStream_t stream);
+void launch_eval_jvps(int count, const double* f, const double* df, const double* dirs, double mu, double lame,
+                      double target, double* rotation, double* volume, double* nh, double* fibre,
+                      hipStream_t stream);
+
+}  // namespace dphd

@@ -1,0 +1,886 @@
+// Host orchestration of the DiffPhD path and the solver half of the C-ABI.
+//
+// A solver owns packed bodies, their factors and every device buffer. The
+// local/global/Anderson loop has no host synchronization inside an
+// iteration: `iteration_chunk` iterations are captured once into a hipGraph
+// whose kernels are predicated on each body's device-side `active` flag, and
+// the host polls one mapped flag between graph launches (HIP 7.2 has no
+// conditional graph nodes). Reference orchestration: DP/solver.py:582-786
+// (forward) and DP/adjoint.py:1054-1259 (backward).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/diffphd.h"
+#include "error.h"
+#include "kernels.h"
+#include "setup.h"
+
+namespace dphd {
+namespace {
+
+#define HIP_CHECK(expr)                                                                            \
+  do {                                                                                             \
+    hipError_t _e = (expr);                                                                        \
+    if (_e != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+template <typename T>
+T* dalloc(std::vector<void*>& owned, size_t count) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+  owned.push_back(p);
+  return static_cast<T*>(p);
+}
+
+template <typename T>
+T* dupload(std::vector<void*>& owned, const std::vector<T>& host) {
+  T* p = dalloc<T>(owned, host.size());
+  if (!host.empty()) HIP_CHECK(hipMemcpy(p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+template <typename T>
+T* dzeros(std::vector<void*>& owned, size_t count) {
+  T* p = dalloc<T>(owned, count);
+  HIP_CHECK(hipMemset(p, 0, std::max<size_t>(count, 1) * sizeof(T)));
+  return p;
+}
+
+int pick_lanes(int len) {
+  if (len > 2048) return kBlock;
+  int lanes = 1;
+  while (lanes < 64 && lanes * 4 < len) lanes <<= 1;
+  return lanes;
+}
+
+// Row binning of one CSR: rows grouped by lanes-per-row, ascending row id
+// within a group; one task per workgroup.
+void build_tasks(const std::vector<int>& off, std::vector<int>& rows, std::vector<SpmvTask>& tasks) {
+  const int n = static_cast<int>(off.size()) - 1;
+  const int classes[] = {1, 2, 4, 8, 16, 32, 64, kBlock};
+  rows.clear();
+  tasks.clear();
+  for (int L : classes) {
+    const int first = static_cast<int>(rows.size());
+    for (int r = 0; r < n; ++r)
+      if (pick_lanes(off[r + 1] - off[r]) == L) rows.push_back(r);
+    const int count = static_cast<int>(rows.size()) - first;
+    const int per = L == kBlock ? 1 : kBlock / L;
+    for (int k = 0; k < count; k += per) tasks.push_back(SpmvTask{first + k, std::min(per, count - k), L});
+  }
+}
+
+}  // namespace
+}  // namespace dphd
+
+struct dphd_solver {
+  dphd_config cfg{};
+  int device = 0;
+  // topology / inputs (host)
+  int B = 0, N = 0, E = 0, K = 4, Q = 1;
+  std::vector<int> voff, eoff;
+  std::vector<int> indices;
+  std::vector<float> poses;
+  std::vector<double> rest;
+  std::vector<float> inv_mass;
+  std::vector<double> gravity, density;
+  std::vector<double> materials;
+  std::vector<double> mdir_in, mstiff_in;
+  bool have_topology = false, have_particles = false, have_materials = false;
+  // build products
+  bool built = false;
+  double dt = 0.0;
+  dphd_info info{};
+  bool any_pinned = false;
+  std::vector<void*> owned;
+  dphd::DeviceState d;
+  int* any_host = nullptr;
+  double* targets = nullptr;  // device copy of muscle targets [E]
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraphExec_t fwd_exec = nullptr, adj_exec = nullptr;
+  int fwd_key_iter = -1, adj_key_iter = -1;
+  double fwd_key_abs = -1, fwd_key_rel = -1, adj_key_abs = -1, adj_key_rel = -1;
+  int last_fwd_iters = 0, last_adj_iters = 0;
+
+  ~dphd_solver() { release(); }
+
+  void release() {
+    if (fwd_exec) (void)hipGraphExecDestroy(fwd_exec);
+    if (adj_exec) (void)hipGraphExecDestroy(adj_exec);
+    fwd_exec = adj_exec = nullptr;
+    for (void* p : owned) (void)hipFree(p);
+    owned.clear();
+    if (any_host) (void)hipHostFree(any_host);
+    any_host = nullptr;
+    built = false;
+  }
+};
+
+namespace dphd {
+namespace {
+
+int chunk_size(const dphd_solver* s) { return s->cfg.iteration_chunk > 0 ? s->cfg.iteration_chunk : 8; }
+
+void build_solver(dphd_solver* s, double dt) {
+  const auto t0 = std::chrono::steady_clock::now();
+  HIP_CHECK(hipSetDevice(s->device));
+  if (!s->have_topology || !s->have_particles || !s->have_materials)
+    throw std::invalid_argument("dphd_build: bodies, topology, particles and materials must be set first");
+  if (!std::isfinite(dt) || dt <= 0.0) throw std::invalid_argument("dt must be finite and positive");
+  s->release();
+  const int B = s->B, N = s->N, E = s->E, K = s->K, Q = s->Q;
+  const bool hex = s->cfg.element_kind == DPHD_HEX8;
+  const bool pd = s->cfg.material == DPHD_PD;
+  // Validation of DP/solver.py:464-489.
+  for (int e = 0; e < E; ++e) {
+    const double mu = s->materials[2 * e], lam = s->materials[2 * e + 1];
+    if (!std::isfinite(mu) || !std::isfinite(lam) || mu <= 0.0 || lam < 0.0)
+      throw std::invalid_argument("DiffPhD requires positive shear modulus and nonnegative Lame lambda");
+  }
+  std::vector<uint8_t> free(N);
+  for (int i = 0; i < N; ++i) {
+    const float w = s->inv_mass[i];
+    if (!std::isfinite(w) || w < 0.0f) throw std::invalid_argument("Particle inverse masses must be finite and nonnegative");
+    free[i] = w > 0.0f;
+  }
+  s->any_pinned = false;
+  for (int i = 0; i < N; ++i) s->any_pinned |= !free[i];
+  std::vector<int> vbody(N), ebody(E);
+  for (int b = 0; b < B; ++b) {
+    bool any_free = false;
+    for (int i = s->voff[b]; i < s->voff[b + 1]; ++i) {
+      vbody[i] = b;
+      any_free |= free[i] != 0;
+    }
+    if (!any_free) throw std::invalid_argument("SolverDiffPhD needs at least one dynamic particle per body");
+    for (int e = s->eoff[b]; e < s->eoff[b + 1]; ++e) {
+      ebody[e] = b;
+      for (int k = 0; k < K; ++k) {
+        const int v = s->indices[static_cast<size_t>(e) * K + k];
+        if (v < s->voff[b] || v >= s->voff[b + 1])
+          throw std::invalid_argument("element indices must reference vertices of their own body");
+      }
+    }
+  }
+  // Geometry per body.
+  std::vector<double> grads;  // Tet4 [E*12]
+  std::vector<double> hexg(hex ? static_cast<size_t>(B) * 192 : 0);
+  std::vector<double> mean_volume(B), svol(B), mass(B), inertia(B);
+  if (!hex) {
+    grads.resize(static_cast<size_t>(E) * 12);
+    std::vector<double> vols(E);
+    tet_geometry(E, s->poses.data(), grads.data(), vols.data());
+    for (int b = 0; b < B; ++b) {
+      double total = 0.0;
+      const int e0 = s->eoff[b], e1 = s->eoff[b + 1];
+      for (int e = e0; e < e1; ++e) {
+        if (!std::isfinite(vols[e]) || vols[e] <= 0.0)
+          throw std::invalid_argument("tetrahedra must have nondegenerate rest shapes");
+        total += vols[e];
+      }
+      mean_volume[b] = total / (e1 - e0);
+    }
+  } else {
+    for (int b = 0; b < B; ++b) {
+      std::string err;
+      const int e0 = s->eoff[b], e1 = s->eoff[b + 1];
+      const double mv = hex_mean_volume(e1 - e0, s->indices.data() + static_cast<size_t>(e0) * 8, s->rest.data(), &err);
+      if (!err.empty()) throw std::invalid_argument(err);
+      mean_volume[b] = mv;
+      hex_gradients(std::pow(mv, 1.0 / 3.0), hexg.data() + 192 * static_cast<size_t>(b));
+    }
+  }
+  for (int b = 0; b < B; ++b) {
+    svol[b] = mean_volume[b] / Q;
+    mass[b] = s->density[b] * mean_volume[b];
+    inertia[b] = mass[b] / (dt * dt);
+  }
+  // Element weights (DP/kernels.py:112-133, DP/hex.py:95-112) and Jacobian weights.
+  std::vector<double> wc(E, 0.0), wv(E, 0.0), wn(E, 0.0), jwc(E, 0.0), jwv(E, 0.0), jwn(E, 0.0), nh(3 * B);
+  for (int b = 0; b < B; ++b) {
+    const int e0 = s->eoff[b], e1 = s->eoff[b + 1];
+    double smu = 0.0, slam = 0.0;
+    for (int e = e0; e < e1; ++e) {
+      smu += s->materials[2 * e];
+      slam += s->materials[2 * e + 1];
+    }
+    const double mmu = smu / (e1 - e0), mlam = slam / (e1 - e0);
+    nh[3 * b] = mmu;
+    nh[3 * b + 1] = mlam;
+    nh[3 * b + 2] = 2.0 * mmu;
+    for (int e = e0; e < e1; ++e) {
+      const double mu = s->materials[2 * e], lam = s->materials[2 * e + 1];
+      if (pd) {
+        wc[e] = 2.0 * mu * svol[b];
+        wv[e] = lam * svol[b];
+      } else {
+        wn[e] = 2.0 * mu * svol[b];
+      }
+      if (s->cfg.jacobian_mode == 0) {
+        jwc[e] = pd ? 2.0 * mmu * svol[b] : 0.0;
+        jwv[e] = pd ? mlam * svol[b] : 0.0;
+        jwn[e] = pd ? 0.0 : 2.0 * mmu * svol[b];
+      } else {
+        jwc[e] = wc[e];
+        jwv[e] = wv[e];
+        jwn[e] = wn[e];
+      }
+    }
+  }
+  // Fibres (DP/solver.py:429-462).
+  const bool muscles = !s->mstiff_in.empty();
+  std::vector<double> mdir, mw;
+  if (muscles) {
+    mdir.assign(static_cast<size_t>(E) * 3, 0.0);
+    mw.assign(E, 0.0);
+    for (int e = 0; e < E; ++e) {
+      const double k = s->mstiff_in[e];
+      if (!(k > 0.0)) continue;
+      const double* m = &s->mdir_in[3 * static_cast<size_t>(e)];
+      const double len = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+      if (len <= 2.220446049250313e-16) throw std::invalid_argument("Active muscle directions must have nonzero length");
+      for (int a = 0; a < 3; ++a) mdir[3 * static_cast<size_t>(e) + a] = m[a] / len;
+      mw[e] = k * svol[ebody[e]];
+    }
+  }
+  // Assembly and factor per body, in parallel over bodies.
+  std::vector<Factor> factors(B);
+  std::vector<Csr> amats(B);
+  std::vector<std::string> errors(B);
+  {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nthreads = static_cast<int>(std::min<unsigned>(hw, static_cast<unsigned>(B)));
+    std::vector<std::thread> pool;
+    std::atomic<int> next{0};
+    for (int t = 0; t < nthreads; ++t)
+      pool.emplace_back([&]() {
+        for (int b = next++; b < B; b = next++) {
+          try {
+            const int v0 = s->voff[b], v1 = s->voff[b + 1], e0 = s->eoff[b], e1 = s->eoff[b + 1];
+            std::vector<int> local(static_cast<size_t>(e1 - e0) * K);
+            for (size_t t2 = 0; t2 < local.size(); ++t2) local[t2] = s->indices[static_cast<size_t>(e0) * K + t2] - v0;
+            std::vector<double> weights(e1 - e0);
+            for (int e = e0; e < e1; ++e) weights[e - e0] = (wc[e] + wv[e]) + wn[e];
+            AssemblyInput in;
+            in.vertices = v1 - v0;
+            in.elements = e1 - e0;
+            in.corners = K;
+            in.samples = Q;
+            in.indices = local.data();
+            in.grads = hex ? hexg.data() + 192 * static_cast<size_t>(b) : grads.data() + static_cast<size_t>(e0) * 12;
+            in.shared_grads = hex;
+            in.weights = weights.data();
+            in.fibre_dir = muscles ? mdir.data() + 3 * static_cast<size_t>(e0) : nullptr;
+            in.fibre_w = muscles ? mw.data() + e0 : nullptr;
+            in.free = free.data() + v0;
+            in.inertia = inertia[b];
+            amats[b] = assemble_matrix(in);
+            factors[b] = build_factor(amats[b]);
+          } catch (const std::exception& ex) {
+            errors[b] = ex.what();
+          }
+        }
+      });
+    for (auto& th : pool) th.join();
+  }
+  for (int b = 0; b < B; ++b)
+    if (!errors[b].empty()) throw std::invalid_argument(errors[b]);
+  // Global CSR (rows/columns offset by the body's first vertex).
+  int64_t znnz = 0, annz = 0;
+  for (int b = 0; b < B; ++b) {
+    znnz += static_cast<int64_t>(factors[b].s.col.size());
+    annz += static_cast<int64_t>(amats[b].col.size());
+  }
+  if (znnz > INT32_MAX) throw std::invalid_argument("inverse factor fill exceeds int32 sparse indexing");
+  std::vector<int> s_off(N + 1, 0), s_col(znnz), t_off(N + 1, 0), t_col(znnz);
+  std::vector<double> s_val(znnz), t_val(znnz);
+  {
+    int64_t zs = 0, zt = 0;
+    for (int b = 0; b < B; ++b) {
+      const int v0 = s->voff[b];
+      const Factor& f = factors[b];
+      for (int r = 0; r < f.n; ++r) {
+        for (int p = f.s.off[r]; p < f.s.off[r + 1]; ++p) {
+          s_col[zs] = f.s.col[p] + v0;
+          s_val[zs] = f.s.val[p];
+          ++zs;
+        }
+        s_off[v0 + r + 1] = static_cast<int>(zs);
+        for (int p = f.st.off[r]; p < f.st.off[r + 1]; ++p) {
+          t_col[zt] = f.st.col[p] + v0;
+          t_val[zt] = f.st.val[p];
+          ++zt;
+        }
+        t_off[v0 + r + 1] = static_cast<int>(zt);
+      }
+    }
+  }
+  std::vector<int> a_off, a_col;
+  std::vector<double> a_val;
+  const bool defect = s->cfg.factor_precision == 32;
+  if (defect) {
+    a_off.assign(N + 1, 0);
+    for (int b = 0; b < B; ++b) {
+      const int v0 = s->voff[b];
+      const Csr& a = amats[b];
+      for (int r = 0; r < a.n; ++r) {
+        for (int p = a.off[r]; p < a.off[r + 1]; ++p) {
+          a_col.push_back(a.col[p] + v0);
+          a_val.push_back(a.val[p]);
+        }
+        a_off[v0 + r + 1] = static_cast<int>(a_col.size());
+      }
+    }
+  }
+  std::vector<int> s_rows, t_rows;
+  std::vector<SpmvTask> s_tasks, t_tasks;
+  build_tasks(s_off, s_rows, s_tasks);
+  build_tasks(t_off, t_rows, t_tasks);
+  // Chunks and incidence.
+  std::vector<Chunk> vchunks, echunks;
+  std::vector<int> vcb(B + 1, 0);
+  const int eper = hex ? kBlock / 8 : kBlock;
+  for (int b = 0; b < B; ++b) {
+    vcb[b] = static_cast<int>(vchunks.size());
+    for (int i = s->voff[b]; i < s->voff[b + 1]; i += kBlock) vchunks.push_back(Chunk{b, i, std::min(i + kBlock, s->voff[b + 1])});
+    for (int e = s->eoff[b]; e < s->eoff[b + 1]; e += eper) echunks.push_back(Chunk{b, e, std::min(e + eper, s->eoff[b + 1])});
+  }
+  vcb[B] = static_cast<int>(vchunks.size());
+  std::vector<int> inc_off, inc_ent;
+  build_incidence(N, E, K, s->indices.data(), inc_off, inc_ent);
+  std::vector<uint8_t> pinmask(E, 0);
+  for (int e = 0; e < E; ++e)
+    for (int k = 0; k < K; ++k)
+      if (!free[s->indices[static_cast<size_t>(e) * K + k]]) pinmask[e] |= static_cast<uint8_t>(1u << k);
+
+  // Upload.
+  auto& o = s->owned;
+  DeviceState& d = s->d;
+  d = DeviceState{};
+  d.B = B;
+  d.N = N;
+  d.E = E;
+  d.K = K;
+  d.Q = Q;
+  d.idx = dupload(o, s->indices);
+  if (!hex) d.poses = dupload(o, s->poses);
+  if (hex) d.hexg = dupload(o, hexg);
+  d.pinmask = dupload(o, pinmask);
+  d.ebody = dupload(o, ebody);
+  d.vbody = dupload(o, vbody);
+  d.free = dupload(o, free);
+  d.inc_off = dupload(o, inc_off);
+  d.inc_ent = dupload(o, inc_ent);
+  d.wc = dupload(o, wc);
+  d.wv = dupload(o, wv);
+  d.wn = dupload(o, wn);
+  d.jwc = dupload(o, jwc);
+  d.jwv = dupload(o, jwv);
+  d.jwn = dupload(o, jwn);
+  if (muscles) {
+    d.mdir = dupload(o, mdir);
+    d.mw = dupload(o, mw);
+  }
+  d.inertia = dupload(o, inertia);
+  d.mass = dupload(o, mass);
+  d.gravity = dupload(o, s->gravity);
+  d.nh = dupload(o, nh);
+  d.svol = dupload(o, svol);
+  d.s_off = dupload(o, s_off);
+  d.s_col = dupload(o, s_col);
+  d.t_off = dupload(o, t_off);
+  d.t_col = dupload(o, t_col);
+  if (defect) {
+    std::vector<float> sv(s_val.begin(), s_val.end()), tv(t_val.begin(), t_val.end());
+    d.s_val = dupload(o, sv);
+    d.t_val = dupload(o, tv);
+    d.f32 = true;
+    d.a_off = dupload(o, a_off);
+    d.a_col = dupload(o, a_col);
+    d.a_val = dupload(o, a_val);
+    d.defect = 1;
+  } else {
+    d.s_val = dupload(o, s_val);
+    d.t_val = dupload(o, t_val);
+  }
+  d.s_rows = dupload(o, s_rows);
+  d.t_rows = dupload(o, t_rows);
+  d.s_tasks = dupload(o, s_tasks);
+  d.t_tasks = dupload(o, t_tasks);
+  d.s_ntasks = static_cast<int>(s_tasks.size());
+  d.t_ntasks = static_cast<int>(t_tasks.size());
+  const size_t v3 = static_cast<size_t>(N) * 3;
+  d.q0 = dzeros<double>(o, v3);
+  d.qt = dzeros<double>(o, v3);
+  d.xd = dzeros<double>(o, v3);
+  d.x = dzeros<double>(o, v3);
+  d.xr = dzeros<double>(o, v3);
+  d.p = dzeros<double>(o, v3);
+  d.rhs = dzeros<double>(o, v3);
+  d.y = dzeros<double>(o, v3);
+  d.contrib = dzeros<double>(o, static_cast<size_t>(E) * K * 3);
+  d.W = s->cfg.anderson_window;
+  if (d.W > 0) {
+    d.hg = dzeros<double>(o, static_cast<size_t>(d.W + 1) * v3);
+    d.hr = dzeros<double>(o, static_cast<size_t>(d.W + 1) * v3);
+  }
+  d.aa = dzeros<int>(o, 4 * static_cast<size_t>(B));
+  d.accept = dzeros<int>(o, B);
+  d.gamma = dzeros<double>(o, static_cast<size_t>(B) * kMaxWindow);
+  d.vchunks = dupload(o, vchunks);
+  d.nvc = static_cast<int>(vchunks.size());
+  d.vcb = dupload(o, vcb);
+  d.echunks = dupload(o, echunks);
+  d.nec = static_cast<int>(echunks.size());
+  d.gram_part = dzeros<double>(o, static_cast<size_t>(d.nvc) * kGramEntries);
+  d.conv_part = dzeros<double>(o, static_cast<size_t>(d.nvc) * 4);
+  d.clip_part = dzeros<double>(o, d.nvc);
+  d.iter = dzeros<int>(o, B);
+  d.active = dzeros<int>(o, B);
+  d.status = dzeros<int>(o, 1);
+  d.arhs = dzeros<double>(o, v3);
+  d.sol = dzeros<double>(o, v3);
+  d.aiter = dzeros<int>(o, B);
+  d.aactive = dzeros<int>(o, B);
+  d.ascale = dzeros<double>(o, 2 * static_cast<size_t>(B));
+  d.route = dzeros<double>(o, 2 * static_cast<size_t>(B));
+  d.gmat_tmp = dzeros<double>(o, 2 * static_cast<size_t>(E));
+  d.valid = dzeros<int>(o, B);
+  d.ones = dupload(o, std::vector<int>(B, 1));
+  d.eoff = dupload(o, s->eoff);
+  d.jac = nullptr;  // allocated on the first backward
+  d.material = pd ? 0 : 1;
+  d.has_muscles = muscles ? 1 : 0;
+  d.integrate = s->cfg.integrate_pinned_velocities ? 1 : 0;
+  s->targets = muscles ? dupload(o, std::vector<double>(E, 1.0)) : nullptr;
+  void* host = nullptr;
+  HIP_CHECK(hipHostMalloc(&host, sizeof(int), hipHostMallocMapped));
+  s->any_host = static_cast<int*>(host);
+  *s->any_host = 0;
+  void* dev = nullptr;
+  HIP_CHECK(hipHostGetDevicePointer(&dev, host, 0));
+  d.any_active = static_cast<int*>(dev);
+  if (!s->stream) HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  if (!s->ev_in) HIP_CHECK(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming));
+  if (!s->ev_out) HIP_CHECK(hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming));
+  HIP_CHECK(hipDeviceSynchronize());
+  s->dt = dt;
+  s->built = true;
+  s->fwd_key_iter = s->adj_key_iter = -1;
+  s->last_fwd_iters = s->last_adj_iters = 0;
+  int64_t jac_bytes = static_cast<int64_t>(E) * Q * 81 * 8;
+  s->info.bodies = B;
+  s->info.vertices = N;
+  s->info.elements = E;
+  s->info.samples_per_element = Q;
+  s->info.corners = K;
+  s->info.factor_nnz = znnz;
+  s->info.matrix_nnz = annz;
+  s->info.jacobian_bytes = jac_bytes;
+  s->info.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+StepScalars scalars(const dphd_solver* s) {
+  return StepScalars{s->dt, s->cfg.abs_tol, s->cfg.rel_tol, s->cfg.max_iterations};
+}
+
+hipGraphExec_t capture(dphd_solver* s, bool adjoint) {
+  const StepScalars sc = scalars(s);
+  const int C = chunk_size(s);
+  hipGraph_t graph = nullptr;
+  HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  for (int c = 0; c < C; ++c) {
+    if (adjoint)
+      launch_adjoint_iteration(s->d, sc, s->stream);
+    else
+      launch_forward_iteration(s->d, sc, s->targets, s->stream);
+  }
+  launch_any_active(s->d, adjoint ? s->d.aactive : s->d.active, s->stream);
+  HIP_CHECK(hipStreamEndCapture(s->stream, &graph));
+  hipGraphExec_t exec = nullptr;
+  HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  HIP_CHECK(hipGraphDestroy(graph));
+  return exec;
+}
+
+// Run the predicated loop to every body's convergence: launch enough chunks
+// to cover the previous solve's iterations back to back, then poll.
+int run_loop(dphd_solver* s, bool adjoint) {
+  hipGraphExec_t& exec = adjoint ? s->adj_exec : s->fwd_exec;
+  int& key_iter = adjoint ? s->adj_key_iter : s->fwd_key_iter;
+  double& key_abs = adjoint ? s->adj_key_abs : s->fwd_key_abs;
+  double& key_rel = adjoint ? s->adj_key_rel : s->fwd_key_rel;
+  if (!exec || key_iter != s->cfg.max_iterations || key_abs != s->cfg.abs_tol || key_rel != s->cfg.rel_tol) {
+    if (exec) HIP_CHECK(hipGraphExecDestroy(exec));
+    exec = capture(s, adjoint);
+    key_iter = s->cfg.max_iterations;
+    key_abs = s->cfg.abs_tol;
+    key_rel = s->cfg.rel_tol;
+  }
+  const int C = chunk_size(s);
+  const int max_chunks = (s->cfg.max_iterations + C - 1) / C;
+  const int last = adjoint ? s->last_adj_iters : s->last_fwd_iters;
+  // One chunk fewer than last time, so a step that converges faster wastes
+  // at most one predicated-off chunk.
+  int first = std::max(1, std::min(max_chunks, last / C - 1));
+  int launched = 0;
+  while (launched < max_chunks) {
+    const int burst = launched == 0 ? first : 1;
+    for (int k = 0; k < burst && launched < max_chunks; ++k, ++launched) HIP_CHECK(hipGraphLaunch(exec, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    if (!*reinterpret_cast<volatile int*>(s->any_host)) break;
+  }
+  return launched;
+}
+
+}  // namespace
+}  // namespace dphd
+
+using dphd::set_error;
+
+#define DPHD_TRY(body)                                    \
+  try {                                                   \
+    body;                                                 \
+    return DPHD_OK;                                       \
+  } catch (const std::invalid_argument& e) {              \
+    set_error(e.what());                                  \
+    return DPHD_EINVAL;                                   \
+  } catch (const std::bad_alloc&) {                       \
+    set_error("out of memory");                           \
+    return DPHD_ENOMEM;                                   \
+  } catch (const std::exception& e) {                     \
+    set_error(e.what());                                  \
+    return DPHD_EHIP;                                     \
+  }
+
+extern "C" {
+
+int dphd_create(const dphd_config* config, int device, dphd_solver** out) {
+  if (!config || !out) {
+    set_error("dphd_create: null argument");
+    return DPHD_EINVAL;
+  }
+  const dphd_config& c = *config;
+  if (c.element_kind != DPHD_TET4 && c.element_kind != DPHD_HEX8) {
+    set_error("element_kind must be DPHD_TET4 or DPHD_HEX8");
+    return DPHD_EINVAL;
+  }
+  if (c.material != DPHD_PD && c.material != DPHD_NEO_HOOKEAN) {
+    set_error("material must be 'pd' or 'neo_hookean'");
+    return DPHD_EINVAL;
+  }
+  if (c.anderson_window < 0 || c.anderson_window > dphd::kMaxWindow) {
+    set_error("anderson_window must be an integer in [0, 8]");
+    return DPHD_EINVAL;
+  }
+  if (c.max_iterations < 1) {
+    set_error("max_iterations must be a positive integer");
+    return DPHD_EINVAL;
+  }
+  if (!std::isfinite(c.abs_tol) || c.abs_tol < 0 || !std::isfinite(c.rel_tol) || c.rel_tol < 0) {
+    set_error("abs_tol and rel_tol must be finite and nonnegative");
+    return DPHD_EINVAL;
+  }
+  if (c.factor_precision != 64 && c.factor_precision != 32) {
+    set_error("factor_precision must be 64 or 32");
+    return DPHD_EINVAL;
+  }
+  DPHD_TRY({
+    HIP_CHECK(hipSetDevice(device));
+    auto* s = new dphd_solver;
+    s->cfg = c;
+    s->device = device;
+    s->K = c.element_kind == DPHD_TET4 ? 4 : 8;
+    s->Q = c.element_kind == DPHD_TET4 ? 1 : 8;
+    *out = s;
+  })
+}
+
+void dphd_destroy(dphd_solver* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  s->release();
+  if (s->ev_in) (void)hipEventDestroy(s->ev_in);
+  if (s->ev_out) (void)hipEventDestroy(s->ev_out);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int dphd_set_bodies(dphd_solver* s, int bodies, const int* vertex_offsets, const int* element_offsets) {
+  DPHD_TRY({
+    if (!s || bodies < 1 || !vertex_offsets || !element_offsets) throw std::invalid_argument("dphd_set_bodies: invalid arguments");
+    if (vertex_offsets[0] != 0 || element_offsets[0] != 0) throw std::invalid_argument("offsets must start at 0");
+    for (int b = 0; b < bodies; ++b)
+      if (vertex_offsets[b + 1] <= vertex_offsets[b] || element_offsets[b + 1] <= element_offsets[b])
+        throw std::invalid_argument("every body needs vertices and elements");
+    s->B = bodies;
+    s->voff.assign(vertex_offsets, vertex_offsets + bodies + 1);
+    s->eoff.assign(element_offsets, element_offsets + bodies + 1);
+    s->N = s->voff[bodies];
+    s->E = s->eoff[bodies];
+    s->have_topology = false;
+    s->release();
+  })
+}
+
+int dphd_set_tet4(dphd_solver* s, const int* indices, const float* poses) {
+  DPHD_TRY({
+    if (!s || !indices || !poses || s->B == 0) throw std::invalid_argument("dphd_set_tet4: set bodies first");
+    if (s->cfg.element_kind != DPHD_TET4) throw std::invalid_argument("solver was created for Hex8");
+    s->indices.assign(indices, indices + static_cast<size_t>(s->E) * 4);
+    s->poses.assign(poses, poses + static_cast<size_t>(s->E) * 9);
+    std::vector<int> used(s->N, 0);
+    for (int v : s->indices) {
+      if (v < 0 || v >= s->N) throw std::invalid_argument("tet indices out of range");
+      used[v] = 1;
+    }
+    for (int i = 0; i < s->N; ++i)
+      if (!used[i]) throw std::invalid_argument("Every particle must belong to a tetrahedron");
+    s->have_topology = true;
+    s->release();
+  })
+}
+
+int dphd_set_hex8(dphd_solver* s, const int* indices, const double* rest_vertices) {
+  DPHD_TRY({
+    if (!s || !indices || !rest_vertices || s->B == 0) throw std::invalid_argument("dphd_set_hex8: set bodies first");
+    if (s->cfg.element_kind != DPHD_HEX8) throw std::invalid_argument("solver was created for Tet4");
+    s->indices.assign(indices, indices + static_cast<size_t>(s->E) * 8);
+    s->rest.assign(rest_vertices, rest_vertices + static_cast<size_t>(s->N) * 3);
+    std::vector<int> used(s->N, 0);
+    for (int e = 0; e < s->E; ++e) {
+      for (int k = 0; k < 8; ++k) {
+        const int v = s->indices[8 * static_cast<size_t>(e) + k];
+        if (v < 0 || v >= s->N) throw std::invalid_argument("hex indices out of range");
+        for (int j = 0; j < k; ++j)
+          if (s->indices[8 * static_cast<size_t>(e) + j] == v)
+            throw std::invalid_argument("hex_mesh indices must have shape (element_count,8) with eight distinct valid vertices");
+        used[v] = 1;
+      }
+    }
+    for (int i = 0; i < s->N; ++i)
+      if (!used[i]) throw std::invalid_argument("Every particle must belong to a hexahedron");
+    s->have_topology = true;
+    s->release();
+  })
+}
+
+int dphd_set_particles(dphd_solver* s, const float* inverse_mass, const double* gravity, const double* density) {
+  DPHD_TRY({
+    if (!s || !inverse_mass || !gravity || !density || s->B == 0) throw std::invalid_argument("dphd_set_particles: invalid arguments");
+    s->inv_mass.assign(inverse_mass, inverse_mass + s->N);
+    s->gravity.assign(gravity, gravity + 3 * static_cast<size_t>(s->B));
+    s->density.assign(density, density + s->B);
+    for (double r : s->density)
+      if (!std::isfinite(r) || r <= 0.0) throw std::invalid_argument("density must be finite and positive");
+    s->have_particles = true;
+    s->release();
+  })
+}
+
+int dphd_set_materials(dphd_solver* s, const double* materials) {
+  DPHD_TRY({
+    if (!s || !materials || s->E == 0) throw std::invalid_argument("dphd_set_materials: invalid arguments");
+    s->materials.assign(materials, materials + 2 * static_cast<size_t>(s->E));
+    s->have_materials = true;
+    s->release();
+  })
+}
+
+int dphd_set_muscles(dphd_solver* s, const double* directions, const double* stiffness) {
+  DPHD_TRY({
+    if (!s || s->E == 0) throw std::invalid_argument("dphd_set_muscles: invalid arguments");
+    if (!directions || !stiffness) {
+      s->mdir_in.clear();
+      s->mstiff_in.clear();
+    } else {
+      s->mdir_in.assign(directions, directions + 3 * static_cast<size_t>(s->E));
+      s->mstiff_in.assign(stiffness, stiffness + s->E);
+      for (int e = 0; e < s->E; ++e)
+        if (!std::isfinite(s->mstiff_in[e]) || s->mstiff_in[e] < 0.0)
+          throw std::invalid_argument("Muscle directions and stiffness must be finite, with nonnegative stiffness");
+    }
+    s->release();
+  })
+}
+
+int dphd_set_tolerances(dphd_solver* s, int max_iterations, double abs_tol, double rel_tol) {
+  DPHD_TRY({
+    if (!s || max_iterations < 1 || !std::isfinite(abs_tol) || abs_tol < 0 || !std::isfinite(rel_tol) || rel_tol < 0)
+      throw std::invalid_argument("invalid tolerances");
+    s->cfg.max_iterations = max_iterations;
+    s->cfg.abs_tol = abs_tol;
+    s->cfg.rel_tol = rel_tol;
+  })
+}
+
+int dphd_build(dphd_solver* s, double dt) {
+  DPHD_TRY({
+    if (!s) throw std::invalid_argument("null solver");
+    dphd::build_solver(s, dt);
+  })
+}
+
+int dphd_get_info(const dphd_solver* s, dphd_info* info) {
+  if (!s || !info) {
+    set_error("dphd_get_info: null argument");
+    return DPHD_EINVAL;
+  }
+  *info = s->info;
+  return DPHD_OK;
+}
+
+int dphd_step(dphd_solver* s, const float* q_in, const float* qd_in, const float* f_in, const float* dirichlet,
+              const double* muscle_targets, float* q_out, float* qd_out, double* checkpoint, void* stream) {
+  if (!s || !s->built) {
+    set_error("dphd_step: call dphd_build first");
+    return DPHD_ESTATE;
+  }
+  if (!q_in || !qd_in || !f_in || !q_out || !qd_out) {
+    set_error("dphd_step: null state pointer");
+    return DPHD_EINVAL;
+  }
+  if (q_in == q_out) {
+    set_error("DiffPhD requires distinct input and output states");
+    return DPHD_EINVAL;
+  }
+  DPHD_TRY({
+    HIP_CHECK(hipSetDevice(s->device));
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    HIP_CHECK(hipEventRecord(s->ev_in, user));
+    HIP_CHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+    if (s->d.has_muscles) {
+      if (muscle_targets)
+        HIP_CHECK(hipMemcpyAsync(s->targets, muscle_targets, sizeof(double) * s->E, hipMemcpyDeviceToDevice, s->stream));
+    }
+    const dphd::StepScalars sc = dphd::scalars(s);
+    dphd::launch_begin_step(s->d, sc, q_in, qd_in, f_in, dirichlet, s->stream);
+    const int chunks = dphd::run_loop(s, false);
+    s->last_fwd_iters = chunks * dphd::chunk_size(s);
+    dphd::launch_write_state(s->d, sc, q_out, qd_out, checkpoint, s->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(s->ev_out, s->stream));
+    HIP_CHECK(hipStreamWaitEvent(user, s->ev_out, 0));
+  })
+}
+
+int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle_targets, const float* gq_out,
+                  const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
+                  float* gdirichlet, void* stream) {
+  if (!s || !s->built) {
+    set_error("dphd_backward: call dphd_build first");
+    return DPHD_ESTATE;
+  }
+  if (!checkpoint || !gq_in || !gqd_in || !gf_in) {
+    set_error("dphd_backward: null pointer");
+    return DPHD_EINVAL;
+  }
+  DPHD_TRY({
+    HIP_CHECK(hipSetDevice(s->device));
+    if (!s->d.jac) {
+      s->d.jac = dphd::dalloc<double>(s->owned, static_cast<size_t>(s->E) * s->Q * 81);
+      if (s->adj_exec) {
+        HIP_CHECK(hipGraphExecDestroy(s->adj_exec));
+        s->adj_exec = nullptr;
+      }
+    }
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    HIP_CHECK(hipEventRecord(s->ev_in, user));
+    HIP_CHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+    if (s->d.has_muscles && muscle_targets)
+      HIP_CHECK(hipMemcpyAsync(s->targets, muscle_targets, sizeof(double) * s->E, hipMemcpyDeviceToDevice, s->stream));
+    const dphd::StepScalars sc = dphd::scalars(s);
+    dphd::launch_adjoint_setup(s->d, sc, checkpoint, s->targets, gq_out, gqd_out, s->stream);
+    const int chunks = dphd::run_loop(s, true);
+    s->last_adj_iters = chunks * dphd::chunk_size(s);
+    dphd::launch_adjoint_outputs(s->d, sc, checkpoint, gqd_out, gq_in, gqd_in, gf_in, gmaterials, gdirichlet,
+                                 s->any_pinned, s->stream);
+    if (gmaterials) dphd::launch_material_finish(s->d, gmaterials, s->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(s->ev_out, s->stream));
+    HIP_CHECK(hipStreamWaitEvent(user, s->ev_out, 0));
+  })
+}
+
+int dphd_read_status(dphd_solver* s, int clear, int* bits) {
+  DPHD_TRY({
+    if (!s || !bits || !s->built) throw std::invalid_argument("dphd_read_status: invalid arguments");
+    HIP_CHECK(hipSetDevice(s->device));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    HIP_CHECK(hipMemcpy(bits, s->d.status, sizeof(int), hipMemcpyDeviceToHost));
+    if (clear) HIP_CHECK(hipMemset(s->d.status, 0, sizeof(int)));
+  })
+}
+
+int dphd_iterations(dphd_solver* s, int adjoint, int* per_body) {
+  DPHD_TRY({
+    if (!s || !per_body || !s->built) throw std::invalid_argument("dphd_iterations: invalid arguments");
+    HIP_CHECK(hipSetDevice(s->device));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    HIP_CHECK(hipMemcpy(per_body, adjoint ? s->d.aiter : s->d.iter, sizeof(int) * s->B, hipMemcpyDeviceToHost));
+  })
+}
+
+int dphd_synchronize(dphd_solver* s) {
+  DPHD_TRY({
+    if (!s) throw std::invalid_argument("null solver");
+    HIP_CHECK(hipSetDevice(s->device));
+    if (s->stream) HIP_CHECK(hipStreamSynchronize(s->stream));
+  })
+}
+
+int dphd_apply_inverse(dphd_solver* s, const double* rhs, double* out, void* stream) {
+  DPHD_TRY({
+    if (!s || !s->built || !rhs || !out) throw std::invalid_argument("dphd_apply_inverse: invalid arguments");
+    if (s->d.f32) throw std::invalid_argument("dphd_apply_inverse needs factor_precision 64");
+    HIP_CHECK(hipSetDevice(s->device));
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    dphd::launch_apply_inverse(s->d, rhs, out, user);
+    HIP_CHECK(hipGetLastError());
+  })
+}
+
+int dphd_eval_projections(int count, const double* f, const double* directions, double mu, double lame,
+                          double penalty, double fibre_target, double* sigma, double* reconstruction,
+                          double* rotation, double* volume, double* neo_hookean, double* fibre, void* stream) {
+  DPHD_TRY({
+    if (count < 0 || (count && (!f || !directions || !sigma || !reconstruction || !rotation || !volume ||
+                                !neo_hookean || !fibre)))
+      throw std::invalid_argument("dphd_eval_projections: invalid arguments");
+    dphd::launch_eval_projections(count, f, directions, mu, lame, penalty, fibre_target, sigma, reconstruction,
+                                  rotation, volume, neo_hookean, fibre, static_cast<hipStream_t>(stream));
+    HIP_CHECK(hipGetLastError());
+  })
+}
+
+int dphd_eval_jvps(int count, const double* f, const double* df, const double* directions, double mu, double lame,
+                   double fibre_target, double* rotation, double* volume, double* neo_hookean, double* fibre,
+                   void* stream) {
+  DPHD_TRY({
+    if (count < 0 || (count && (!f || !df || !directions || !rotation || !volume || !neo_hookean || !fibre)))
+      throw std::invalid_argument("dphd_eval_jvps: invalid arguments");
+    dphd::launch_eval_jvps(count, f, df, directions, mu, lame, fibre_target, rotation, volume, neo_hookean, fibre,
+                           static_cast<hipStream_t>(stream));
+    HIP_CHECK(hipGetLastError());
+  })
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+/*
+ * TEST ORACLE ONLY — plain-C FP64 restatement of the reference DiffPhD step
+ * and its backward pass, following the reference's CPU execution path.
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may
+ * load it, and only as the checker. See diffphd_oracle.c for citations.
+ */
+#ifndef DIFFPHD_ORACLE_H
+#define DIFFPHD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int n;          /* vertices */
+  int elements;   /* elements */
+  int corners;    /* 4 (Tet4) or 8 (Hex8) */
+  int samples;    /* 1 (Tet4) or 8 (Hex8) */
+  const int* indices;     /* [elements, corners] */
+  const double* grads;    /* Tet4: [elements, 4, 3]; Hex8: [8, 8, 3] shared */
+  const double* wc;       /* [elements] per-sample corotated weight */
+  const double* wv;       /* [elements] per-sample volume weight */
+  const double* wn;       /* [elements] per-sample neo-Hookean weight */
+  const double* jwc;      /* [elements] Jacobian weights (mean or element) */
+  const double* jwv;
+  const double* jwn;
+  const double* mdir;     /* [elements, 3] or NULL */
+  const double* mw;       /* [elements] per-sample fibre weight or NULL */
+  int material;           /* 0 pd, 1 neo-Hookean */
+  double nh_mu, nh_lame, nh_penalty;
+  const unsigned char* free; /* [n] */
+  double mass;            /* particle mass */
+  double sample_volume;   /* Vbar / samples: the volume each quadrature sample carries */
+  double gravity[3];
+  /* Reference incidence: entries index [elements * samples, corners]. */
+  const int* inc_off;
+  const int* inc_ent;
+  /* S and S^T (DP/sparse.py). */
+  const int* s_off;
+  const int* s_col;
+  const double* s_val;
+  const int* t_off;
+  const int* t_col;
+  const double* t_val;
+  int window;
+  int max_iterations;
+  double abs_tol, rel_tol;
+  int integrate_pinned;
+  int threads;            /* OpenMP threads, <= 0 = default */
+} oracle_problem;
+
+int oracle_forward(const oracle_problem* p, double dt, const float* q, const float* qd, const float* f,
+                   const float* dirichlet, const double* targets, double* x_out, float* q_out, float* qd_out,
+                   int* iterations, int* status);
+
+int oracle_backward(const oracle_problem* p, double dt, const double* x, const double* targets, const float* gq_out,
+                    const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
+                    float* gdirichlet, double* mu_out, int* iterations, int* status);
+
+void oracle_projections(int count, const double* f, const double* dirs, double mu, double lame, double penalty,
+                        double fibre_target, double* sigma, double* recon, double* rotation, double* volume,
+                        double* neo_hookean, double* fibre);
+
+void oracle_jvps(int count, const double* f, const double* df, const double* dirs, double mu, double lame,
+                 double fibre_target, double* rotation, double* volume, double* neo_hookean, double* fibre);
+
+void oracle_apply_inverse(const oracle_problem* p, const double* rhs, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,252 @@
+"""Write the golden vectors under tests/golden/ from the reference's own NumPy oracle.
+
+Run in a container that has the reference checkout (default /root/reference):
+
+    python tests/golden/make_golden.py [--reference /root/reference]
+
+It loads ``tests/reference_oracle.py`` of the reference (it imports only
+NumPy) and evaluates it on small meshes built by this package's mesh
+builders; the NPZ files hold inputs and outputs only. Cases follow the
+reference's own tests:
+
+* ``tet_fwd_*``   ``ReferenceStep.step`` on a 3x2x2-cell Kuhn grid with the
+  deformed start state of ``tests/test_diff_phd_solver.py:109-120``:
+  fixed 12 iterations (``:241-273``), converged heterogeneous contrast
+  (``:275-279``), fibres (``:281-307``);
+* ``tet_bwd_*``   ``reference_backward`` (mean-weight K, FD projection
+  Jacobians) at the oracle's converged iterate (``tests/test_diff_phd_adjoint.py:138-153``);
+* ``hex_fwd_*``   ``ReferenceStep.step`` with an isoparametric 2x2x2-Gauss
+  Hex8 integration written here from ``HexOracle``
+  (``tests/test_diff_phd_hex.py:39-109``) on its two-cube fixture (``:25-36``);
+* ``projections`` the oracle projections on the F set of
+  ``tests/test_diff_phd_solver.py:124-161``.
+"""
+
+import argparse
+import importlib.util
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT / "newton-solver-diff-phd_amd"))
+
+from diffphd_amd import meshes  # noqa: E402
+
+DENSITY = 1000.0
+
+
+def load_reference_oracle(reference):
+    path = Path(reference) / "tests" / "reference_oracle.py"
+    spec = importlib.util.spec_from_file_location("reference_oracle", path)
+    module = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(module)
+    return module
+
+
+def tet_scene(contrast=1.0):
+    """3x2x2 cells of 0.1 m at (0, 0, 1), x = 0 face pinned, mu 2e4, lambda 5e4."""
+    vertices, tets, poses = meshes.kuhn_tet_grid(3, 2, 2, 0.1, origin=(0.0, 0.0, 1.0))
+    vertices = vertices.astype(np.float32).astype(np.float64)
+    free = vertices[:, 0] > vertices[:, 0].min()
+    materials = np.tile([2.0e4, 5.0e4, 0.0], (len(tets), 1)).astype(np.float32)
+    if contrast != 1.0:
+        stiff = vertices[tets].mean(axis=1)[:, 0] > vertices[:, 0].mean()
+        materials[stiff, :2] *= contrast
+    return vertices, tets, poses, materials, free
+
+
+def deformed_state(vertices, free, seed=3, stretch=(1.3, 0.8, 1.1), speed=0.5):
+    rng = np.random.default_rng(seed)
+    center = vertices.mean(axis=0)
+    q = np.where(free[:, None], center + (vertices - center) * np.asarray(stretch), vertices)
+    q[free] += 0.01 * rng.standard_normal((free.sum(), 3))
+    v = np.where(free[:, None], speed * rng.standard_normal(vertices.shape), 0.0)
+    return q.astype(np.float32), v.astype(np.float32)
+
+
+def make_hex_oracle(ro):
+    class HexQuadrature(ro.ReferenceStep):
+        """Isoparametric 2x2x2 Gauss integration of binary-order Hex8 cells."""
+
+        def __init__(self, vertices, cells, materials, free, density, material="pd"):
+            self.elements = np.asarray(cells)
+            self.tets = self.elements
+            self.free = np.asarray(free, dtype=bool)
+            self.n = len(vertices)
+            self.mu, self.lame = np.asarray(materials, dtype=np.float64)[:, :2].T
+            self.material = material
+            self.nh = (self.mu.mean(), self.lame.mean(), 2 * self.mu.mean())
+            self.muscle_directions = None
+            points, _weights = np.polynomial.legendre.leggauss(2)
+            signs = np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], dtype=np.float64)
+            self.quadrature = []
+            volumes = []
+            for nodes in self.elements:
+                samples = []
+                for xi in np.array([[a, b, c] for a in points for b in points for c in points]):
+                    dn = np.empty((8, 3))
+                    for axis in range(3):
+                        rest = [b for b in range(3) if b != axis]
+                        dn[:, axis] = signs[:, axis] * np.prod(1 + signs[:, rest] * xi[rest], axis=1) / 8
+                    jac = vertices[nodes].T @ dn
+                    samples.append((dn @ np.linalg.inv(jac), np.linalg.det(jac)))
+                self.quadrature.append(samples)
+                volumes.append(sum(w for _g, w in samples))
+            self.vbar = float(np.mean(volumes))
+            self.mass = density * self.vbar
+
+        def matrix(self, dt):
+            a = np.diag(np.where(self.free, self.mass / dt**2, 1.0))
+            for e, nodes in enumerate(self.elements):
+                k = 2 * self.mu[e] + (self.lame[e] if self.material == "pd" else 0.0)
+                for g, w in self.quadrature[e]:
+                    block = k * w * (g @ g.T)
+                    for i_, i in enumerate(nodes):
+                        for j_, j in enumerate(nodes):
+                            if self.free[i] and self.free[j]:
+                                a[i, j] += block[i_, j_]
+            return a
+
+        def projection_rhs(self, x, boundary, targets=None):
+            out = np.zeros_like(x)
+            for e, nodes in enumerate(self.elements):
+                for g, w in self.quadrature[e]:
+                    f = x[nodes].T @ g
+                    fd = boundary[nodes].T @ g
+                    if self.material == "pd":
+                        s = 2 * self.mu[e] * (ro.corotated_projection(f) - fd)
+                        s = s + self.lame[e] * (ro.volume_projection(f) - fd)
+                    else:
+                        s = 2 * self.mu[e] * (ro.nh_projection(f, *self.nh) - fd)
+                    out[nodes] += w * (s @ g.T).T
+            return out
+
+    return HexQuadrature
+
+
+def main():
+    parser = argparse.ArgumentParser()
+    parser.add_argument("--reference", default="/root/reference")
+    args = parser.parse_args()
+    ro = load_reference_oracle(args.reference)
+    dt = 0.004
+    gravity = np.array([0.0, 0.0, -9.81])
+
+    # Forward, fixed iteration count and converged contrast.
+    cases = [("fixed", 1.0, {"max_iterations": 12, "abs_tol": 0.0, "rel_tol": 0.0})]
+    for c in (10.0, 100.0):
+        cases.append((f"conv_c{int(c)}", c, {"max_iterations": 500, "abs_tol": 1.0e-11, "rel_tol": 1.0e-10}))
+    for material in ("pd", "neo_hookean"):
+        for name, contrast, opts in cases:
+            vertices, tets, poses, materials, free = tet_scene(contrast)
+            q, v = deformed_state(vertices, free)
+            f = np.zeros_like(q)
+            step = ro.ReferenceStep(
+                tets, poses, materials[:, 0].astype(np.float64), materials[:, 1].astype(np.float64), DENSITY, free, material
+            )
+            x, iterations, converged = step.step(
+                q.astype(np.float64), v.astype(np.float64), f.astype(np.float64), gravity, dt, **opts
+            )
+            np.savez_compressed(
+                HERE / f"tet_fwd_{name}_{'pd' if material == 'pd' else 'nh'}.npz",
+                tets=tets, poses=poses, materials=materials, free=free, q=q, qd=v, f=f, gravity=gravity,
+                dt=dt, density=DENSITY, max_iterations=opts["max_iterations"], abs_tol=opts["abs_tol"],
+                rel_tol=opts["rel_tol"], x=x, iterations=iterations, converged=converged,
+            )
+
+    # Fibres with a 0.25 contraction on every other element.
+    vertices, tets, poses, materials, free = tet_scene()
+    rng = np.random.default_rng(5)
+    directions = rng.standard_normal((len(tets), 3))
+    stiffness = np.where(np.arange(len(tets)) % 2 == 0, 4.0e4, 0.0)
+    contraction = np.where(stiffness > 0, 0.25, 0.0).astype(np.float32)
+    q, v = deformed_state(vertices, free, speed=0.0)
+    f = np.zeros_like(q)
+    opts = {"max_iterations": 500, "abs_tol": 1.0e-11, "rel_tol": 1.0e-10}
+    step = ro.ReferenceStep(
+        tets, poses, materials[:, 0].astype(np.float64), materials[:, 1].astype(np.float64), DENSITY, free, "pd",
+        muscle_directions=directions, muscle_stiffness=stiffness,
+    )
+    targets = 1.0 - contraction.astype(np.float64)
+    x, iterations, converged = step.step(
+        q.astype(np.float64), v.astype(np.float64), f.astype(np.float64), gravity, dt, targets=targets, **opts
+    )
+    np.savez_compressed(
+        HERE / "tet_fwd_muscle_pd.npz", tets=tets, poses=poses, materials=materials, free=free, q=q, qd=v, f=f,
+        gravity=gravity, dt=dt, density=DENSITY, directions=directions, stiffness=stiffness, targets=targets,
+        x=x, iterations=iterations, converged=converged, **opts,
+    )
+
+    # Backward: reference mean-weight K at a converged iterate.
+    tight = {"max_iterations": 3000, "abs_tol": 1.0e-12, "rel_tol": 1.0e-12}
+    for name, contrast, material, seed_v in (("pd_c2", 2.0, "pd", False), ("pd_qd", 1.0, "pd", True), ("nh", 1.0, "neo_hookean", True)):
+        vertices, tets, poses, materials, free = tet_scene(contrast)
+        q, v = deformed_state(vertices, free, stretch=(1.2, 0.85, 1.05), speed=0.3)
+        rng = np.random.default_rng(21)
+        f = rng.uniform(-1.0, 1.0, q.shape).astype(np.float32)
+        step = ro.ReferenceStep(
+            tets, poses, materials[:, 0].astype(np.float64), materials[:, 1].astype(np.float64), DENSITY, free, material
+        )
+        x, iterations, converged = step.step(
+            q.astype(np.float64), v.astype(np.float64), f.astype(np.float64), gravity, dt, **tight
+        )
+        gq = np.random.default_rng(49).standard_normal(q.shape).astype(np.float32)
+        gqd = (0.01 * np.random.default_rng(50).standard_normal(q.shape)).astype(np.float32) if seed_v else np.zeros_like(gq)
+        out = ro.reference_backward(step, x, dt, gq.astype(np.float64), gqd.astype(np.float64))
+        np.savez_compressed(
+            HERE / f"tet_bwd_{name}.npz", tets=tets, poses=poses, materials=materials, free=free, q=q, qd=v, f=f,
+            gravity=gravity, dt=dt, density=DENSITY, material=material, x=x, iterations=iterations,
+            gq=gq, gqd=gqd, **{f"out_{k}": np.asarray(val) for k, val in out.items()}, **tight,
+        )
+
+    # Hex8: two cubes, four pinned vertices, heterogeneous materials, no Anderson.
+    HexQuadrature = make_hex_oracle(ro)
+    vertices = np.array([[x, y, z] for x in range(3) for y in range(2) for z in range(2)], dtype=np.float64) * 0.1
+    cells = np.array([np.arange(8), np.arange(4, 12)], dtype=np.int32)
+    free = np.arange(12) >= 4
+    materials = np.array([[2000.0, 4000.0], [20000.0, 40000.0]])
+    for material in ("pd", "neo_hookean"):
+        q = vertices.copy()
+        q[4:, 0] *= 1.04
+        q[4:, 2] += 0.01 * q[4:, 0]
+        v = np.random.default_rng(4).normal(0, 0.03, q.shape)
+        v[:4] = 0
+        f = np.random.default_rng(5).normal(0, 0.2, q.shape)
+        q32, v32, f32 = q.astype(np.float32), v.astype(np.float32), f.astype(np.float32)
+        oracle = HexQuadrature(vertices, cells, materials, free, DENSITY, material)
+        opts = {"max_iterations": 1000, "abs_tol": 1e-12, "rel_tol": 1e-10, "window": 0}
+        x, iterations, converged = oracle.step(
+            q32.astype(np.float64), v32.astype(np.float64), f32.astype(np.float64), np.zeros(3), 0.003, **opts
+        )
+        inverse = np.linalg.inv(oracle.matrix(0.003))
+        np.savez_compressed(
+            HERE / f"hex_fwd_{'pd' if material == 'pd' else 'nh'}.npz", vertices=vertices, cells=cells,
+            materials=materials, free=free, q=q32, qd=v32, f=f32, dt=0.003, density=DENSITY, x=x,
+            iterations=iterations, converged=converged, inverse=inverse, mass=oracle.mass, **opts,
+        )
+
+    # Pointwise projections.
+    rng = np.random.default_rng(7)
+    fs = [np.eye(3), np.diag([1.4, 0.7, 1.1]), np.diag([1.0, 1.0, -0.5]), 1.2 * np.eye(3)]
+    fs += [np.eye(3) + 0.4 * rng.standard_normal((3, 3)) for _ in range(12)]
+    fs += [rng.standard_normal((3, 3)) for _ in range(8)]
+    fs = np.array([m if abs(np.linalg.det(m)) > 1.0e-2 else m + np.eye(3) for m in fs])
+    dirs = rng.standard_normal((len(fs), 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    mu, lame = 3.0e4, 8.0e4
+    np.savez_compressed(
+        HERE / "projections.npz", f=fs, directions=dirs, mu=mu, lame=lame,
+        sigma=np.array([np.linalg.svd(m, compute_uv=False) for m in fs]),
+        rotation=np.array([ro.corotated_projection(m) for m in fs]),
+        volume=np.array([ro.volume_projection(m) for m in fs]),
+        neo_hookean=np.array([ro.nh_projection(m, mu, lame, 2.0 * mu) for m in fs]),
+        fibre=np.array([ro.muscle_projection(m, d, 0.8) for m, d in zip(fs, dirs)]),
+    )
+    print("golden vectors written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
