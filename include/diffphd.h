@@ -122,6 +122,8 @@ int dphd_set_tolerances(dphd_solver* s, int max_iterations, double abs_tol, doub
 /* Element weights, assembly and factor per body (DP/solver.py:491-552). */
 int dphd_build(dphd_solver* s, double dt);
 int dphd_get_info(const dphd_solver* s, dphd_info* info);
+/* nnz(S) of every body after dphd_build (per_body [B]); sizes the byte model. */
+int dphd_body_nnz(const dphd_solver* s, int64_t* per_body);
 
 /* One forward step of all bodies (DP/solver.py:689-786). q/qd/f are [N,3]
  * float32 device arrays; dirichlet is [N,3] float32 or NULL

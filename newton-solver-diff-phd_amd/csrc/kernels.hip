@@ -261,16 +261,18 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d) {
 }
 
 // ---------------------------------------------------------------------------
-// CSR products y = M x for every active body's rows.
+// CSR products y = M x for every active body's rows. PRED = false is the
+// plain product of every row (SparseInverse.apply), PRED = true skips the
+// rows of bodies whose fixed point has converged.
 
-template <typename Real>
+template <typename Real, bool PRED>
 __global__ __launch_bounds__(kBlock) void k_spmv(const SpmvTask* tasks, const int* rows, const int* off,
                                                   const int* col, const Real* val, const double* in, double* out,
                                                   const double* base, const int* vbody, const int* active) {
   const SpmvTask t = tasks[blockIdx.x];
   if (t.lanes == kBlock) {
     const int row = rows[t.first];
-    if (!active[vbody[row]]) return;
+    if (PRED && !active[vbody[row]]) return;
     __shared__ double smem[3 * kBlock / kWave];
     __shared__ double res[3];
     double acc[3] = {0.0, 0.0, 0.0};
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const SpmvTask* tasks, const in
   int row = -1;
   if (group < t.count) {
     row = rows[t.first + group];
-    if (!active[vbody[row]]) row = -1;
+    if (PRED && !active[vbody[row]]) row = -1;
   }
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   if (row >= 0) {
@@ -1040,23 +1042,23 @@ __global__ void k_eval_jvps(int count, const double* fs, const double* dfs, cons
   st3(fibre, i, muscle_jvp(f, ld3(dirs, i), target, df));
 }
 
-template <typename Real>
+template <typename Real, bool PRED>
 void spmv_pair(const DeviceState& d, const double* in, double* mid, double* out, const double* base,
                const int* active, hipStream_t stream) {
   if (d.s_ntasks)
-    hipLaunchKernelGGL(k_spmv<Real>, dim3(d.s_ntasks), dim3(kBlock), 0, stream, d.s_tasks, d.s_rows, d.s_off,
-                       d.s_col, static_cast<const Real*>(d.s_val), in, mid, nullptr, d.vbody, active);
+    hipLaunchKernelGGL((k_spmv<Real, PRED>), dim3(d.s_ntasks), dim3(kBlock), 0, stream, d.s_tasks, d.s_rows,
+                       d.s_off, d.s_col, static_cast<const Real*>(d.s_val), in, mid, nullptr, d.vbody, active);
   if (d.t_ntasks)
-    hipLaunchKernelGGL(k_spmv<Real>, dim3(d.t_ntasks), dim3(kBlock), 0, stream, d.t_tasks, d.t_rows, d.t_off,
-                       d.t_col, static_cast<const Real*>(d.t_val), mid, out, base, d.vbody, active);
+    hipLaunchKernelGGL((k_spmv<Real, PRED>), dim3(d.t_ntasks), dim3(kBlock), 0, stream, d.t_tasks, d.t_rows,
+                       d.t_off, d.t_col, static_cast<const Real*>(d.t_val), mid, out, base, d.vbody, active);
 }
 
 void spmv_pair_any(const DeviceState& d, const double* in, double* mid, double* out, const double* base,
                    const int* active, hipStream_t stream) {
   if (d.f32)
-    spmv_pair<float>(d, in, mid, out, base, active, stream);
+    spmv_pair<float, true>(d, in, mid, out, base, active, stream);
   else
-    spmv_pair<double>(d, in, mid, out, base, active, stream);
+    spmv_pair<double, true>(d, in, mid, out, base, active, stream);
 }
 
 }  // namespace
@@ -1149,7 +1151,10 @@ void launch_material_finish(const DeviceState& d, double* gmat, hipStream_t stre
 }
 
 void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, hipStream_t stream) {
-  spmv_pair_any(d, rhs, d.y, out, nullptr, d.ones, stream);
+  if (d.f32)
+    spmv_pair<float, false>(d, rhs, d.y, out, nullptr, d.ones, stream);
+  else
+    spmv_pair<double, false>(d, rhs, d.y, out, nullptr, d.ones, stream);
 }
 
 void launch_eval_projections(int count, const double* f, const double* dirs, double mu, double lame, double penalty,

@@ -107,6 +107,7 @@ struct dphd_solver {
   bool built = false;
   double dt = 0.0;
   dphd_info info{};
+  std::vector<int64_t> body_nnz;
   bool any_pinned = false;
   std::vector<void*> owned;
   dphd::DeviceState d;
@@ -304,7 +305,9 @@ void build_solver(dphd_solver* s, double dt) {
     if (!errors[b].empty()) throw std::invalid_argument(errors[b]);
   // Global CSR (rows/columns offset by the body's first vertex).
   int64_t znnz = 0, annz = 0;
+  s->body_nnz.assign(B, 0);
   for (int b = 0; b < B; ++b) {
+    s->body_nnz[b] = static_cast<int64_t>(factors[b].s.col.size());
     znnz += static_cast<int64_t>(factors[b].s.col.size());
     annz += static_cast<int64_t>(amats[b].col.size());
   }
@@ -748,6 +751,15 @@ int dphd_get_info(const dphd_solver* s, dphd_info* info) {
   return DPHD_OK;
 }
 
+int dphd_body_nnz(const dphd_solver* s, int64_t* per_body) {
+  if (!s || !per_body || !s->built) {
+    set_error("dphd_body_nnz: build the solver first");
+    return DPHD_ESTATE;
+  }
+  std::copy(s->body_nnz.begin(), s->body_nnz.end(), per_body);
+  return DPHD_OK;
+}
+
 int dphd_step(dphd_solver* s, const float* q_in, const float* qd_in, const float* f_in, const float* dirichlet,
               const double* muscle_targets, float* q_out, float* qd_out, double* checkpoint, void* stream) {
   if (!s || !s->built) {
@@ -850,7 +862,6 @@ int dphd_synchronize(dphd_solver* s) {
 int dphd_apply_inverse(dphd_solver* s, const double* rhs, double* out, void* stream) {
   DPHD_TRY({
     if (!s || !s->built || !rhs || !out) throw std::invalid_argument("dphd_apply_inverse: invalid arguments");
-    if (s->d.f32) throw std::invalid_argument("dphd_apply_inverse needs factor_precision 64");
     HIP_CHECK(hipSetDevice(s->device));
     hipStream_t user = static_cast<hipStream_t>(stream);
     dphd::launch_apply_inverse(s->d, rhs, out, user);

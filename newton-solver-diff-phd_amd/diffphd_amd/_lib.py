@@ -76,6 +76,7 @@ SIGNATURES = {
     "dphd_set_tolerances": ([_P, _I, _D, _D], _I),
     "dphd_build": ([_P, _D], _I),
     "dphd_get_info": ([_P, ctypes.POINTER(Info)], _I),
+    "dphd_body_nnz": ([_P, _P], _I),
     "dphd_step": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "dphd_backward": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "dphd_read_status": ([_P, _I, ctypes.POINTER(_I)], _I),

@@ -191,6 +191,8 @@ class SolverDiffPhD:
         self._version = 0
         self._targets = None
         self.last_setup_seconds = 0.0
+        # When a list, every native forward/backward appends its per-body iteration counts.
+        self.iteration_log = None
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -257,8 +259,11 @@ class SolverDiffPhD:
             if np.any(w != w[0]):
                 raise ValueError("every vertex of a body must belong to the same world")
             gravity[b] = gravity_table[max(int(w[0]), 0)]
-        check(lib().dphd_set_particles(self._h, ptr(inv_mass), ptr(np.ascontiguousarray(gravity)), ptr(self.density)), "particles")
-        check(lib().dphd_set_materials(self._h, ptr(self._materials_host())), "materials")
+        # Every array handed to C is bound to a name for the duration of the call.
+        gravity = np.ascontiguousarray(gravity)
+        check(lib().dphd_set_particles(self._h, ptr(inv_mass), ptr(gravity), ptr(self.density)), "particles")
+        materials = self._materials_host()
+        check(lib().dphd_set_materials(self._h, ptr(materials)), "materials")
         if self._muscles is not None:
             check(lib().dphd_set_muscles(self._h, ptr(self._muscles[0]), ptr(self._muscles[1])), "muscles")
         check(lib().dphd_build(self._h, float(dt)), "build")
@@ -351,6 +356,12 @@ class SolverDiffPhD:
         """Build weights and the inverse factor for ``dt`` now instead of at the first step."""
         self._build(float(dt))
 
+    def body_factor_nnz(self) -> np.ndarray:
+        """nnz(S) of every body (S^T has the same)."""
+        out = np.zeros(self.body_count_solved, np.int64)
+        check(lib().dphd_body_nnz(self._h, ptr(out)), "body_nnz")
+        return out
+
     def solution(self) -> torch.Tensor:
         """FP64 converged iterate of the last untracked step (the reference's ``solver._x``)."""
         return self._x
@@ -365,6 +376,8 @@ class SolverDiffPhD:
             ),
             "step",
         )
+        if self.iteration_log is not None:
+            self.iteration_log.append(("forward", self.iterations.numpy()))
 
     def _native_backward(self, checkpoint, targets, gq_out, gqd_out, gq, gqd, gf, gmat, gdir):
         check(
@@ -374,6 +387,8 @@ class SolverDiffPhD:
             ),
             "backward",
         )
+        if self.iteration_log is not None:
+            self.iteration_log.append(("adjoint", self.adjoint_iterations.numpy()))
 
     # -- diagnostics ---------------------------------------------------------------
     @property

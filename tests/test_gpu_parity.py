@@ -101,7 +101,10 @@ def test_jvps_match_oracle_and_finite_differences(cuda):
     outs = [torch.empty((c, 3, 3), dtype=torch.float64, device=cuda) for _ in range(3)] + [
         torch.empty((c, 3), dtype=torch.float64, device=cuda)
     ]
-    L.check(L.lib().dphd_eval_jvps(c, L.ptr(T(f)), L.ptr(T(df)), L.ptr(T(d)), mu, lame, 0.8, *[L.ptr(o) for o in outs], None))
+    # Keep the inputs referenced: a temporary would return its block to the
+    # caching allocator before the kernel reads it.
+    ft, dft, dt_ = T(f), T(df), T(d)
+    L.check(L.lib().dphd_eval_jvps(c, L.ptr(ft), L.ptr(dft), L.ptr(dt_), mu, lame, 0.8, *[L.ptr(o) for o in outs], None))
     torch.cuda.synchronize()
     got = [o.cpu().numpy() for o in outs]
     ref = O.jvps(f, df, d, mu, lame, 0.8)
@@ -262,10 +265,17 @@ def test_cube_rollout_matches_oracle(cuda, precision):
     q, qd = vertices.astype(np.float32), np.zeros_like(vertices, dtype=np.float32)
     for t in range(steps):
         out = body.forward(dt, q, qd)
-        scale = np.abs(out["x"] - vertices).max()
-        tol = 1e-9 if precision == 64 else 1e-5
-        np.testing.assert_allclose(xs[t], out["x"], rtol=0, atol=tol * max(scale, 1e-12) + 1e-12, err_msg=f"step {t}")
-        assert abs(int(iters[t][0]) - out["iterations"]) <= 1, (t, iters[t], out["iterations"])
+        if precision == 64:
+            # Same iteration path as the oracle: agreement far below the gate.
+            atol = 1e-9 * np.abs(out["x"] - vertices).max() + 1e-12
+            slack = 1
+        else:
+            # FP32 factor: positions within 1e-5 relative (the gate itself is
+            # relative to |x|, so iterates may stop ~rel_tol*|x| apart).
+            atol = 1e-5 * np.abs(out["x"]).max()
+            slack = 2
+        np.testing.assert_allclose(xs[t], out["x"], rtol=0, atol=atol, err_msg=f"step {t}")
+        assert abs(int(iters[t][0]) - out["iterations"]) <= slack, (t, iters[t], out["iterations"])
         q, qd = out["q"], out["qd"]
 
 
