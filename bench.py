@@ -197,7 +197,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - start
     status = solver.read_status()
-    log(f"[bench] batched rollout {elapsed:.3f}s for {args.steps} fwd+adj steps, loss {float(loss):.6e}, status {status}")
+    log(f"[bench] batched rollout {elapsed:.3f}s for {args.steps} fwd+adj steps, loss {float(loss.detach()):.6e}, status {status}")
     fwd_iters = np.array([it for kind, it in solver.iteration_log if kind == "forward"])
     adj_iters = np.array([it for kind, it in solver.iteration_log if kind == "adjoint"])
     solver.iteration_log = None
@@ -231,6 +231,23 @@ def main():
     launch_bytes = z_total * (s_bytes + 4) + n_total * (4 + 4 + 24 + 24) + 4
     probe_gbps = launch_bytes / per_launch_s / 1e9
     log(f"[bench] spmv probe {per_launch_s * 1e6:.1f} us/launch, {probe_gbps:.0f} GB/s")
+    del rhs
+
+    # Copy-kernel ceiling: a plain 256 MiB device copy (also the reference
+    # kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters).
+    src = torch.empty(2**25, dtype=torch.float64, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(10):
+        dst.copy_(src)
+    ev1.record()
+    torch.cuda.synchronize()
+    copy_gbps = 2 * src.numel() * 8 * 10 / (ev0.elapsed_time(ev1) / 1e3) / 1e9
+    del src, dst
+    log(f"[bench] copy ceiling {copy_gbps:.0f} GB/s")
 
     # Plain per-body path: one single-body solver per body, run one after another.
     plain = None
@@ -262,6 +279,13 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(bodies, z_b, dt, args.cpu_sample_bodies, solver)
+
+    # HBM traffic per launch of the probe kernel from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs (calibrated against a plain copy kernel).
+    traffic = None
+    traffic_file = ROOT / "profiles" / "traffic_k_spmv_batched256.json"
+    if traffic_file.exists() and args.precision == 64 and len(bodies) == 256 and (args.kmin, args.kmax) == (4, 16):
+        traffic = json.loads(traffic_file.read_text())["traffic_bytes_per_launch"]
 
     value = args.steps / elapsed
     result = {
@@ -305,9 +329,11 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": probe_gbps / HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": traffic,
             "launch_us": per_launch_s * 1e6,
             "bytes_per_launch": launch_bytes,
+            "copy_ceiling_gbps": copy_gbps,
+            "frac_of_copy": probe_gbps / copy_gbps,
         },
         "cpu_baseline": cpu,
     }
