@@ -10,6 +10,10 @@
 //   chunks     256-vertex / 256-thread element chunks never straddle bodies;
 //              per-chunk partial sums are reduced per body in chunk order, so
 //              every reduction is deterministic and float-atomic free.
+//   work lists every iteration kernel runs a fixed grid whose workgroups loop
+//              over the chunks / SpMV tasks of the bodies that are still
+//              iterating (an LDS prefix over the per-body active flags), so a
+//              converged body costs nothing in later iterations.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,6 +32,7 @@ struct SpmvTask {
   int first;
   int count;
   int lanes;
+  int base;  // first vertex of the task's body (16-bit columns are body-local)
 };
 
 struct Chunk {
@@ -68,7 +73,9 @@ struct DeviceState {
   SpmvTask* s_tasks = nullptr;
   SpmvTask* t_tasks = nullptr;
   int s_ntasks = 0, t_ntasks = 0;
-  bool f32 = false;
+  bool f32 = false;                         // FP32 S values
+  uint16_t *s_col16 = nullptr, *t_col16 = nullptr;  // body-local columns (FP32 mode, bodies <= 65536 vertices)
+  bool col16 = false;
   // constant matrix for the defect correction (per coordinate)
   int *a_off = nullptr, *a_col = nullptr;
   double* a_val = nullptr;
@@ -76,21 +83,27 @@ struct DeviceState {
   double *q0 = nullptr, *qt = nullptr, *xd = nullptr, *x = nullptr, *xr = nullptr, *p = nullptr, *rhs = nullptr,
          *y = nullptr;
   double* contrib = nullptr;  // [E*K*3]
+  float *rhs32 = nullptr, *y32 = nullptr;  // FP32 mode: residual and S-product vectors [N*3]
   // Anderson
   int W = 0;
   double *hg = nullptr, *hr = nullptr;  // [(W+1)*N*3]
-  int* aa = nullptr;                    // [B*4] count, start, saved count, saved start
-  int* accept = nullptr;                // [B]
-  double* gamma = nullptr;              // [B*8]
+  int* aa = nullptr;                    // [2*B*2] (iteration parity, body): count, start
   // chunks
   Chunk* vchunks = nullptr;
   int nvc = 0;
-  int* vcb = nullptr;  // [B+1] body chunk ranges
+  int* vcb = nullptr;  // [B+1] vertex-chunk range of each body
   Chunk* echunks = nullptr;
   int nec = 0;
+  int* ecb = nullptr;  // [B+1] element-chunk range of each body
+  int* stb = nullptr;  // [B+1] S task range of each body
+  int* ttb = nullptr;  // [B+1] S^T task range of each body
+  // Fixed grid sizes of the work-list kernels (each workgroup loops over
+  // the items of the still-active bodies).
+  int g_vert = 0, g_elem = 0, g_stask = 0, g_ttask = 0;
   double* gram_part = nullptr;  // [nvc*kGramEntries]
   double* conv_part = nullptr;  // [nvc*4]
   double* clip_part = nullptr;  // [nvc]
+  double* clip_part2 = nullptr; // [nvc]
   // iteration control
   int* iter = nullptr;      // [B]
   int* active = nullptr;    // [B]

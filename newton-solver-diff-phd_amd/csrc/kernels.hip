@@ -1,22 +1,27 @@
 // gfx950 kernels of the DiffPhD projective-dynamics step and its adjoint.
 //
 // Reference kernels (paths under src/newton_solver_diff_phd/_src/solvers/diff_phd/):
-//   k_begin_step        kernels.py:155-207 (begin_step, explicit_boundary)
-//   k_local_tet4        kernels.py:222-272 (local_step)
-//   k_local_hex8        hex.py:122-169 (one lane per Gauss sample, 8 lanes
+//   begin_step_chunk    kernels.py:155-207 (begin_step, explicit_boundary)
+//   local_tet4_chunk    kernels.py:222-272 (local_step)
+//   local_hex8_chunk    hex.py:122-169 (one lane per Gauss sample, 8 lanes
 //                       per cell summed by a lane butterfly)
-//   k_gather            kernels.py:75-91 + global_rhs :275-284 + the
+//   gather_chunk        kernels.py:75-91 + global_rhs :275-284 + the
 //                       projection half of the convergence gate :354-401
-//   k_spmv              sparse.py:209-227 (CSR-vector, rows binned by length)
-//   k_push_gram         anderson.py:46-94 (push + Gram partials) with
+//   spmv_task           sparse.py:209-227 (CSR-vector, rows binned by length)
+//   push_gram_chunk     anderson.py:46-94 (push + Gram partials) with
 //                       impose_boundary kernels.py:287-291
-//   k_gram_solve        anderson.py:150-215 (reduce + pivoted LDL^T solve)
-//   k_apply_conv        anderson.py:218-243 + convergence_tiled :354-401
+//   gram_solve          anderson.py:150-215 (reduce + pivoted LDL^T solve)
+//   apply_conv_chunk    anderson.py:218-243 + convergence_tiled :354-401
 //   k_conv_check        kernels.py:322-347
 //   k_write_state       kernels.py:404-414
 //   adjoint kernels     adjoint.py:70-321, 474-659
-// Every reduction is a fixed-shape butterfly / chunk-ordered sum: results
-// are bitwise repeatable run to run. No float atomics.
+//
+// Iteration kernels take a work list: a fixed grid whose workgroups loop over
+// the items (vertex chunks, element chunks or SpMV tasks) of the bodies that
+// are still iterating, found through a prefix sum of the per-body active
+// flags built in LDS at kernel start. Every reduction is a fixed-shape
+// butterfly or a chunk-ordered sum: results are bitwise repeatable run to
+// run, and no float atomics are used.
 #include "kernels.h"
 #include "pd_math.h"
 
@@ -24,6 +29,7 @@ namespace dphd {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kRhsIndex = kMaxWindow * (kMaxWindow + 1) / 2;  // first right-hand-side entry of the Gram partials
 
 __device__ __forceinline__ V3 ld3(const double* p, int i) { return V3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 __device__ __forceinline__ void st3(double* p, int i, V3 v) {
@@ -31,14 +37,15 @@ __device__ __forceinline__ void st3(double* p, int i, V3 v) {
   p[3 * i + 1] = v.y;
   p[3 * i + 2] = v.z;
 }
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
   return v;
 }
 
-// Sum NV values over the 256-thread block. Result in out_smem[0..NV) after
-// the final barrier (every thread may read it).
+// Sum NV values over the 256-thread block into result[0..NV) (LDS), visible
+// to every thread after the final barrier.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* smem, double* result) {
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -56,7 +63,72 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* smem, double*
   __syncthreads();
 }
 
-__device__ __forceinline__ M3 tet_grads(const float* poses, int e, V3 (&g)[4]) {
+// ---------------------------------------------------------------------------
+// Work lists.
+
+// pref[b] = number of items of active bodies before body b; returns the
+// total. `base` holds each body's item range. Whole block participates.
+__device__ int active_prefix(const int* active, const int* base, int B, int* pref) {
+  __shared__ int scan[kBlock];
+  const int per = (B + kBlock - 1) / kBlock;
+  const int b0 = min(B, static_cast<int>(threadIdx.x) * per), b1 = min(B, b0 + per);
+  int local = 0;
+  for (int b = b0; b < b1; ++b) local += active[b] ? base[b + 1] - base[b] : 0;
+  scan[threadIdx.x] = local;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    const int t = threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = scan[threadIdx.x] - local;
+  for (int b = b0; b < b1; ++b) {
+    pref[b] = run;
+    run += active[b] ? base[b + 1] - base[b] : 0;
+  }
+  if (threadIdx.x == kBlock - 1) pref[B] = scan[kBlock - 1];
+  __syncthreads();
+  return pref[B];
+}
+
+// Body whose active item range holds v: pref[b] <= v < pref[b + 1].
+__device__ __forceinline__ int find_body(const int* pref, int B, int v) {
+  int lo = 0, hi = B;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pref[mid] <= v)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Loop `fn(item, body)` over the active items with a static grid stride; the
+// whole block iterates together, so fn may use block barriers.
+template <typename Fn>
+__device__ __forceinline__ void for_active(const int* active, const int* base, int B, Fn fn) {
+  extern __shared__ int pref[];
+  const int total = active_prefix(active, base, B, pref);
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int b = find_body(pref, B, v);
+    fn(base[b] + (v - pref[b]), b);
+  }
+}
+
+// Right-hand side of the S product: FP64, or FP32 in the FP32-factor mode.
+__device__ __forceinline__ void store_rhs(const DeviceState& d, int i, V3 v) {
+  if (d.rhs32) {
+    d.rhs32[3 * i] = static_cast<float>(v.x);
+    d.rhs32[3 * i + 1] = static_cast<float>(v.y);
+    d.rhs32[3 * i + 2] = static_cast<float>(v.z);
+  } else {
+    st3(d.rhs, i, v);
+  }
+}
+
+__device__ __forceinline__ void tet_grads(const float* poses, int e, V3 (&g)[4]) {
   const float* p = poses + 9 * e;
   double r[9];
 #pragma unroll
@@ -65,7 +137,11 @@ __device__ __forceinline__ M3 tet_grads(const float* poses, int e, V3 (&g)[4]) {
   g[1] = V3{r[0], r[1], r[2]};
   g[2] = V3{r[3], r[4], r[5]};
   g[3] = V3{r[6], r[7], r[8]};
-  return M3{};
+}
+
+__device__ __forceinline__ V3 hex_grad(const DeviceState& d, int b, int s, int j) {
+  const double* G = d.hexg + 192 * b + 24 * s + 3 * j;
+  return V3{G[0], G[1], G[2]};
 }
 
 __device__ __forceinline__ void add_outer(M3& f, V3 x, V3 g) {
@@ -76,8 +152,24 @@ __device__ __forceinline__ void add_outer(M3& f, V3 x, V3 g) {
     for (int j = 0; j < 3; ++j) f.a[i][j] = f.a[i][j] + xx[i] * gg[j];
 }
 
+// Sum the per-corner vectors of the 8 Gauss-sample lanes of a Hex8 cell; lane
+// s then holds corner s's total.
+__device__ __forceinline__ V3 hex_lane_sum(double (&out)[8][3], int s) {
+#pragma unroll
+  for (int off = 1; off < 8; off <<= 1)
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) out[j][k] += __shfl_xor(out[j][k], off, kWave);
+  V3 r{0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (j == s) r = V3{out[j][0], out[j][1], out[j][2]};
+  return r;
+}
+
 // ---------------------------------------------------------------------------
-// Step begin / end.
+// Step begin / end (one workgroup per vertex chunk, every body).
 
 __global__ __launch_bounds__(kBlock) void k_begin_step(DeviceState d, StepScalars s, const float* q,
                                                         const float* qd, const float* f, const float* dir) {
@@ -86,8 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_begin_step(DeviceState d, StepScalar
   if (threadIdx.x == 0 && static_cast<int>(blockIdx.x) == d.vcb[b]) {
     d.iter[b] = 0;
     d.active[b] = 1;
-    d.aa[4 * b] = d.aa[4 * b + 1] = d.aa[4 * b + 2] = d.aa[4 * b + 3] = 0;
-    d.accept[b] = 0;
+    d.aa[2 * b] = d.aa[2 * b + 1] = 0;
   }
   const int i = c.begin + threadIdx.x;
   if (i >= c.end) return;
@@ -138,12 +229,10 @@ __global__ void k_any_active(const int* active, int B, int* out) {
 // ---------------------------------------------------------------------------
 // Local step.
 
-__global__ __launch_bounds__(kBlock) void k_local_tet4(DeviceState d, const double* targets) {
-  const Chunk c = d.echunks[blockIdx.x];
-  if (!d.active[c.body]) return;
+__device__ __forceinline__ void local_tet4_chunk(const DeviceState& d, int ci, int b, const double* targets) {
+  const Chunk c = d.echunks[ci];
   const int e = c.begin + threadIdx.x;
   if (e >= c.end) return;
-  const int b = c.body;
   V3 g[4];
   tet_grads(d.poses, e, g);
   const int* nd = d.idx + 4 * e;
@@ -173,11 +262,9 @@ __global__ __launch_bounds__(kBlock) void k_local_tet4(DeviceState d, const doub
   for (int k = 0; k < 4; ++k) st3(d.contrib, 4 * e + k, mul(stress, g[k]));
 }
 
-// 32 cells per block, 8 lanes per cell (one per Gauss sample).
-__global__ __launch_bounds__(kBlock) void k_local_hex8(DeviceState d, const double* targets) {
-  const Chunk c = d.echunks[blockIdx.x];
-  if (!d.active[c.body]) return;
-  const int b = c.body;
+// 32 cells per chunk, 8 lanes per cell (one per Gauss sample).
+__device__ __forceinline__ void local_hex8_chunk(const DeviceState& d, int ci, int b, const double* targets) {
+  const Chunk c = d.echunks[ci];
   const int s = threadIdx.x & 7;
   const int e = c.begin + (threadIdx.x >> 3);
   const bool valid = e < c.end;
@@ -185,17 +272,16 @@ __global__ __launch_bounds__(kBlock) void k_local_hex8(DeviceState d, const doub
 #pragma unroll
   for (int j = 0; j < 8; ++j) out[j][0] = out[j][1] = out[j][2] = 0.0;
   if (valid) {
-    const double* G = d.hexg + 192 * b + 24 * s;
     const int* nd = d.idx + 8 * e;
     M3 f = zero3();
 #pragma unroll
-    for (int j = 0; j < 8; ++j) add_outer(f, ld3(d.x, nd[j]), V3{G[3 * j], G[3 * j + 1], G[3 * j + 2]});
+    for (int j = 0; j < 8; ++j) add_outer(f, ld3(d.x, nd[j]), hex_grad(d, b, s, j));
     M3 fd = zero3();
     const unsigned mask = d.pinmask[e];
     if (mask) {
 #pragma unroll
       for (int j = 0; j < 8; ++j)
-        if (mask & (1u << j)) add_outer(fd, ld3(d.xd, nd[j]), V3{G[3 * j], G[3 * j + 1], G[3 * j + 2]});
+        if (mask & (1u << j)) add_outer(fd, ld3(d.xd, nd[j]), hex_grad(d, b, s, j));
     }
     SampleWeights w{d.wc[e], d.wv[e], d.wn[e], 0.0};
     V3 m{0, 0, 0};
@@ -211,101 +297,101 @@ __global__ __launch_bounds__(kBlock) void k_local_hex8(DeviceState d, const doub
     if (st) atomicOr(d.status, st);
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const V3 r = mul(stress, V3{G[3 * j], G[3 * j + 1], G[3 * j + 2]});
+      const V3 r = mul(stress, hex_grad(d, b, s, j));
       out[j][0] = r.x;
       out[j][1] = r.y;
       out[j][2] = r.z;
     }
   }
-#pragma unroll
-  for (int off = 1; off < 8; off <<= 1)
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) out[j][k] += __shfl_xor(out[j][k], off, kWave);
-  if (valid) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (j == s) st3(d.contrib, 8 * e + j, V3{out[j][0], out[j][1], out[j][2]});
-  }
+  const V3 sum = hex_lane_sum(out, s);
+  if (valid) st3(d.contrib, 8 * e + s, sum);
+}
+
+__global__ __launch_bounds__(kBlock) void k_local(DeviceState d, const double* targets) {
+  for_active(d.active, d.ecb, d.B, [&](int ci, int b) {
+    if (d.Q == 1)
+      local_tet4_chunk(d, ci, b, targets);
+    else
+      local_hex8_chunk(d, ci, b, targets);
+  });
 }
 
 // ---------------------------------------------------------------------------
 // Gather + right-hand side (+ projection-change partials).
 
 __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d) {
-  const Chunk c = d.vchunks[blockIdx.x];
-  if (!d.active[c.body]) return;
   __shared__ double smem[2 * kBlock / kWave];
   __shared__ double res[2];
-  const int i = c.begin + threadIdx.x;
-  double part[2] = {0.0, 0.0};
-  if (i < c.end) {
-    V3 t{0, 0, 0};
-    for (int a = d.inc_off[i]; a < d.inc_off[i + 1]; ++a) t = t + ld3(d.contrib, d.inc_ent[a]);
-    const V3 old = ld3(d.p, i);
-    V3 r = d.inertia[c.body] * ld3(d.qt, i) + t;
-    if (d.defect) {
-      V3 ax{0, 0, 0};
-      for (int q = d.a_off[i]; q < d.a_off[i + 1]; ++q) ax = ax + d.a_val[q] * ld3(d.x, d.a_col[q]);
-      r = r - ax;
+  for_active(d.active, d.vcb, d.B, [&](int ci, int b) {
+    const Chunk c = d.vchunks[ci];
+    const int i = c.begin + threadIdx.x;
+    double part[2] = {0.0, 0.0};
+    if (i < c.end) {
+      V3 t{0, 0, 0};
+      for (int a = d.inc_off[i]; a < d.inc_off[i + 1]; ++a) t = t + ld3(d.contrib, d.inc_ent[a]);
+      const V3 old = ld3(d.p, i);
+      V3 r = d.inertia[b] * ld3(d.qt, i) + t;
+      if (d.defect) {
+        V3 ax{0, 0, 0};
+        for (int q = d.a_off[i]; q < d.a_off[i + 1]; ++q) ax = ax + d.a_val[q] * ld3(d.x, d.a_col[q]);
+        r = r - ax;
+      }
+      st3(d.p, i, t);
+      store_rhs(d, i, r);
+      const V3 dp = t - old;
+      part[0] = dot(dp, dp);
+      part[1] = dot(old, old);
     }
-    st3(d.p, i, t);
-    st3(d.rhs, i, r);
-    const V3 dp = t - old;
-    part[0] = dot(dp, dp);
-    part[1] = dot(old, old);
-  }
-  block_sum<2>(part, smem, res);
-  if (threadIdx.x < 2) d.conv_part[4 * blockIdx.x + 2 + threadIdx.x] = res[threadIdx.x];
+    block_sum<2>(part, smem, res);
+    if (threadIdx.x < 2) d.conv_part[4 * ci + 2 + threadIdx.x] = res[threadIdx.x];
+  });
 }
 
 // ---------------------------------------------------------------------------
-// CSR products y = M x for every active body's rows. PRED = false is the
-// plain product of every row (SparseInverse.apply), PRED = true skips the
-// rows of bodies whose fixed point has converged.
+// CSR products y = M x (+ base). Template parameters: Real the stored value
+// type, Col the column type (int32 global ids, or uint16 ids local to the
+// task's body), In / Out the vector element types. Accumulation is FP64.
+// PRED = true restricts the work list to the tasks of bodies that are still
+// iterating; PRED = false is the plain product of every row (one workgroup
+// per task, SparseInverse.apply).
 
-template <typename Real, bool PRED>
-__global__ __launch_bounds__(kBlock) void k_spmv(const SpmvTask* tasks, const int* rows, const int* off,
-                                                  const int* col, const Real* val, const double* in, double* out,
-                                                  const double* base, const int* vbody, const int* active) {
-  const SpmvTask t = tasks[blockIdx.x];
+template <typename Real, typename Col, typename In, typename Out>
+__device__ __forceinline__ void spmv_task(const SpmvTask t, const int* rows, const int* off, const Col* col,
+                                          const Real* val, const In* in, Out* out, const double* base,
+                                          double* smem, double* res) {
+  const int cb = sizeof(Col) == 2 ? t.base : 0;
+  const In* xin = in + 3 * cb;
   if (t.lanes == kBlock) {
     const int row = rows[t.first];
-    if (PRED && !active[vbody[row]]) return;
-    __shared__ double smem[3 * kBlock / kWave];
-    __shared__ double res[3];
     double acc[3] = {0.0, 0.0, 0.0};
     for (int q = off[row] + threadIdx.x; q < off[row + 1]; q += kBlock) {
       const double v = static_cast<double>(val[q]);
-      const double* x = in + 3 * col[q];
-      acc[0] += v * x[0];
-      acc[1] += v * x[1];
-      acc[2] += v * x[2];
+      const In* x = xin + 3 * static_cast<int>(col[q]);
+      acc[0] += v * static_cast<double>(x[0]);
+      acc[1] += v * static_cast<double>(x[1]);
+      acc[2] += v * static_cast<double>(x[2]);
     }
     block_sum<3>(acc, smem, res);
     if (threadIdx.x == 0) {
       V3 r{res[0], res[1], res[2]};
       if (base) r = r + ld3(base, row);
-      st3(out, row, r);
+      out[3 * row] = static_cast<Out>(r.x);
+      out[3 * row + 1] = static_cast<Out>(r.y);
+      out[3 * row + 2] = static_cast<Out>(r.z);
     }
     return;
   }
   const int L = t.lanes;
   const int group = threadIdx.x / L, lane = threadIdx.x % L;
-  int row = -1;
-  if (group < t.count) {
-    row = rows[t.first + group];
-    if (PRED && !active[vbody[row]]) row = -1;
-  }
+  const int row = group < t.count ? rows[t.first + group] : -1;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   if (row >= 0) {
     for (int q = off[row] + lane; q < off[row + 1]; q += L) {
       const double v = static_cast<double>(val[q]);
-      const double* x = in + 3 * col[q];
-      a0 += v * x[0];
-      a1 += v * x[1];
-      a2 += v * x[2];
+      const In* x = xin + 3 * static_cast<int>(col[q]);
+      a0 += v * static_cast<double>(x[0]);
+      a1 += v * static_cast<double>(x[1]);
+      a2 += v * static_cast<double>(x[2]);
     }
   }
   for (int o = L / 2; o > 0; o >>= 1) {
@@ -316,24 +402,46 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const SpmvTask* tasks, const in
   if (row >= 0 && lane == 0) {
     V3 r{a0, a1, a2};
     if (base) r = r + ld3(base, row);
-    st3(out, row, r);
+    out[3 * row] = static_cast<Out>(r.x);
+    out[3 * row + 1] = static_cast<Out>(r.y);
+    out[3 * row + 2] = static_cast<Out>(r.z);
+  }
+}
+
+template <typename Real, typename Col, typename In, typename Out, bool PRED>
+__global__ __launch_bounds__(kBlock) void k_spmv(const SpmvTask* tasks, const int* rows, const int* off,
+                                                  const Col* col, const Real* val, const In* in, Out* out,
+                                                  const double* base, const int* active, const int* tb, int B) {
+  __shared__ double smem[3 * kBlock / kWave];
+  __shared__ double res[3];
+  if (PRED) {
+    for_active(active, tb, B, [&](int ti, int) {
+      spmv_task<Real, Col, In, Out>(tasks[ti], rows, off, col, val, in, out, base, smem, res);
+    });
+  } else {
+    spmv_task<Real, Col, In, Out>(tasks[blockIdx.x], rows, off, col, val, in, out, base, smem, res);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Anderson mixing.
+// Anderson mixing. The ring state (count, start) is double-buffered by the
+// parity of the body's iteration count: the push and apply of iteration k
+// read aa[k & 1], and the apply's writer stores the updated state in
+// aa[(k + 1) & 1], so no workgroup can observe a half-updated state.
 
 __device__ __forceinline__ int gram_index(int a, int b) {  // a <= b < 8
   return a * kMaxWindow - (a * (a - 1)) / 2 + (b - a);
 }
 
-// Push (x_prev, g) and emit Gram partials of the body's history for this
-// chunk. `prev` is the previous iterate, `raw` the map output; when `zero_pinned`
-// Dirichlet entries of raw are zeroed (adjoint) instead of set to xd (forward).
-__device__ __forceinline__ void push_and_gram(const DeviceState& d, const Chunk& c, int i, V3 prev, V3 raw,
+__device__ __forceinline__ int parity_base(const DeviceState& d, const int* iters, int b) {
+  return 2 * ((iters[b] & 1) * d.B + b);
+}
+
+// Push (x_prev, g = raw - x_prev) and accumulate this vertex's Gram terms.
+__device__ __forceinline__ void push_and_gram(const DeviceState& d, int b, int i, V3 prev, V3 raw, const int* iters,
                                               double (&acc)[kGramEntries]) {
-  const int b = c.body;
-  const int count = d.aa[4 * b], start = d.aa[4 * b + 1];
+  const int pb = parity_base(d, iters, b);
+  const int count = d.aa[pb], start = d.aa[pb + 1];
   const int cap = d.W + 1;
   const int slot = (start + count) % cap;
   const int N = d.N;
@@ -357,36 +465,31 @@ __device__ __forceinline__ void push_and_gram(const DeviceState& d, const Chunk&
   for (int a = 0; a < kMaxWindow; ++a) {
 #pragma unroll
     for (int bb = a; bb < kMaxWindow; ++bb) acc[gram_index(a, bb)] += dot(dg[a], dg[bb]);
-    acc[kMaxWindow * (kMaxWindow + 1) / 2 + a] += dot(dg[a], g);
+    acc[kRhsIndex + a] += dot(dg[a], g);
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_push_gram(DeviceState d) {
-  const Chunk c = d.vchunks[blockIdx.x];
-  if (!d.active[c.body]) return;
-  __shared__ double smem[kGramEntries * kBlock / kWave];
-  __shared__ double res[kGramEntries];
-  const int i = c.begin + threadIdx.x;
-  double acc[kGramEntries];
+template <int N>
+__device__ __forceinline__ void solve_fixed(const double* gram, const double* rhs, double* coef) {
+  double a[N * N], b[N], x[N];
 #pragma unroll
-  for (int k = 0; k < kGramEntries; ++k) acc[k] = 0.0;
-  if (i < c.end) {
-    V3 raw = ld3(d.xr, i);
-    if (!d.free[i]) {
-      raw = ld3(d.xd, i);
-      st3(d.xr, i, raw);
-    }
-    push_and_gram(d, c, i, ld3(d.x, i), raw, acc);
+  for (int i = 0; i < N; ++i) {
+    b[i] = rhs[i];
+#pragma unroll
+    for (int j = 0; j < N; ++j) a[i * N + j] = gram[i * kMaxWindow + j];
   }
-  block_sum<kGramEntries>(acc, smem, res);
-  for (int k = threadIdx.x; k < kGramEntries; k += blockDim.x) d.gram_part[kGramEntries * blockIdx.x + k] = res[k];
+  ldlt_solve<N>(a, N, b, N, x);
+#pragma unroll
+  for (int i = 0; i < N; ++i) coef[i] = x[i];
 }
 
-// One block per body: chunk-ordered reduction and the reference _solve.
-__global__ void k_gram_solve(DeviceState d, const int* active) {
-  const int b = blockIdx.x;
-  if (!active[b]) return;
-  __shared__ double red[kGramEntries];
+// Reduce the body's Gram partials in chunk order and run the reference
+// _solve: mixing coefficients and the ring update. Results in LDS:
+// gamma[0..count), meta[0] = accepted, meta[1] = count used by the apply,
+// meta[2] = its start. The writer stores the updated ring state.
+__device__ void gram_solve(const DeviceState& d, int b, const int* iters, bool writer, double* red, double* gamma,
+                           int* meta, int* status) {
+  const int pb = parity_base(d, iters, b);
   const int c0 = d.vcb[b], c1 = d.vcb[b + 1];
   for (int k = threadIdx.x; k < kGramEntries; k += blockDim.x) {
     double t = 0.0;
@@ -394,83 +497,130 @@ __global__ void k_gram_solve(DeviceState d, const int* active) {
     red[k] = t;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const int W = d.W, cap = W + 1;
-  int count = d.aa[4 * b], start = d.aa[4 * b + 1];
-  d.aa[4 * b + 2] = count;
-  d.aa[4 * b + 3] = start;
-  int accept = 0;
-  if (count >= 1) {
-    double gram[kMaxWindow * kMaxWindow], rhs[kMaxWindow], coef[kMaxWindow];
-    for (int a = 0; a < count; ++a) {
-      rhs[a] = red[kMaxWindow * (kMaxWindow + 1) / 2 + a];
-      for (int bb = a; bb < count; ++bb) gram[a * kMaxWindow + bb] = red[gram_index(a, bb)];
-    }
-    double trace = 0.0;
-    for (int a = 0; a < count; ++a) {
-      trace += gram[a * kMaxWindow + a];
-      for (int bb = a + 1; bb < count; ++bb) gram[bb * kMaxWindow + a] = gram[a * kMaxWindow + bb];
-    }
-    const double reg = fmax(1.0e-6 * trace / static_cast<double>(count), 1.0e-12);
-    for (int a = 0; a < count; ++a) gram[a * kMaxWindow + a] = gram[a * kMaxWindow + a] + reg;
-    ldlt_solve<kMaxWindow>(gram, kMaxWindow, rhs, count, coef);
-    double norm = 0.0;
-    for (int a = 0; a < count; ++a) {
-      d.gamma[kMaxWindow * b + a] = coef[a];
-      norm += coef[a] * coef[a];
-    }
-    if (sqrt(norm) >= 10.0) {
-      count = 0;
-      start = 0;
+  if (threadIdx.x == 0) {
+    const int W = d.W, cap = W + 1;
+    int count = d.aa[pb], start = d.aa[pb + 1];
+    meta[1] = count;
+    meta[2] = start;
+    int accept = 0;
+    if (count >= 1) {
+      double gram[kMaxWindow * kMaxWindow], rhs[kMaxWindow], coef[kMaxWindow];
+      for (int a = 0; a < count; ++a) {
+        rhs[a] = red[kRhsIndex + a];
+        for (int bb = a; bb < count; ++bb) gram[a * kMaxWindow + bb] = red[gram_index(a, bb)];
+      }
+      double trace = 0.0;
+      for (int a = 0; a < count; ++a) {
+        trace += gram[a * kMaxWindow + a];
+        for (int bb = a + 1; bb < count; ++bb) gram[bb * kMaxWindow + a] = gram[a * kMaxWindow + bb];
+      }
+      const double reg = fmax(1.0e-6 * trace / static_cast<double>(count), 1.0e-12);
+      for (int a = 0; a < count; ++a) gram[a * kMaxWindow + a] = gram[a * kMaxWindow + a] + reg;
+      switch (count) {
+        case 1: solve_fixed<1>(gram, rhs, coef); break;
+        case 2: solve_fixed<2>(gram, rhs, coef); break;
+        case 3: solve_fixed<3>(gram, rhs, coef); break;
+        case 4: solve_fixed<4>(gram, rhs, coef); break;
+        case 5: solve_fixed<5>(gram, rhs, coef); break;
+        case 6: solve_fixed<6>(gram, rhs, coef); break;
+        case 7: solve_fixed<7>(gram, rhs, coef); break;
+        default: solve_fixed<8>(gram, rhs, coef); break;
+      }
+      double norm = 0.0;
+      for (int a = 0; a < count; ++a) {
+        gamma[a] = coef[a];
+        norm += coef[a] * coef[a];
+      }
+      if (sqrt(norm) >= 10.0) {
+        count = 0;
+        start = 0;
+      } else {
+        accept = 1;
+        count += 1;
+      }
     } else {
-      accept = 1;
       count += 1;
     }
-  } else {
-    count += 1;
+    while (count > W) {
+      start = (start + 1) % cap;
+      count -= 1;
+    }
+    meta[0] = accept;
+    if (writer) {
+      const int nb = 2 * ((((iters[b] + 1) & 1)) * d.B + b);
+      d.aa[nb] = count;
+      d.aa[nb + 1] = start;
+    }
   }
-  while (count > W) {
-    start = (start + 1) % cap;
-    count -= 1;
-  }
-  d.aa[4 * b] = count;
-  d.aa[4 * b + 1] = start;
-  d.accept[b] = accept;
+  __syncthreads();
+  (void)status;
 }
 
-__device__ __forceinline__ V3 anderson_value(const DeviceState& d, int b, int i, V3 raw) {
-  if (d.W == 0 || !d.accept[b]) return raw;
+__device__ __forceinline__ V3 anderson_value(const DeviceState& d, int i, V3 raw, const double* gamma,
+                                             const int* meta) {
+  if (d.W == 0 || !meta[0]) return raw;
   const int cap = d.W + 1, N = d.N;
-  const int sc = d.aa[4 * b + 2], ss = d.aa[4 * b + 3];
+  const int sc = meta[1], ss = meta[2];
   V3 v = ld3(d.hr, ((ss + sc) % cap) * N + i);
   V3 lo = ld3(d.hr, (ss % cap) * N + i);
   for (int a = 0; a < sc; ++a) {
     const V3 hi = ld3(d.hr, ((ss + a + 1) % cap) * N + i);
-    v = v - d.gamma[kMaxWindow * b + a] * (hi - lo);
+    v = v - gamma[a] * (hi - lo);
     lo = hi;
   }
   return v;
 }
 
+__global__ __launch_bounds__(kBlock) void k_push_gram(DeviceState d) {
+  __shared__ double smem[kGramEntries * kBlock / kWave];
+  __shared__ double res[kGramEntries];
+  for_active(d.active, d.vcb, d.B, [&](int ci, int b) {
+    const Chunk c = d.vchunks[ci];
+    const int i = c.begin + threadIdx.x;
+    double acc[kGramEntries];
+#pragma unroll
+    for (int k = 0; k < kGramEntries; ++k) acc[k] = 0.0;
+    if (i < c.end) {
+      V3 raw = ld3(d.xr, i);
+      if (!d.free[i]) {
+        raw = ld3(d.xd, i);
+        st3(d.xr, i, raw);
+      }
+      push_and_gram(d, b, i, ld3(d.x, i), raw, d.iter, acc);
+    }
+    block_sum<kGramEntries>(acc, smem, res);
+    for (int k = threadIdx.x; k < kGramEntries; k += blockDim.x) d.gram_part[kGramEntries * ci + k] = res[k];
+  });
+}
+
 __global__ __launch_bounds__(kBlock) void k_apply_conv(DeviceState d) {
-  const Chunk c = d.vchunks[blockIdx.x];
-  const int b = c.body;
-  if (!d.active[b]) return;
   __shared__ double smem[2 * kBlock / kWave];
   __shared__ double res[2];
-  const int i = c.begin + threadIdx.x;
-  double part[2] = {0.0, 0.0};
-  if (i < c.end) {
-    const V3 prev = ld3(d.x, i);
-    V3 v = anderson_value(d, b, i, ld3(d.xr, i));
-    if (!d.free[i]) v = ld3(d.xd, i);
-    const V3 step = v - prev;
-    part[0] = dot(step, step);
-    part[1] = dot(prev, prev);
-    st3(d.x, i, v);
-  }
-  block_sum<2>(part, smem, res);
-  if (threadIdx.x < 2) d.conv_part[4 * blockIdx.x + threadIdx.x] = res[threadIdx.x];
+  __shared__ double red[kGramEntries];
+  __shared__ double gamma[kMaxWindow];
+  __shared__ int meta[3];
+  for_active(d.active, d.vcb, d.B, [&](int ci, int b) {
+    if (d.W > 0) {
+      gram_solve(d, b, d.iter, ci == d.vcb[b], red, gamma, meta, d.status);
+    } else if (threadIdx.x == 0) {
+      meta[0] = 0;
+    }
+    __syncthreads();
+    const Chunk c = d.vchunks[ci];
+    const int i = c.begin + threadIdx.x;
+    double part[2] = {0.0, 0.0};
+    if (i < c.end) {
+      const V3 prev = ld3(d.x, i);
+      V3 v = anderson_value(d, i, ld3(d.xr, i), gamma, meta);
+      if (!d.free[i]) v = ld3(d.xd, i);
+      const V3 step = v - prev;
+      part[0] = dot(step, step);
+      part[1] = dot(prev, prev);
+      st3(d.x, i, v);
+    }
+    block_sum<2>(part, smem, res);
+    if (threadIdx.x < 2) d.conv_part[4 * ci + threadIdx.x] = res[threadIdx.x];
+  });
 }
 
 __global__ void k_conv_check(DeviceState d, StepScalars s) {
@@ -506,8 +656,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_seed(DeviceState d, double dt, c
   if (threadIdx.x == 0 && static_cast<int>(blockIdx.x) == d.vcb[b]) {
     d.aiter[b] = 0;
     d.aactive[b] = 1;
-    d.aa[4 * b] = d.aa[4 * b + 1] = d.aa[4 * b + 2] = d.aa[4 * b + 3] = 0;
-    d.accept[b] = 0;
+    d.aa[2 * b] = d.aa[2 * b + 1] = 0;
   }
   const int i = c.begin + threadIdx.x;
   if (i >= c.end) return;
@@ -516,11 +665,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_seed(DeviceState d, double dt, c
     d.arhs[3 * i + k] = (gq ? static_cast<double>(gq[3 * i + k]) : 0.0) + vel;
     d.sol[3 * i + k] = 0.0;
   }
-}
-
-__device__ __forceinline__ V3 hex_grad(const DeviceState& d, int b, int s, int j) {
-  const double* G = d.hexg + 192 * b + 24 * s + 3 * j;
-  return V3{G[0], G[1], G[2]};
 }
 
 // Jacobian columns of every sample, stored structure-of-arrays [81][samples]
@@ -566,12 +710,11 @@ __device__ __forceinline__ M3 jac_apply(const DeviceState& d, int sample, const 
   return ds;
 }
 
-// Element rows of K mu (mode 0) or of the Dirichlet coupling (A - K) mu
-// (mode 1, elements with a pinned corner only).
+// Element rows of K mu (MODE 0) or of the Dirichlet coupling (A - K) mu
+// (MODE 1, elements with a pinned corner only).
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_adj_local_tet4(DeviceState d, const double* mu) {
-  const Chunk c = d.echunks[blockIdx.x];
-  if (MODE == 0 && !d.aactive[c.body]) return;
+__device__ __forceinline__ void adj_local_tet4_chunk(const DeviceState& d, int ci, int b, const double* mu) {
+  const Chunk c = d.echunks[ci];
   const int e = c.begin + threadIdx.x;
   if (e >= c.end) return;
   if (MODE == 1 && d.pinmask[e] == 0) return;
@@ -585,7 +728,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_local_tet4(DeviceState d, const 
   if (MODE == 0) {
     ds = jac_apply(d, e, df);
   } else {
-    if (d.route[2 * c.body + 1] != 0.0) {
+    if (d.route[2 * b + 1] != 0.0) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) st3(d.contrib, 4 * e + k, V3{0, 0, 0});
       return;
@@ -602,10 +745,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_local_tet4(DeviceState d, const 
 }
 
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_adj_local_hex8(DeviceState d, const double* mu) {
-  const Chunk c = d.echunks[blockIdx.x];
-  if (MODE == 0 && !d.aactive[c.body]) return;
-  const int b = c.body;
+__device__ __forceinline__ void adj_local_hex8_chunk(const DeviceState& d, int ci, int b, const double* mu) {
+  const Chunk c = d.echunks[ci];
   const int s = threadIdx.x & 7;
   const int e = c.begin + (threadIdx.x >> 3);
   bool valid = e < c.end;
@@ -638,150 +779,172 @@ __global__ __launch_bounds__(kBlock) void k_adj_local_hex8(DeviceState d, const 
       out[j][2] = r.z;
     }
   }
-#pragma unroll
-  for (int off = 1; off < 8; off <<= 1)
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) out[j][k] += __shfl_xor(out[j][k], off, kWave);
-  if (valid) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (j == s) st3(d.contrib, 8 * e + j, V3{out[j][0], out[j][1], out[j][2]});
-  }
+  const V3 sum = hex_lane_sum(out, s);
+  if (valid) st3(d.contrib, 8 * e + s, sum);
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_local(DeviceState d) {
+  for_active(d.aactive, d.ecb, d.B, [&](int ci, int b) {
+    if (d.Q == 1)
+      adj_local_tet4_chunk<0>(d, ci, b, d.sol);
+    else
+      adj_local_hex8_chunk<0>(d, ci, b, d.sol);
+  });
+}
+
+// Dirichlet coupling rows over every element chunk (once per backward).
+__global__ __launch_bounds__(kBlock) void k_adj_coupling(DeviceState d) {
+  const int ci = blockIdx.x;
+  const int b = d.echunks[ci].body;
+  if (d.Q == 1)
+    adj_local_tet4_chunk<1>(d, ci, b, d.sol);
+  else
+    adj_local_hex8_chunk<1>(d, ci, b, d.sol);
 }
 
 // work = sum(contrib) + rhs (gather with accumulate, adjoint.py:939).
 __global__ __launch_bounds__(kBlock) void k_adj_gather(DeviceState d) {
-  const Chunk c = d.vchunks[blockIdx.x];
-  if (!d.aactive[c.body]) return;
-  const int i = c.begin + threadIdx.x;
-  if (i >= c.end) return;
-  V3 t{0, 0, 0};
-  for (int a = d.inc_off[i]; a < d.inc_off[i + 1]; ++a) t = t + ld3(d.contrib, d.inc_ent[a]);
-  t = t + ld3(d.arhs, i);
-  if (d.defect) {
-    V3 ax{0, 0, 0};
-    for (int q = d.a_off[i]; q < d.a_off[i + 1]; ++q) ax = ax + d.a_val[q] * ld3(d.sol, d.a_col[q]);
-    t = t - ax;
-  }
-  st3(d.rhs, i, t);
+  for_active(d.aactive, d.vcb, d.B, [&](int ci, int) {
+    const Chunk c = d.vchunks[ci];
+    const int i = c.begin + threadIdx.x;
+    if (i >= c.end) return;
+    V3 t{0, 0, 0};
+    for (int a = d.inc_off[i]; a < d.inc_off[i + 1]; ++a) t = t + ld3(d.contrib, d.inc_ent[a]);
+    t = t + ld3(d.arhs, i);
+    if (d.defect) {
+      V3 ax{0, 0, 0};
+      for (int q = d.a_off[i]; q < d.a_off[i + 1]; ++q) ax = ax + d.a_val[q] * ld3(d.sol, d.a_col[q]);
+      t = t - ax;
+    }
+    store_rhs(d, i, t);
+  });
+}
+
+__device__ __forceinline__ V3 scrub(V3 v, bool& scrubbed) {
+  if (!isfinite(v.x)) { v.x = 0.0; scrubbed = true; }
+  if (!isfinite(v.y)) { v.y = 0.0; scrubbed = true; }
+  if (!isfinite(v.z)) { v.z = 0.0; scrubbed = true; }
+  return v;
 }
 
 // Safe-clip partial: Dirichlet entries zeroed, nonfinite components scrubbed
 // (adjoint.py:184-204).
 __global__ __launch_bounds__(kBlock) void k_adj_clip_partial(DeviceState d) {
-  const Chunk c = d.vchunks[blockIdx.x];
-  if (!d.aactive[c.body]) return;
   __shared__ double smem[kBlock / kWave];
   __shared__ double res[1];
-  const int i = c.begin + threadIdx.x;
-  double part[1] = {0.0};
-  if (i < c.end) {
-    V3 v = ld3(d.xr, i);
-    if (!d.free[i]) v = V3{0, 0, 0};
-    bool scrubbed = false;
-    if (!isfinite(v.x)) { v.x = 0.0; scrubbed = true; }
-    if (!isfinite(v.y)) { v.y = 0.0; scrubbed = true; }
-    if (!isfinite(v.z)) { v.z = 0.0; scrubbed = true; }
-    if (scrubbed) atomicOr(d.status, 16);
-    st3(d.xr, i, v);
-    part[0] = dot(v, v);
-  }
-  block_sum<1>(part, smem, res);
-  if (threadIdx.x == 0) d.clip_part[blockIdx.x] = res[0];
+  for_active(d.aactive, d.vcb, d.B, [&](int ci, int) {
+    const Chunk c = d.vchunks[ci];
+    const int i = c.begin + threadIdx.x;
+    double part[1] = {0.0};
+    if (i < c.end) {
+      V3 v = ld3(d.xr, i);
+      if (!d.free[i]) v = V3{0, 0, 0};
+      bool scrubbed = false;
+      v = scrub(v, scrubbed);
+      if (scrubbed) atomicOr(d.status, 16);
+      st3(d.xr, i, v);
+      part[0] = dot(v, v);
+    }
+    block_sum<1>(part, smem, res);
+    if (threadIdx.x == 0) d.clip_part[ci] = res[0];
+  });
 }
 
-// Clip scale of one body (adjoint.py:207-223): limit / ||v|| when above 1e6.
-__global__ void k_adj_clip_scale(DeviceState d, int which) {
-  const int b = blockIdx.x;
-  if (!d.aactive[b]) return;
-  if (threadIdx.x != 0) return;
+// Clip scale of one body (adjoint.py:207-223): limit / ||v|| above 1e6.
+// Every workgroup recomputes it in the same order; the body's first chunk
+// reports the safeguard.
+__device__ __forceinline__ double clip_scale(const DeviceState& d, const double* parts, int b, bool reporter) {
   double t = 0.0;
-  for (int c = d.vcb[b]; c < d.vcb[b + 1]; ++c) t += d.clip_part[c];
+  for (int c = d.vcb[b]; c < d.vcb[b + 1]; ++c) t += parts[c];
   const double len = sqrt(t);
-  double sc = 1.0;
   const double limit = 1.0e6;
   if (isfinite(len) && len > limit) {
-    sc = limit / len;
-    atomicOr(d.status, 16);
+    if (reporter && threadIdx.x == 0) atomicOr(d.status, 16);
+    return limit / len;
   }
-  d.ascale[2 * b + which] = sc;
+  return 1.0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_adj_push(DeviceState d) {
-  const Chunk c = d.vchunks[blockIdx.x];
-  const int b = c.body;
-  if (!d.aactive[b]) return;
   __shared__ double smem[(kGramEntries + 1) * kBlock / kWave];
   __shared__ double res[kGramEntries + 1];
-  const int i = c.begin + threadIdx.x;
-  double acc[kGramEntries + 1];
+  for_active(d.aactive, d.vcb, d.B, [&](int ci, int b) {
+    const double sc = clip_scale(d, d.clip_part, b, ci == d.vcb[b]);
+    const Chunk c = d.vchunks[ci];
+    const int i = c.begin + threadIdx.x;
+    double acc[kGramEntries + 1];
 #pragma unroll
-  for (int k = 0; k < kGramEntries + 1; ++k) acc[k] = 0.0;
-  if (i < c.end) {
-    const double sc = d.ascale[2 * b];
-    V3 raw = ld3(d.xr, i);
-    if (sc != 1.0) raw = sc * raw;
-    if (!d.free[i]) raw = V3{0, 0, 0};
-    st3(d.xr, i, raw);
-    const V3 prev = ld3(d.sol, i);
-    const V3 step = raw - prev;
-    acc[kGramEntries] = dot(step, step);
-    if (d.W > 0) {
-      double g[kGramEntries];
+    for (int k = 0; k < kGramEntries + 1; ++k) acc[k] = 0.0;
+    if (i < c.end) {
+      V3 raw = ld3(d.xr, i);
+      if (sc != 1.0) raw = sc * raw;
+      if (!d.free[i]) raw = V3{0, 0, 0};
+      st3(d.xr, i, raw);
+      const V3 prev = ld3(d.sol, i);
+      const V3 step = raw - prev;
+      acc[kGramEntries] = dot(step, step);
+      if (d.W > 0) {
+        double g[kGramEntries];
 #pragma unroll
-      for (int k = 0; k < kGramEntries; ++k) g[k] = 0.0;
-      push_and_gram(d, c, i, prev, raw, g);
+        for (int k = 0; k < kGramEntries; ++k) g[k] = 0.0;
+        push_and_gram(d, b, i, prev, raw, d.aiter, g);
 #pragma unroll
-      for (int k = 0; k < kGramEntries; ++k) acc[k] = g[k];
+        for (int k = 0; k < kGramEntries; ++k) acc[k] = g[k];
+      }
     }
-  }
-  block_sum<kGramEntries + 1>(acc, smem, res);
-  for (int k = threadIdx.x; k < kGramEntries; k += blockDim.x) d.gram_part[kGramEntries * blockIdx.x + k] = res[k];
-  if (threadIdx.x == 0) d.conv_part[4 * blockIdx.x] = res[kGramEntries];
+    block_sum<kGramEntries + 1>(acc, smem, res);
+    for (int k = threadIdx.x; k < kGramEntries; k += blockDim.x) d.gram_part[kGramEntries * ci + k] = res[k];
+    if (threadIdx.x == 0) d.conv_part[4 * ci] = res[kGramEntries];
+  });
 }
 
 __global__ __launch_bounds__(kBlock) void k_adj_apply(DeviceState d) {
-  const Chunk c = d.vchunks[blockIdx.x];
-  const int b = c.body;
-  if (!d.aactive[b]) return;
   __shared__ double smem[kBlock / kWave];
   __shared__ double res[1];
-  const int i = c.begin + threadIdx.x;
-  double part[1] = {0.0};
-  if (i < c.end) {
-    V3 v = anderson_value(d, b, i, ld3(d.xr, i));
-    if (!d.free[i]) v = V3{0, 0, 0};
-    bool scrubbed = false;
-    if (!isfinite(v.x)) { v.x = 0.0; scrubbed = true; }
-    if (!isfinite(v.y)) { v.y = 0.0; scrubbed = true; }
-    if (!isfinite(v.z)) { v.z = 0.0; scrubbed = true; }
-    if (scrubbed) atomicOr(d.status, 16);
-    st3(d.xr, i, v);
-    part[0] = dot(v, v);
-  }
-  block_sum<1>(part, smem, res);
-  if (threadIdx.x == 0) d.clip_part[blockIdx.x] = res[0];
+  __shared__ double red[kGramEntries];
+  __shared__ double gamma[kMaxWindow];
+  __shared__ int meta[3];
+  for_active(d.aactive, d.vcb, d.B, [&](int ci, int b) {
+    if (d.W > 0) {
+      gram_solve(d, b, d.aiter, ci == d.vcb[b], red, gamma, meta, d.status);
+    } else if (threadIdx.x == 0) {
+      meta[0] = 0;
+    }
+    __syncthreads();
+    const Chunk c = d.vchunks[ci];
+    const int i = c.begin + threadIdx.x;
+    double part[1] = {0.0};
+    if (i < c.end) {
+      V3 v = anderson_value(d, i, ld3(d.xr, i), gamma, meta);
+      if (!d.free[i]) v = V3{0, 0, 0};
+      bool scrubbed = false;
+      v = scrub(v, scrubbed);
+      if (scrubbed) atomicOr(d.status, 16);
+      st3(d.xr, i, v);
+      part[0] = dot(v, v);
+    }
+    block_sum<1>(part, smem, res);
+    if (threadIdx.x == 0) d.clip_part2[ci] = res[0];
+  });
 }
 
 __global__ __launch_bounds__(kBlock) void k_adj_finish(DeviceState d) {
-  const Chunk c = d.vchunks[blockIdx.x];
-  const int b = c.body;
-  if (!d.aactive[b]) return;
   __shared__ double smem[kBlock / kWave];
   __shared__ double res[1];
-  const int i = c.begin + threadIdx.x;
-  double part[1] = {0.0};
-  if (i < c.end) {
-    const double sc = d.ascale[2 * b + 1];
-    V3 v = ld3(d.xr, i);
-    if (sc != 1.0) v = sc * v;
-    st3(d.sol, i, v);
-    part[0] = dot(v, v);
-  }
-  block_sum<1>(part, smem, res);
-  if (threadIdx.x == 0) d.conv_part[4 * blockIdx.x + 1] = res[0];
+  for_active(d.aactive, d.vcb, d.B, [&](int ci, int b) {
+    const double sc = clip_scale(d, d.clip_part2, b, ci == d.vcb[b]);
+    const Chunk c = d.vchunks[ci];
+    const int i = c.begin + threadIdx.x;
+    double part[1] = {0.0};
+    if (i < c.end) {
+      V3 v = ld3(d.xr, i);
+      if (sc != 1.0) v = sc * v;
+      st3(d.sol, i, v);
+      part[0] = dot(v, v);
+    }
+    block_sum<1>(part, smem, res);
+    if (threadIdx.x == 0) d.conv_part[4 * ci + 1] = res[0];
+  });
 }
 
 __global__ void k_adj_check(DeviceState d, StepScalars s) {
@@ -878,8 +1041,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_dirichlet(DeviceState d, double 
 
 // PD material-gradient terms of one sample (adjoint.py:556-591): F - R and
 // F - U diag(sigma + d) V^T contracted with z.
-__device__ __forceinline__ void material_terms(const DeviceState& d, const M3& f, double& first_out,
-                                               double& second_out, const V3* g, const V3* z, int K) {
+__device__ __forceinline__ void material_terms(const M3& f, double& first_out, double& second_out, const V3* g,
+                                               const V3* z, int K) {
   M3 u, v;
   V3 sg;
   unsigned_svd(f, u, sg, v);
@@ -911,7 +1074,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_material_tet4(DeviceState d, con
   }
   double fv, sv;
   if (d.material == 0) {
-    material_terms(d, f, fv, sv, g, z, 4);
+    material_terms(f, fv, sv, g, z, 4);
   } else {
     const double mu = d.nh[3 * b], lame = d.nh[3 * b + 1];
     const M3 first = scale(1.0 / (2.0 * mu), neo_hookean_stress(f, mu, lame));
@@ -941,7 +1104,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_material_hex8(DeviceState d, con
       z[j] = ld3(d.sol, n);
     }
     if (d.material == 0) {
-      material_terms(d, f, fv, sv, g, z, 8);
+      material_terms(f, fv, sv, g, z, 8);
     } else {
       const double mu = d.nh[3 * b], lame = d.nh[3 * b + 1];
       const M3 first = scale(1.0 / (2.0 * mu), neo_hookean_stress(f, mu, lame));
@@ -961,12 +1124,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_material_hex8(DeviceState d, con
 
 // Aggregate guard per body (adjoint.py:622-644), parallel over the body's elements.
 __global__ __launch_bounds__(kBlock) void k_adj_material_guard(DeviceState d) {
-  const int* ebeg = d.eoff;
   const int b = blockIdx.x;
   __shared__ double smem[2 * kBlock / kWave];
   __shared__ double res[2];
   double part[2] = {0.0, 0.0};
-  for (int e = ebeg[b] + threadIdx.x; e < ebeg[b + 1]; e += kBlock) {
+  for (int e = d.eoff[b] + threadIdx.x; e < d.eoff[b + 1]; e += kBlock) {
     part[0] += d.gmat_tmp[2 * e];
     part[1] += d.gmat_tmp[2 * e + 1];
   }
@@ -1042,23 +1204,37 @@ __global__ void k_eval_jvps(int count, const double* fs, const double* dfs, cons
   st3(fibre, i, muscle_jvp(f, ld3(dirs, i), target, df));
 }
 
-template <typename Real, bool PRED>
-void spmv_pair(const DeviceState& d, const double* in, double* mid, double* out, const double* base,
-               const int* active, hipStream_t stream) {
-  if (d.s_ntasks)
-    hipLaunchKernelGGL((k_spmv<Real, PRED>), dim3(d.s_ntasks), dim3(kBlock), 0, stream, d.s_tasks, d.s_rows,
-                       d.s_off, d.s_col, static_cast<const Real*>(d.s_val), in, mid, nullptr, d.vbody, active);
-  if (d.t_ntasks)
-    hipLaunchKernelGGL((k_spmv<Real, PRED>), dim3(d.t_ntasks), dim3(kBlock), 0, stream, d.t_tasks, d.t_rows,
-                       d.t_off, d.t_col, static_cast<const Real*>(d.t_val), mid, out, base, d.vbody, active);
+size_t prefix_lds(const DeviceState& d) { return sizeof(int) * static_cast<size_t>(d.B + 1); }
+
+// One CSR product of S (transpose = false) or S^T over the work list (PRED)
+// or over every task.
+template <typename Real, typename Col, typename In, typename Out, bool PRED>
+void spmv_one(const DeviceState& d, bool transpose, const In* in, Out* out, const double* base, const int* active,
+              hipStream_t stream) {
+  const int ntasks = transpose ? d.t_ntasks : d.s_ntasks;
+  if (!ntasks) return;
+  const void* col = sizeof(Col) == 2 ? static_cast<const void*>(transpose ? d.t_col16 : d.s_col16)
+                                     : static_cast<const void*>(transpose ? d.t_col : d.s_col);
+  const int grid = PRED ? (transpose ? d.g_ttask : d.g_stask) : ntasks;
+  hipLaunchKernelGGL((k_spmv<Real, Col, In, Out, PRED>), dim3(grid), dim3(kBlock), prefix_lds(d), stream,
+                     transpose ? d.t_tasks : d.s_tasks, transpose ? d.t_rows : d.s_rows,
+                     transpose ? d.t_off : d.s_off, static_cast<const Col*>(col),
+                     static_cast<const Real*>(transpose ? d.t_val : d.s_val), in, out, base, active,
+                     transpose ? d.ttb : d.stb, d.B);
 }
 
-void spmv_pair_any(const DeviceState& d, const double* in, double* mid, double* out, const double* base,
-                   const int* active, hipStream_t stream) {
-  if (d.f32)
-    spmv_pair<float, true>(d, in, mid, out, base, active, stream);
-  else
-    spmv_pair<double, true>(d, in, mid, out, base, active, stream);
+// Iteration products: x_raw = S^T (S rhs) (+ base, the defect-correction form).
+void spmv_iteration(const DeviceState& d, const double* base, const int* active, hipStream_t stream) {
+  if (!d.f32) {
+    spmv_one<double, int, double, double, true>(d, false, d.rhs, d.y, nullptr, active, stream);
+    spmv_one<double, int, double, double, true>(d, true, d.y, d.xr, base, active, stream);
+  } else if (d.col16) {
+    spmv_one<float, uint16_t, float, float, true>(d, false, d.rhs32, d.y32, nullptr, active, stream);
+    spmv_one<float, uint16_t, float, double, true>(d, true, d.y32, d.xr, base, active, stream);
+  } else {
+    spmv_one<float, int, float, float, true>(d, false, d.rhs32, d.y32, nullptr, active, stream);
+    spmv_one<float, int, float, double, true>(d, true, d.y32, d.xr, base, active, stream);
+  }
 }
 
 }  // namespace
@@ -1072,17 +1248,12 @@ void launch_begin_step(const DeviceState& d, const StepScalars& s, const float* 
 }
 
 void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const double* targets, hipStream_t stream) {
-  if (d.Q == 1)
-    hipLaunchKernelGGL(k_local_tet4, dim3(d.nec), dim3(kBlock), 0, stream, d, targets);
-  else
-    hipLaunchKernelGGL(k_local_hex8, dim3(d.nec), dim3(kBlock), 0, stream, d, targets);
-  hipLaunchKernelGGL(k_gather, dim3(d.nvc), dim3(kBlock), 0, stream, d);
-  spmv_pair_any(d, d.rhs, d.y, d.xr, d.defect ? d.x : nullptr, d.active, stream);
-  if (d.W > 0) {
-    hipLaunchKernelGGL(k_push_gram, dim3(d.nvc), dim3(kBlock), 0, stream, d);
-    hipLaunchKernelGGL(k_gram_solve, dim3(d.B), dim3(64), 0, stream, d, static_cast<const int*>(d.active));
-  }
-  hipLaunchKernelGGL(k_apply_conv, dim3(d.nvc), dim3(kBlock), 0, stream, d);
+  const size_t lds = prefix_lds(d);
+  hipLaunchKernelGGL(k_local, dim3(d.g_elem), dim3(kBlock), lds, stream, d, targets);
+  hipLaunchKernelGGL(k_gather, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
+  spmv_iteration(d, d.defect ? d.x : nullptr, d.active, stream);
+  if (d.W > 0) hipLaunchKernelGGL(k_push_gram, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
+  hipLaunchKernelGGL(k_apply_conv, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   hipLaunchKernelGGL(k_conv_check, dim3(d.B), dim3(64), 0, stream, d, s);
 }
 
@@ -1103,21 +1274,14 @@ void launch_adjoint_setup(const DeviceState& d, const StepScalars& s, const doub
 }
 
 void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStream_t stream) {
-  if (d.Q == 1)
-    hipLaunchKernelGGL(k_adj_local_tet4<0>, dim3(d.nec), dim3(kBlock), 0, stream, d,
-                       static_cast<const double*>(d.sol));
-  else
-    hipLaunchKernelGGL(k_adj_local_hex8<0>, dim3(d.nec), dim3(kBlock), 0, stream, d,
-                       static_cast<const double*>(d.sol));
-  hipLaunchKernelGGL(k_adj_gather, dim3(d.nvc), dim3(kBlock), 0, stream, d);
-  spmv_pair_any(d, d.rhs, d.y, d.xr, d.defect ? d.sol : nullptr, d.aactive, stream);
-  hipLaunchKernelGGL(k_adj_clip_partial, dim3(d.nvc), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_adj_clip_scale, dim3(d.B), dim3(64), 0, stream, d, 0);
-  hipLaunchKernelGGL(k_adj_push, dim3(d.nvc), dim3(kBlock), 0, stream, d);
-  if (d.W > 0) hipLaunchKernelGGL(k_gram_solve, dim3(d.B), dim3(64), 0, stream, d, static_cast<const int*>(d.aactive));
-  hipLaunchKernelGGL(k_adj_apply, dim3(d.nvc), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_adj_clip_scale, dim3(d.B), dim3(64), 0, stream, d, 1);
-  hipLaunchKernelGGL(k_adj_finish, dim3(d.nvc), dim3(kBlock), 0, stream, d);
+  const size_t lds = prefix_lds(d);
+  hipLaunchKernelGGL(k_adj_local, dim3(d.g_elem), dim3(kBlock), lds, stream, d);
+  hipLaunchKernelGGL(k_adj_gather, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
+  spmv_iteration(d, d.defect ? d.sol : nullptr, d.aactive, stream);
+  hipLaunchKernelGGL(k_adj_clip_partial, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
+  hipLaunchKernelGGL(k_adj_push, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
+  hipLaunchKernelGGL(k_adj_apply, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
+  hipLaunchKernelGGL(k_adj_finish, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   hipLaunchKernelGGL(k_adj_check, dim3(d.B), dim3(64), 0, stream, d, s);
 }
 
@@ -1128,12 +1292,7 @@ void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const do
   hipLaunchKernelGGL(k_adj_state_grads, dim3(d.nvc), dim3(kBlock), 0, stream, d, s.dt, gqd_out, gq_in, gqd_in,
                      gf_in);
   if (any_pinned) {
-    if (d.Q == 1)
-      hipLaunchKernelGGL(k_adj_local_tet4<1>, dim3(d.nec), dim3(kBlock), 0, stream, d,
-                         static_cast<const double*>(d.sol));
-    else
-      hipLaunchKernelGGL(k_adj_local_hex8<1>, dim3(d.nec), dim3(kBlock), 0, stream, d,
-                         static_cast<const double*>(d.sol));
+    hipLaunchKernelGGL(k_adj_coupling, dim3(d.nec), dim3(kBlock), 0, stream, d);
     hipLaunchKernelGGL(k_adj_dirichlet, dim3(d.nvc), dim3(kBlock), 0, stream, d, s.dt, gq_in, gqd_in, gdir);
   }
   if (gmat) {
@@ -1142,7 +1301,6 @@ void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const do
     else
       hipLaunchKernelGGL(k_adj_material_hex8, dim3(d.nec), dim3(kBlock), 0, stream, d, x);
   }
-  (void)gmat;
 }
 
 void launch_material_finish(const DeviceState& d, double* gmat, hipStream_t stream) {
@@ -1151,10 +1309,16 @@ void launch_material_finish(const DeviceState& d, double* gmat, hipStream_t stre
 }
 
 void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, hipStream_t stream) {
-  if (d.f32)
-    spmv_pair<float, false>(d, rhs, d.y, out, nullptr, d.ones, stream);
-  else
-    spmv_pair<double, false>(d, rhs, d.y, out, nullptr, d.ones, stream);
+  if (!d.f32) {
+    spmv_one<double, int, double, double, false>(d, false, rhs, d.y, nullptr, d.ones, stream);
+    spmv_one<double, int, double, double, false>(d, true, d.y, out, nullptr, d.ones, stream);
+  } else if (d.col16) {
+    spmv_one<float, uint16_t, double, float, false>(d, false, rhs, d.y32, nullptr, d.ones, stream);
+    spmv_one<float, uint16_t, float, double, false>(d, true, d.y32, out, nullptr, d.ones, stream);
+  } else {
+    spmv_one<float, int, double, float, false>(d, false, rhs, d.y32, nullptr, d.ones, stream);
+    spmv_one<float, int, float, double, false>(d, true, d.y32, out, nullptr, d.ones, stream);
+  }
 }
 
 void launch_eval_projections(int count, const double* f, const double* dirs, double mu, double lame, double penalty,

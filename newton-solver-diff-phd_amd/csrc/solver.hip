@@ -69,22 +69,34 @@ int pick_lanes(int len) {
   return lanes;
 }
 
-// Row binning of one CSR: rows grouped by lanes-per-row, ascending row id
-// within a group; one task per workgroup.
-void build_tasks(const std::vector<int>& off, std::vector<int>& rows, std::vector<SpmvTask>& tasks) {
-  const int n = static_cast<int>(off.size()) - 1;
+// Row binning of one CSR, body by body: within a body, rows are grouped by
+// lanes-per-row (ascending row id within a group); one task per workgroup.
+// task_base[b] is the first task of body b.
+void build_tasks(const std::vector<int>& off, const std::vector<int>& voff, std::vector<int>& rows,
+                 std::vector<SpmvTask>& tasks, std::vector<int>& task_base) {
   const int classes[] = {1, 2, 4, 8, 16, 32, 64, kBlock};
+  const int B = static_cast<int>(voff.size()) - 1;
   rows.clear();
   tasks.clear();
-  for (int L : classes) {
-    const int first = static_cast<int>(rows.size());
-    for (int r = 0; r < n; ++r)
-      if (pick_lanes(off[r + 1] - off[r]) == L) rows.push_back(r);
-    const int count = static_cast<int>(rows.size()) - first;
-    const int per = L == kBlock ? 1 : kBlock / L;
-    for (int k = 0; k < count; k += per) tasks.push_back(SpmvTask{first + k, std::min(per, count - k), L});
+  task_base.assign(B + 1, 0);
+  for (int b = 0; b < B; ++b) {
+    task_base[b] = static_cast<int>(tasks.size());
+    for (int L : classes) {
+      const int first = static_cast<int>(rows.size());
+      for (int r = voff[b]; r < voff[b + 1]; ++r)
+        if (pick_lanes(off[r + 1] - off[r]) == L) rows.push_back(r);
+      const int count = static_cast<int>(rows.size()) - first;
+      const int per = L == kBlock ? 1 : kBlock / L;
+      for (int k = 0; k < count; k += per)
+        tasks.push_back(SpmvTask{first + k, std::min(per, count - k), L, voff[b]});
+    }
   }
+  task_base[B] = static_cast<int>(tasks.size());
 }
+
+// Workgroups of a work-list kernel: enough to fill the device, never more
+// than there are items.
+int grid_for(int items) { return std::max(1, std::min(items, 2048)); }
 
 }  // namespace
 }  // namespace dphd
@@ -352,20 +364,23 @@ void build_solver(dphd_solver* s, double dt) {
       }
     }
   }
-  std::vector<int> s_rows, t_rows;
+  std::vector<int> s_rows, t_rows, stb, ttb;
   std::vector<SpmvTask> s_tasks, t_tasks;
-  build_tasks(s_off, s_rows, s_tasks);
-  build_tasks(t_off, t_rows, t_tasks);
+  build_tasks(s_off, s->voff, s_rows, s_tasks, stb);
+  build_tasks(t_off, s->voff, t_rows, t_tasks, ttb);
   // Chunks and incidence.
   std::vector<Chunk> vchunks, echunks;
-  std::vector<int> vcb(B + 1, 0);
+  std::vector<int> vcb(B + 1, 0), ecb(B + 1, 0);
   const int eper = hex ? kBlock / 8 : kBlock;
   for (int b = 0; b < B; ++b) {
     vcb[b] = static_cast<int>(vchunks.size());
+    ecb[b] = static_cast<int>(echunks.size());
     for (int i = s->voff[b]; i < s->voff[b + 1]; i += kBlock) vchunks.push_back(Chunk{b, i, std::min(i + kBlock, s->voff[b + 1])});
     for (int e = s->eoff[b]; e < s->eoff[b + 1]; e += eper) echunks.push_back(Chunk{b, e, std::min(e + eper, s->eoff[b + 1])});
   }
   vcb[B] = static_cast<int>(vchunks.size());
+  ecb[B] = static_cast<int>(echunks.size());
+  if (B + 1 > 8192) throw std::invalid_argument("at most 8191 bodies per solver (LDS work-list prefix)");
   std::vector<int> inc_off, inc_ent;
   build_incidence(N, E, K, s->indices.data(), inc_off, inc_ent);
   std::vector<uint8_t> pinmask(E, 0);
@@ -415,6 +430,22 @@ void build_solver(dphd_solver* s, double dt) {
     d.s_val = dupload(o, sv);
     d.t_val = dupload(o, tv);
     d.f32 = true;
+    // Body-local 16-bit columns when every body fits (6 bytes per nonzero).
+    int largest = 0;
+    for (int b = 0; b < B; ++b) largest = std::max(largest, s->voff[b + 1] - s->voff[b]);
+    if (largest <= 65536) {
+      std::vector<uint16_t> sc16(znnz), tc16(znnz);
+      for (int b = 0; b < B; ++b) {
+        const int v0 = s->voff[b];
+        for (int64_t p = s_off[v0]; p < s_off[s->voff[b + 1]]; ++p) sc16[p] = static_cast<uint16_t>(s_col[p] - v0);
+        for (int64_t p = t_off[v0]; p < t_off[s->voff[b + 1]]; ++p) tc16[p] = static_cast<uint16_t>(t_col[p] - v0);
+      }
+      d.s_col16 = dupload(o, sc16);
+      d.t_col16 = dupload(o, tc16);
+      d.col16 = true;
+    }
+    d.rhs32 = dzeros<float>(o, static_cast<size_t>(N) * 3);
+    d.y32 = dzeros<float>(o, static_cast<size_t>(N) * 3);
     d.a_off = dupload(o, a_off);
     d.a_col = dupload(o, a_col);
     d.a_val = dupload(o, a_val);
@@ -429,6 +460,8 @@ void build_solver(dphd_solver* s, double dt) {
   d.t_tasks = dupload(o, t_tasks);
   d.s_ntasks = static_cast<int>(s_tasks.size());
   d.t_ntasks = static_cast<int>(t_tasks.size());
+  d.stb = dupload(o, stb);
+  d.ttb = dupload(o, ttb);
   const size_t v3 = static_cast<size_t>(N) * 3;
   d.q0 = dzeros<double>(o, v3);
   d.qt = dzeros<double>(o, v3);
@@ -445,16 +478,20 @@ void build_solver(dphd_solver* s, double dt) {
     d.hr = dzeros<double>(o, static_cast<size_t>(d.W + 1) * v3);
   }
   d.aa = dzeros<int>(o, 4 * static_cast<size_t>(B));
-  d.accept = dzeros<int>(o, B);
-  d.gamma = dzeros<double>(o, static_cast<size_t>(B) * kMaxWindow);
   d.vchunks = dupload(o, vchunks);
   d.nvc = static_cast<int>(vchunks.size());
   d.vcb = dupload(o, vcb);
   d.echunks = dupload(o, echunks);
   d.nec = static_cast<int>(echunks.size());
+  d.ecb = dupload(o, ecb);
+  d.g_vert = grid_for(d.nvc);
+  d.g_elem = grid_for(d.nec);
+  d.g_stask = grid_for(d.s_ntasks);
+  d.g_ttask = grid_for(d.t_ntasks);
   d.gram_part = dzeros<double>(o, static_cast<size_t>(d.nvc) * kGramEntries);
   d.conv_part = dzeros<double>(o, static_cast<size_t>(d.nvc) * 4);
   d.clip_part = dzeros<double>(o, d.nvc);
+  d.clip_part2 = dzeros<double>(o, d.nvc);
   d.iter = dzeros<int>(o, B);
   d.active = dzeros<int>(o, B);
   d.status = dzeros<int>(o, 1);
