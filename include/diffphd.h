@@ -97,7 +97,7 @@ typedef struct {
   int bodies, vertices, elements, samples_per_element, corners;
   int64_t factor_nnz;       /* nnz(S) summed over bodies (S^T has the same) */
   int64_t matrix_nnz;       /* nnz(A) per coordinate summed over bodies */
-  int64_t jacobian_bytes;   /* bytes of the stored adjoint Jacobian */
+  int64_t jacobian_bytes;   /* device bytes of the adjoint's Jacobian, element blocks and assembled operator */
   double setup_seconds;     /* host time of the last dphd_build */
 } dphd_info;
 

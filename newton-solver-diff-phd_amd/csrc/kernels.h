@@ -111,6 +111,18 @@ struct DeviceState {
   int* any_active = nullptr;  // mapped host int
   // adjoint
   double* jac = nullptr;     // [E*Q*81]
+  // Assembled adjoint operator K (minus A in the defect-correction mode):
+  // 3x3 blocks in a sliced-ELL layout whose slices are the vertex chunks.
+  // Slot s of row i of chunk c is block kbase[c] + s*256 + (i - begin); its
+  // nine values are slot-major, kval[9*kbase[c] + (9*s + t)*256 + lane], so
+  // a wavefront reads 64 consecutive values per (slot, entry).
+  int* kwidth = nullptr;     // [nvc] slots per row (the chunk's longest row)
+  int* kbase = nullptr;      // [nvc] first block of the chunk
+  int* kcol = nullptr;       // [kblocks] column vertex, -1 = padding (rows end with padding)
+  double* kdiag = nullptr;   // [kblocks] A_ij (defect mode only)
+  double* ke = nullptr;      // [E*K*K*9] element blocks sum_s G_s^T J_s G_s
+  double* kval = nullptr;    // [kblocks*9]
+  int kblocks = 0;
   double *arhs = nullptr, *sol = nullptr;  // [N*3]
   int* aiter = nullptr;      // [B]
   int* aactive = nullptr;    // [B]

@@ -710,48 +710,40 @@ __device__ __forceinline__ M3 jac_apply(const DeviceState& d, int sample, const 
   return ds;
 }
 
-// Element rows of K mu (MODE 0) or of the Dirichlet coupling (A - K) mu
-// (MODE 1, elements with a pinned corner only).
-template <int MODE>
-__device__ __forceinline__ void adj_local_tet4_chunk(const DeviceState& d, int ci, int b, const double* mu) {
+// Element rows of the Dirichlet coupling (A - K) mu, for elements with a
+// pinned corner (adjoint.py:140-155 with the boundary rows of :474-520).
+__device__ __forceinline__ void coupling_tet4_chunk(const DeviceState& d, int ci, int b, const double* mu) {
   const Chunk c = d.echunks[ci];
   const int e = c.begin + threadIdx.x;
   if (e >= c.end) return;
-  if (MODE == 1 && d.pinmask[e] == 0) return;
+  if (d.pinmask[e] == 0) return;
+  if (d.route[2 * b + 1] != 0.0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) st3(d.contrib, 4 * e + k, V3{0, 0, 0});
+    return;
+  }
   V3 g[4];
   tet_grads(d.poses, e, g);
   const int* nd = d.idx + 4 * e;
   M3 df = zero3();
 #pragma unroll
   for (int k = 0; k < 4; ++k) add_outer(df, ld3(mu, nd[k]), g[k]);
-  M3 ds;
-  if (MODE == 0) {
-    ds = jac_apply(d, e, df);
-  } else {
-    if (d.route[2 * b + 1] != 0.0) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) st3(d.contrib, 4 * e + k, V3{0, 0, 0});
-      return;
-    }
-    ds = scale(d.wc[e] + d.wv[e] + d.wn[e], df);
-    if (d.has_muscles) {
-      const V3 m = ld3(d.mdir, e);
-      ds = add(ds, scale(d.mw[e], outer(mul(df, m), m)));
-    }
-    ds = sub(ds, jac_apply(d, e, df));
+  M3 ds = scale(d.wc[e] + d.wv[e] + d.wn[e], df);
+  if (d.has_muscles) {
+    const V3 m = ld3(d.mdir, e);
+    ds = add(ds, scale(d.mw[e], outer(mul(df, m), m)));
   }
+  ds = sub(ds, jac_apply(d, e, df));
 #pragma unroll
   for (int k = 0; k < 4; ++k) st3(d.contrib, 4 * e + k, mul(ds, g[k]));
 }
 
-template <int MODE>
-__device__ __forceinline__ void adj_local_hex8_chunk(const DeviceState& d, int ci, int b, const double* mu) {
+__device__ __forceinline__ void coupling_hex8_chunk(const DeviceState& d, int ci, int b, const double* mu) {
   const Chunk c = d.echunks[ci];
   const int s = threadIdx.x & 7;
   const int e = c.begin + (threadIdx.x >> 3);
-  bool valid = e < c.end;
-  if (MODE == 1 && valid && d.pinmask[e] == 0) valid = false;
-  const bool zero = MODE == 1 && d.route[2 * b + 1] != 0.0;
+  const bool valid = e < c.end && d.pinmask[e] != 0;
+  const bool zero = d.route[2 * b + 1] != 0.0;
   double out[8][3];
 #pragma unroll
   for (int j = 0; j < 8; ++j) out[j][0] = out[j][1] = out[j][2] = 0.0;
@@ -760,17 +752,12 @@ __device__ __forceinline__ void adj_local_hex8_chunk(const DeviceState& d, int c
     M3 df = zero3();
 #pragma unroll
     for (int j = 0; j < 8; ++j) add_outer(df, ld3(mu, nd[j]), hex_grad(d, b, s, j));
-    M3 ds;
-    if (MODE == 0) {
-      ds = jac_apply(d, 8 * e + s, df);
-    } else {
-      ds = scale(d.wc[e] + d.wv[e] + d.wn[e], df);
-      if (d.has_muscles) {
-        const V3 m = ld3(d.mdir, e);
-        ds = add(ds, scale(d.mw[e], outer(mul(df, m), m)));
-      }
-      ds = sub(ds, jac_apply(d, 8 * e + s, df));
+    M3 ds = scale(d.wc[e] + d.wv[e] + d.wn[e], df);
+    if (d.has_muscles) {
+      const V3 m = ld3(d.mdir, e);
+      ds = add(ds, scale(d.mw[e], outer(mul(df, m), m)));
     }
+    ds = sub(ds, jac_apply(d, 8 * e + s, df));
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const V3 r = mul(ds, hex_grad(d, b, s, j));
@@ -783,40 +770,142 @@ __device__ __forceinline__ void adj_local_hex8_chunk(const DeviceState& d, int c
   if (valid) st3(d.contrib, 8 * e + s, sum);
 }
 
-__global__ __launch_bounds__(kBlock) void k_adj_local(DeviceState d) {
-  for_active(d.aactive, d.ecb, d.B, [&](int ci, int b) {
-    if (d.Q == 1)
-      adj_local_tet4_chunk<0>(d, ci, b, d.sol);
-    else
-      adj_local_hex8_chunk<0>(d, ci, b, d.sol);
-  });
-}
-
 // Dirichlet coupling rows over every element chunk (once per backward).
 __global__ __launch_bounds__(kBlock) void k_adj_coupling(DeviceState d) {
   const int ci = blockIdx.x;
   const int b = d.echunks[ci].body;
   if (d.Q == 1)
-    adj_local_tet4_chunk<1>(d, ci, b, d.sol);
+    coupling_tet4_chunk(d, ci, b, d.sol);
   else
-    adj_local_hex8_chunk<1>(d, ci, b, d.sol);
+    coupling_hex8_chunk(d, ci, b, d.sol);
 }
 
-// work = sum(contrib) + rhs (gather with accumulate, adjoint.py:939).
-__global__ __launch_bounds__(kBlock) void k_adj_gather(DeviceState d) {
+// The adjoint's element term mu -> sum_e G_e^T J_e (G_e mu) (adjoint.py:140-155)
+// is linear in mu and fixed for the whole backward, so it is assembled once
+// into 3x3 vertex blocks and every iteration applies it as one sparse
+// product.
+//
+// Row block k of one element: K_e[k][l] = sum_s sum_{j,n} g_sk[j] g_sl[n]
+// J_s[(beta, n) -> (alpha, j)], with J stored as jac[(c * 9 + t)] for
+// c = beta + 3 n (entry of dF) and t = 3 alpha + j (entry of the stress
+// change), the convention of jac_apply. One thread per (element, corner).
+__global__ __launch_bounds__(kBlock) void k_element_operator(DeviceState d) {
+  const int K = d.K;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= d.E * K) return;
+  const int e = tid / K, k = tid % K;
+  const int b = d.ebody[e];
+  const size_t stride = static_cast<size_t>(d.E) * d.Q;
+  double blk[8][9];
+#pragma unroll
+  for (int l = 0; l < 8; ++l)
+#pragma unroll
+    for (int t = 0; t < 9; ++t) blk[l][t] = 0.0;
+  V3 tg[4];
+  if (d.Q == 1) tet_grads(d.poses, e, tg);
+  for (int s = 0; s < d.Q; ++s) {
+    const size_t sample = static_cast<size_t>(e) * d.Q + s;
+    const V3 gk = d.Q == 1 ? tg[k] : hex_grad(d, b, s, k);
+    const double gkv[3] = {gk.x, gk.y, gk.z};
+    // p[c][alpha] = sum_j g_k[j] J[c][3 alpha + j]
+    double p[9][3];
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) t += gkv[j] * d.jac[(c * 9 + 3 * a + j) * stride + sample];
+        p[c][a] = t;
+      }
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      if (l >= K) break;
+      const V3 gl = d.Q == 1 ? tg[l & 3] : hex_grad(d, b, s, l);
+      const double glv[3] = {gl.x, gl.y, gl.z};
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int be = 0; be < 3; ++be) {
+          double t = blk[l][3 * a + be];
+#pragma unroll
+          for (int n = 0; n < 3; ++n) t += glv[n] * p[be + 3 * n][a];
+          blk[l][3 * a + be] = t;
+        }
+    }
+  }
+  double* out = d.ke + static_cast<size_t>(tid) * K * 9;
+#pragma unroll
+  for (int l = 0; l < 8; ++l) {
+    if (l >= K) break;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) out[9 * l + t] = blk[l][t];
+  }
+}
+
+// Rows of the assembled operator, one thread per row of a vertex chunk:
+// block (i, j) = sum over the incident elements of i, in incidence order, of
+// K_e[k][l] with idx[e][k] = i and idx[e][l] = j; minus A_ij I in the defect
+// mode.
+__global__ __launch_bounds__(kBlock) void k_operator_assemble(DeviceState d) {
+  const int ci = blockIdx.x;
+  const Chunk c = d.vchunks[ci];
+  const int i = c.begin + threadIdx.x;
+  const int w = d.kwidth[ci];
+  const size_t base = static_cast<size_t>(d.kbase[ci]);
+  const int K = d.K;
+  for (int s = 0; s < w; ++s) {
+    const size_t blk = base + static_cast<size_t>(s) * kBlock + threadIdx.x;
+    double acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[t] = 0.0;
+    const int j = i < c.end ? d.kcol[blk] : -1;
+    if (j >= 0) {
+      for (int a = d.inc_off[i]; a < d.inc_off[i + 1]; ++a) {
+        const int ent = d.inc_ent[a];
+        const int e = ent / K;
+        for (int l = 0; l < K; ++l) {
+          if (d.idx[e * K + l] != j) continue;
+          const double* kb = d.ke + (static_cast<size_t>(ent) * K + l) * 9;
+#pragma unroll
+          for (int t = 0; t < 9; ++t) acc[t] += kb[t];
+        }
+      }
+      if (d.defect) {
+        const double a = d.kdiag[blk];
+        acc[0] -= a;
+        acc[4] -= a;
+        acc[8] -= a;
+      }
+    }
+    double* out = d.kval + 9 * base + static_cast<size_t>(s) * 9 * kBlock + threadIdx.x;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) out[t * kBlock] = acc[t];
+  }
+}
+
+// work = K mu + rhs (gather with accumulate, adjoint.py:939), or
+// rhs + (K - A) mu in the defect mode; one thread per row.
+__global__ __launch_bounds__(kBlock) void k_adj_rhs(DeviceState d) {
   for_active(d.aactive, d.vcb, d.B, [&](int ci, int) {
     const Chunk c = d.vchunks[ci];
     const int i = c.begin + threadIdx.x;
     if (i >= c.end) return;
-    V3 t{0, 0, 0};
-    for (int a = d.inc_off[i]; a < d.inc_off[i + 1]; ++a) t = t + ld3(d.contrib, d.inc_ent[a]);
-    t = t + ld3(d.arhs, i);
-    if (d.defect) {
-      V3 ax{0, 0, 0};
-      for (int q = d.a_off[i]; q < d.a_off[i + 1]; ++q) ax = ax + d.a_val[q] * ld3(d.sol, d.a_col[q]);
-      t = t - ax;
+    const int w = d.kwidth[ci];
+    const size_t base = static_cast<size_t>(d.kbase[ci]);
+    const int* col = d.kcol + base + threadIdx.x;
+    const double* val = d.kval + 9 * base + threadIdx.x;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    for (int s = 0; s < w; ++s) {
+      const int j = col[static_cast<size_t>(s) * kBlock];
+      if (j < 0) break;
+      const V3 m = ld3(d.sol, j);
+      const double* v = val + static_cast<size_t>(s) * 9 * kBlock;
+      t0 += v[0 * kBlock] * m.x + v[1 * kBlock] * m.y + v[2 * kBlock] * m.z;
+      t1 += v[3 * kBlock] * m.x + v[4 * kBlock] * m.y + v[5 * kBlock] * m.z;
+      t2 += v[6 * kBlock] * m.x + v[7 * kBlock] * m.y + v[8 * kBlock] * m.z;
     }
-    store_rhs(d, i, t);
+    store_rhs(d, i, V3{t0, t1, t2} + ld3(d.arhs, i));
   });
 }
 
@@ -1271,12 +1360,14 @@ void launch_adjoint_setup(const DeviceState& d, const StepScalars& s, const doub
   hipLaunchKernelGGL(k_adj_seed, dim3(d.nvc), dim3(kBlock), 0, stream, d, s.dt, gq_out, gqd_out);
   const int total = d.E * d.Q;
   hipLaunchKernelGGL(k_jacobian, dim3((total + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d, x, targets);
+  const int rows = d.E * d.K;
+  hipLaunchKernelGGL(k_element_operator, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_operator_assemble, dim3(d.nvc), dim3(kBlock), 0, stream, d);
 }
 
 void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStream_t stream) {
   const size_t lds = prefix_lds(d);
-  hipLaunchKernelGGL(k_adj_local, dim3(d.g_elem), dim3(kBlock), lds, stream, d);
-  hipLaunchKernelGGL(k_adj_gather, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
+  hipLaunchKernelGGL(k_adj_rhs, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   spmv_iteration(d, d.defect ? d.sol : nullptr, d.aactive, stream);
   hipLaunchKernelGGL(k_adj_clip_partial, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   hipLaunchKernelGGL(k_adj_push, dim3(d.g_vert), dim3(kBlock), lds, stream, d);

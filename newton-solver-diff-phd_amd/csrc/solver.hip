@@ -387,6 +387,50 @@ void build_solver(dphd_solver* s, double dt) {
   for (int e = 0; e < E; ++e)
     for (int k = 0; k < K; ++k)
       if (!free[s->indices[static_cast<size_t>(e) * K + k]]) pinmask[e] |= static_cast<uint8_t>(1u << k);
+  // Sliced-ELL pattern of the adjoint operator: row i holds itself and every
+  // vertex that shares an element with it, ascending.
+  const int nvc_host = static_cast<int>(vchunks.size());
+  std::vector<int> kwidth(nvc_host), kbase(nvc_host);
+  std::vector<int> kcol;
+  std::vector<double> kdiag;
+  {
+    std::vector<std::vector<int>> pattern(kBlock);
+    int64_t blocks = 0;
+    for (int ci = 0; ci < nvc_host; ++ci) {
+      const Chunk c = vchunks[ci];
+      int width = 0;
+      for (int i = c.begin; i < c.end; ++i) {
+        std::vector<int>& row = pattern[i - c.begin];
+        row.assign(1, i);
+        for (int a = inc_off[i]; a < inc_off[i + 1]; ++a) {
+          const int e = inc_ent[a] / K;
+          for (int l = 0; l < K; ++l) row.push_back(s->indices[static_cast<size_t>(e) * K + l]);
+        }
+        std::sort(row.begin(), row.end());
+        row.erase(std::unique(row.begin(), row.end()), row.end());
+        width = std::max(width, static_cast<int>(row.size()));
+      }
+      if (blocks + static_cast<int64_t>(width) * kBlock > INT32_MAX / 9)
+        throw std::invalid_argument("adjoint operator exceeds int32 block indexing");
+      kwidth[ci] = width;
+      kbase[ci] = static_cast<int>(blocks);
+      blocks += static_cast<int64_t>(width) * kBlock;
+      kcol.resize(blocks, -1);
+      if (defect) kdiag.resize(blocks, 0.0);
+      for (int i = c.begin; i < c.end; ++i) {
+        const std::vector<int>& row = pattern[i - c.begin];
+        const size_t lane = static_cast<size_t>(i - c.begin);
+        for (size_t t = 0; t < row.size(); ++t) kcol[kbase[ci] + t * kBlock + lane] = row[t];
+        if (defect) {
+          for (int p = a_off[i]; p < a_off[i + 1]; ++p) {
+            const auto it = std::lower_bound(row.begin(), row.end(), a_col[p]);
+            if (it == row.end() || *it != a_col[p]) throw std::logic_error("matrix entry outside the element pattern");
+            kdiag[kbase[ci] + static_cast<size_t>(it - row.begin()) * kBlock + lane] = a_val[p];
+          }
+        }
+      }
+    }
+  }
 
   // Upload.
   auto& o = s->owned;
@@ -505,7 +549,12 @@ void build_solver(dphd_solver* s, double dt) {
   d.valid = dzeros<int>(o, B);
   d.ones = dupload(o, std::vector<int>(B, 1));
   d.eoff = dupload(o, s->eoff);
-  d.jac = nullptr;  // allocated on the first backward
+  d.jac = nullptr;  // allocated on the first backward, with ke and kval
+  d.kwidth = dupload(o, kwidth);
+  d.kbase = dupload(o, kbase);
+  d.kcol = dupload(o, kcol);
+  if (defect) d.kdiag = dupload(o, kdiag);
+  d.kblocks = static_cast<int>(kcol.size());
   d.material = pd ? 0 : 1;
   d.has_muscles = muscles ? 1 : 0;
   d.integrate = s->cfg.integrate_pinned_velocities ? 1 : 0;
@@ -525,7 +574,9 @@ void build_solver(dphd_solver* s, double dt) {
   s->built = true;
   s->fwd_key_iter = s->adj_key_iter = -1;
   s->last_fwd_iters = s->last_adj_iters = 0;
-  int64_t jac_bytes = static_cast<int64_t>(E) * Q * 81 * 8;
+  // Jacobian, element operator blocks and the assembled operator.
+  const int64_t jac_bytes = static_cast<int64_t>(E) * Q * 81 * 8 + static_cast<int64_t>(E) * K * K * 9 * 8 +
+                            static_cast<int64_t>(d.kblocks) * 9 * 8;
   s->info.bodies = B;
   s->info.vertices = N;
   s->info.elements = E;
@@ -846,6 +897,8 @@ int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle
     HIP_CHECK(hipSetDevice(s->device));
     if (!s->d.jac) {
       s->d.jac = dphd::dalloc<double>(s->owned, static_cast<size_t>(s->E) * s->Q * 81);
+      s->d.ke = dphd::dalloc<double>(s->owned, static_cast<size_t>(s->E) * s->K * s->K * 9);
+      s->d.kval = dphd::dalloc<double>(s->owned, static_cast<size_t>(s->d.kblocks) * 9);
       if (s->adj_exec) {
         HIP_CHECK(hipGraphExecDestroy(s->adj_exec));
         s->adj_exec = nullptr;
