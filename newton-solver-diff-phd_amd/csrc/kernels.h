@@ -5,7 +5,9 @@
 //   elements   [E]      indices [E,K] int32, Tet4 poses [E,9] float32 (exact
 //                       source of the gradients), Hex8 gradient table per body
 //                       [B,8,8,3], per-element weights float64
-//   factor     S, S^T   CSR with global row/column ids, values float64 or float32
+//   factor     S        dense supernode blocks over each body's elimination-tree
+//                       postorder (a row of S is one dense column range); both
+//                       products read the same blocks; float64 or float32
 //   history    [W+1,N]  Anderson ring: g = x_raw - x_prev and raw = x_prev + g
 //   chunks     256-vertex / 256-thread element chunks never straddle bodies;
 //              per-chunk partial sums are reduced per body in chunk order, so
@@ -26,13 +28,47 @@ constexpr int kBlock = 256;
 constexpr int kMaxWindow = 8;
 constexpr int kGramEntries = kMaxWindow * (kMaxWindow + 1) / 2 + kMaxWindow;  // upper triangle + rhs
 
-// One workgroup of a CSR product: rows row_list[first, first + count) with
-// `lanes` lanes per row (1..64, or 256 for one row per workgroup).
-struct SpmvTask {
-  int first;
+// The inverse factor on the device: dense supernode blocks over postorder
+// slots. A block holds rows [row0, row0 + rows) by stored columns [col0,
+// col0 + 4*quads), row-major from its value offset (zero where the factor is
+// zero).
+
+// Rows of one S-product work item (one wavefront) and columns of one
+// S^T-product work item (one workgroup: 64 lanes x 4 columns).
+constexpr int kRowsPerGroup = 8;
+constexpr int kColsPerGroup = 256;
+
+// S product work item (one wavefront): rows out_row .. out_row + count - 1
+// (count <= kRowsPerGroup) of one block, whose values start at `offset`
+// with 4*quads values per row from column col0. Self-contained, so a
+// wavefront needs one uniform load per item.
+struct RowItem {
+  long long offset;
+  int col0;
+  int quads;
+  int out_row;
   int count;
-  int lanes;
-  int base;  // first vertex of the task's body (16-bit columns are body-local)
+  int pad[2];
+};
+
+// One block as the S^T column groups that it covers see it.
+struct ColCover {
+  long long offset;
+  int row0;
+  int rows;
+  int col0;
+  int quads;
+  int pad[2];
+};
+
+// S^T product work item (one workgroup): the 256 columns [c0, c0 + 256) of
+// one body (columns from `end` on belong to the next body), over the blocks
+// covers[b0 .. b1) that cover them.
+struct ColGroup {
+  int c0;
+  int b0;
+  int b1;
+  int end;
 };
 
 struct Chunk {
@@ -65,25 +101,31 @@ struct DeviceState {
   double* gravity = nullptr;  // [B*3]
   double* nh = nullptr;       // [B*3] mean mu, mean lambda, penalty
   double* svol = nullptr;     // [B] sample volume
-  // factor
-  int *s_off = nullptr, *s_col = nullptr, *t_off = nullptr, *t_col = nullptr;
-  void *s_val = nullptr, *t_val = nullptr;  // double or float
-  int* s_rows = nullptr;                    // row lists of the tasks
-  int* t_rows = nullptr;
-  SpmvTask* s_tasks = nullptr;
-  SpmvTask* t_tasks = nullptr;
-  int s_ntasks = 0, t_ntasks = 0;
-  bool f32 = false;                         // FP32 S values
-  uint16_t *s_col16 = nullptr, *t_col16 = nullptr;  // body-local columns (FP32 mode, bodies <= 65536 vertices)
-  bool col16 = false;
+  // inverse factor as supernode blocks over postorder slots (each body's
+  // slots start at a multiple of 4; gap slots hold zeros)
+  int slots = 0;              // postorder slots of all bodies
+  int* gpost = nullptr;       // [N] original vertex -> slot
+  int* gorig = nullptr;       // [slots] slot -> original vertex, -1 for a gap
+  void* fval = nullptr;       // block values, double or float
+  RowItem* s_items = nullptr;
+  int s_nitems = 0;
+  int* s_tb = nullptr;        // [B+1] S items of each body
+  ColGroup* t_items = nullptr;
+  int t_nitems = 0;
+  int* t_tb = nullptr;        // [B+1] S^T items of each body
+  ColCover* t_covers = nullptr;  // blocks of the column groups
+  int g_s = 0, g_t = 0;       // grids of the two products
+  bool f32 = false;           // FP32 factor values
   // constant matrix for the defect correction (per coordinate)
   int *a_off = nullptr, *a_col = nullptr;
   double* a_val = nullptr;
   // vectors [N*3]
-  double *q0 = nullptr, *qt = nullptr, *xd = nullptr, *x = nullptr, *xr = nullptr, *p = nullptr, *rhs = nullptr,
-         *y = nullptr;
+  double *q0 = nullptr, *qt = nullptr, *xd = nullptr, *x = nullptr, *xr = nullptr, *p = nullptr;
   double* contrib = nullptr;  // [E*K*3]
-  float *rhs32 = nullptr, *y32 = nullptr;  // FP32 mode: residual and S-product vectors [N*3]
+  // Right-hand side of the S product and the S product itself over the
+  // postorder slots, 4-wide ([(slots+4)*4], fourth lane zero) so that a quad
+  // of columns reads 64 (float) or 128 (double) contiguous bytes.
+  void *rhs4 = nullptr, *y4 = nullptr;
   // Anderson
   int W = 0;
   double *hg = nullptr, *hr = nullptr;  // [(W+1)*N*3]
@@ -95,11 +137,11 @@ struct DeviceState {
   Chunk* echunks = nullptr;
   int nec = 0;
   int* ecb = nullptr;  // [B+1] element-chunk range of each body
-  int* stb = nullptr;  // [B+1] S task range of each body
-  int* ttb = nullptr;  // [B+1] S^T task range of each body
-  // Fixed grid sizes of the work-list kernels (each workgroup loops over
-  // the items of the still-active bodies).
-  int g_vert = 0, g_elem = 0, g_stask = 0, g_ttask = 0;
+  // Fixed grid sizes of the work-list kernels (each workgroup, or each
+  // wavefront for the factor products, loops over the items of the
+  // still-active bodies); multiples of 8 once above 8, so that every XCD
+  // holds the same number of workgroups.
+  int g_vert = 0, g_elem = 0;
   double* gram_part = nullptr;  // [nvc*kGramEntries]
   double* conv_part = nullptr;  // [nvc*4]
   double* clip_part = nullptr;  // [nvc]

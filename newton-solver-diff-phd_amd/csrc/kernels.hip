@@ -105,26 +105,77 @@ __device__ __forceinline__ int find_body(const int* pref, int B, int v) {
   return lo;
 }
 
-// Loop `fn(item, body)` over the active items with a static grid stride; the
-// whole block iterates together, so fn may use block barriers.
+// Static split of `total` work-list items over the grid's workers
+// (`workers` per workgroup). Workgroups are dispatched round-robin over the
+// 8 XCDs (blockIdx % 8), so when the grid is a multiple of 8 each XCD takes
+// one contiguous eighth of the items: the neighbouring items of a body, which
+// read the same vertex data, then share one L2.
+struct ItemRange {
+  int begin, end, first, stride;
+};
+
+__device__ __forceinline__ ItemRange xcd_items(int total, int workers, int worker) {
+  const int g = static_cast<int>(gridDim.x);
+  if (g % 8 != 0) return ItemRange{0, total, static_cast<int>(blockIdx.x) * workers + worker, g * workers};
+  const int x = static_cast<int>(blockIdx.x) % 8, r = static_cast<int>(blockIdx.x) / 8;
+  const long long t = total;
+  return ItemRange{static_cast<int>(t * x / 8), static_cast<int>(t * (x + 1) / 8), r * workers + worker,
+                   (g / 8) * workers};
+}
+
+// Loop `fn(item, body)` over the active items; the whole block iterates
+// together (block-uniform bounds), so fn may use block barriers.
 template <typename Fn>
 __device__ __forceinline__ void for_active(const int* active, const int* base, int B, Fn fn) {
   extern __shared__ int pref[];
   const int total = active_prefix(active, base, B, pref);
-  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+  const ItemRange r = xcd_items(total, 1, 0);
+  for (int v = r.begin + r.first; v < r.end; v += r.stride) {
     const int b = find_body(pref, B, v);
     fn(base[b] + (v - pref[b]), b);
   }
 }
 
-// Right-hand side of the S product: FP64, or FP32 in the FP32-factor mode.
+// Wavefront-granular work list: every wavefront takes its own items
+// (stride over all wavefronts of the grid), and fn must not use block
+// barriers. fn(item, next) also receives the wavefront's next item (or -1),
+// so that it can prefetch that item's descriptor.
+template <typename Fn>
+__device__ __forceinline__ void for_active_waves(const int* active, const int* base, int B, Fn fn) {
+  extern __shared__ int pref[];
+  const int total = active_prefix(active, base, B, pref);
+  const int workers = kBlock / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const int stride = static_cast<int>(gridDim.x) * workers;
+  int v = static_cast<int>(blockIdx.x) * workers + wave;
+  if (v >= total) return;
+  int b = find_body(pref, B, v);
+  int item = base[b] + (v - pref[b]);
+  while (true) {
+    const int vn = v + stride;
+    int next = -1;
+    if (vn < total) {
+      while (pref[b + 1] <= vn) ++b;
+      next = base[b] + (vn - pref[b]);
+    }
+    fn(item, next);
+    if (next < 0) break;
+    v = vn;
+    item = next;
+  }
+}
+
+// Right-hand side of the S product, 4-wide at the vertex's postorder index:
+// float in the FP32-factor mode, double otherwise.
 __device__ __forceinline__ void store_rhs(const DeviceState& d, int i, V3 v) {
-  if (d.rhs32) {
-    d.rhs32[3 * i] = static_cast<float>(v.x);
-    d.rhs32[3 * i + 1] = static_cast<float>(v.y);
-    d.rhs32[3 * i + 2] = static_cast<float>(v.z);
+  const size_t r = static_cast<size_t>(d.gpost[i]);
+  if (d.f32) {
+    static_cast<float4*>(d.rhs4)[r] =
+        make_float4(static_cast<float>(v.x), static_cast<float>(v.y), static_cast<float>(v.z), 0.0f);
   } else {
-    st3(d.rhs, i, v);
+    double2* p = static_cast<double2*>(d.rhs4) + 2 * r;
+    p[0] = make_double2(v.x, v.y);
+    p[1] = make_double2(v.z, 0.0);
   }
 }
 
@@ -348,79 +399,215 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d) {
 }
 
 // ---------------------------------------------------------------------------
-// CSR products y = M x (+ base). Template parameters: Real the stored value
-// type, Col the column type (int32 global ids, or uint16 ids local to the
-// task's body), In / Out the vector element types. Accumulation is FP64.
-// PRED = true restricts the work list to the tasks of bodies that are still
-// iterating; PRED = false is the plain product of every row (one workgroup
-// per task, SparseInverse.apply).
+// Factor products y = S x and z = S^T y (+ base), sparse.py:209-227, over
+// the dense supernode blocks. In both products a lane owns four consecutive
+// columns of a block (one quad: a 16-byte float or 32-byte double value
+// load per row, read nontemporally so that the vectors stay in L2):
+//  - S: a wavefront takes up to 8 rows of one block, keeps the inputs of its
+//    columns in registers across the rows, and reduces each row over the
+//    lanes at the end (a butterfly over the lanes the block spans);
+//  - S^T: a workgroup takes 256 columns of one body and walks every block
+//    that covers them; the row value y_r is uniform, each lane accumulates
+//    its own columns, and the four wavefronts (rows r = w mod 4) are summed
+//    in LDS in wavefront order.
+// Accumulation is in Acc (FP64, or FP32 in the FP32-factor mode, whose
+// products only feed the defect correction); every sum has a fixed order,
+// so results are bitwise repeatable. Values the factor does not hold are
+// stored zeros and multiply vertices of the same elimination tree or zero
+// gap slots, never another body's values.
 
-template <typename Real, typename Col, typename In, typename Out>
-__device__ __forceinline__ void spmv_task(const SpmvTask t, const int* rows, const int* off, const Col* col,
-                                          const Real* val, const In* in, Out* out, const double* base,
-                                          double* smem, double* res) {
-  const int cb = sizeof(Col) == 2 ? t.base : 0;
-  const In* xin = in + 3 * cb;
-  if (t.lanes == kBlock) {
-    const int row = rows[t.first];
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int q = off[row] + threadIdx.x; q < off[row + 1]; q += kBlock) {
-      const double v = static_cast<double>(val[q]);
-      const In* x = xin + 3 * static_cast<int>(col[q]);
-      acc[0] += v * static_cast<double>(x[0]);
-      acc[1] += v * static_cast<double>(x[1]);
-      acc[2] += v * static_cast<double>(x[2]);
-    }
-    block_sum<3>(acc, smem, res);
-    if (threadIdx.x == 0) {
-      V3 r{res[0], res[1], res[2]};
-      if (base) r = r + ld3(base, row);
-      out[3 * row] = static_cast<Out>(r.x);
-      out[3 * row + 1] = static_cast<Out>(r.y);
-      out[3 * row + 2] = static_cast<Out>(r.z);
-    }
-    return;
-  }
-  const int L = t.lanes;
-  const int group = threadIdx.x / L, lane = threadIdx.x % L;
-  const int row = group < t.count ? rows[t.first + group] : -1;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  if (row >= 0) {
-    for (int q = off[row] + lane; q < off[row + 1]; q += L) {
-      const double v = static_cast<double>(val[q]);
-      const In* x = xin + 3 * static_cast<int>(col[q]);
-      a0 += v * static_cast<double>(x[0]);
-      a1 += v * static_cast<double>(x[1]);
-      a2 += v * static_cast<double>(x[2]);
-    }
-  }
-  for (int o = L / 2; o > 0; o >>= 1) {
-    a0 += __shfl_xor(a0, o, kWave);
-    a1 += __shfl_xor(a1, o, kWave);
-    a2 += __shfl_xor(a2, o, kWave);
-  }
-  if (row >= 0 && lane == 0) {
-    V3 r{a0, a1, a2};
-    if (base) r = r + ld3(base, row);
-    out[3 * row] = static_cast<Out>(r.x);
-    out[3 * row + 1] = static_cast<Out>(r.y);
-    out[3 * row + 2] = static_cast<Out>(r.z);
-  }
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__device__ __forceinline__ void load_quad(const double* p, double (&v)[4]) {
+  const f64x2 a = NT ? __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p)) : reinterpret_cast<const f64x2*>(p)[0];
+  const f64x2 b = NT ? __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p) + 1) : reinterpret_cast<const f64x2*>(p)[1];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
 }
 
-template <typename Real, typename Col, typename In, typename Out, bool PRED>
-__global__ __launch_bounds__(kBlock) void k_spmv(const SpmvTask* tasks, const int* rows, const int* off,
-                                                  const Col* col, const Real* val, const In* in, Out* out,
-                                                  const double* base, const int* active, const int* tb, int B) {
-  __shared__ double smem[3 * kBlock / kWave];
-  __shared__ double res[3];
-  if (PRED) {
-    for_active(active, tb, B, [&](int ti, int) {
-      spmv_task<Real, Col, In, Out>(tasks[ti], rows, off, col, val, in, out, base, smem, res);
-    });
-  } else {
-    spmv_task<Real, Col, In, Out>(tasks[blockIdx.x], rows, off, col, val, in, out, base, smem, res);
-  }
+template <bool NT>
+__device__ __forceinline__ void load_quad(const float* p, float (&v)[4]) {
+  const f32x4 a = NT ? __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p)) : reinterpret_cast<const f32x4*>(p)[0];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.w;
+}
+
+template <typename Acc>
+__device__ __forceinline__ void load_vec4(const double* in, size_t j, Acc (&x)[3]) {
+  const f64x2 a = *reinterpret_cast<const f64x2*>(in + 4 * j);
+  x[0] = static_cast<Acc>(a.x);
+  x[1] = static_cast<Acc>(a.y);
+  x[2] = static_cast<Acc>(in[4 * j + 2]);
+}
+
+template <typename Acc>
+__device__ __forceinline__ void load_vec4(const float* in, size_t j, Acc (&x)[3]) {
+  const f32x4 a = *reinterpret_cast<const f32x4*>(in + 4 * j);
+  x[0] = static_cast<Acc>(a.x);
+  x[1] = static_cast<Acc>(a.y);
+  x[2] = static_cast<Acc>(a.z);
+}
+
+template <typename T>
+__device__ __forceinline__ void store_vec4(T* out, size_t row, V3 r) {
+  T* p = out + 4 * row;
+  p[0] = static_cast<T>(r.x);
+  p[1] = static_cast<T>(r.y);
+  p[2] = static_cast<T>(r.z);
+  p[3] = static_cast<T>(0);
+}
+
+static_assert(kColsPerGroup == 4 * kWave, "a lane owns four columns of a column group");
+
+template <typename Real, typename Acc, typename In>
+__global__ __launch_bounds__(kBlock) void k_factor_rows(const RowItem* __restrict__ items, const Real* __restrict__ val,
+                                                         const In* __restrict__ in, In* __restrict__ out,
+                                                         const int* active, const int* tb, int B) {
+  const int lane = static_cast<int>(threadIdx.x) % kWave;
+  RowItem g{};
+  bool have = false;
+  for_active_waves(active, tb, B, [&](int item, int next) {
+    if (!have) g = items[item];
+    // Prefetch the next item's descriptor while this one streams.
+    RowItem gn{};
+    if (next >= 0) gn = items[next];
+    Acc acc[kRowsPerGroup][3];
+#pragma unroll
+    for (int j = 0; j < kRowsPerGroup; ++j) acc[j][0] = acc[j][1] = acc[j][2] = Acc(0);
+    const size_t width = 4 * static_cast<size_t>(g.quads);
+    const Real* rows = val + g.offset;
+    for (int qi = lane; qi < g.quads; qi += kWave) {
+      // All loads of the pass first (rows past the item's count re-read its
+      // last row), then the products of the item's rows.
+      Acc x[4][3];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) load_vec4(in, static_cast<size_t>(g.col0 + 4 * qi + u), x[u]);
+      Real v[kRowsPerGroup][4];
+#pragma unroll
+      for (int j = 0; j < kRowsPerGroup; ++j) load_quad<false>(rows + width * min(j, g.count - 1) + 4 * qi, v[j]);
+#pragma unroll
+      for (int j = 0; j < kRowsPerGroup; ++j) {
+        if (j < g.count) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            acc[j][0] += static_cast<Acc>(v[j][u]) * x[u][0];
+            acc[j][1] += static_cast<Acc>(v[j][u]) * x[u][1];
+            acc[j][2] += static_cast<Acc>(v[j][u]) * x[u][2];
+          }
+        }
+      }
+    }
+    // Butterfly over the lanes the block spans (all 64 once it is wide).
+    const int span = g.quads >= kWave ? kWave : g.quads;
+#pragma unroll
+    for (int j = 0; j < kRowsPerGroup; ++j) {
+      if (j < g.count) {
+        for (int o = 1; o < span; o <<= 1) {
+          acc[j][0] += __shfl_xor(acc[j][0], o, kWave);
+          acc[j][1] += __shfl_xor(acc[j][1], o, kWave);
+          acc[j][2] += __shfl_xor(acc[j][2], o, kWave);
+        }
+        if (lane == 0)
+          store_vec4(out, static_cast<size_t>(g.out_row + j),
+                     V3{static_cast<double>(acc[j][0]), static_cast<double>(acc[j][1]), static_cast<double>(acc[j][2])});
+      }
+    }
+    g = gn;
+    have = true;
+  });
+}
+
+template <typename Real, typename Acc, typename In>
+__global__ __launch_bounds__(kBlock) void k_factor_cols(const ColGroup* __restrict__ items,
+                                                         const ColCover* __restrict__ covers,
+                                                         const Real* __restrict__ val, const In* __restrict__ y,
+                                                         double* __restrict__ out, const int* __restrict__ gorig,
+                                                         const double* base, const int* active, const int* tb, int B) {
+  __shared__ Acc part[kBlock / kWave - 1][12][kWave];
+  const int lane = static_cast<int>(threadIdx.x) % kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  for_active(active, tb, B, [&](int item, int) {
+    const ColGroup g = items[item];
+    const int c = g.c0 + 4 * lane;
+    Acc acc[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u][0] = acc[u][1] = acc[u][2] = Acc(0);
+    constexpr int kWaves = kBlock / kWave, kBatch = 8;
+    ColCover blk = g.b0 < g.b1 ? covers[g.b0] : ColCover{};
+    for (int bi = g.b0; bi < g.b1; ++bi) {
+      const ColCover nxt = bi + 1 < g.b1 ? covers[bi + 1] : ColCover{};
+      const int qi = (c - blk.col0) >> 2;
+      const bool on = c >= blk.col0 && qi < blk.quads;
+      const size_t width = 4 * static_cast<size_t>(blk.quads);
+      // Lanes outside the block read its first quad and drop it.
+      const Real* col = val + blk.offset + 4 * static_cast<size_t>(on ? qi : 0);
+      const int r1 = blk.row0 + blk.rows;
+      // Rows r = first + wave (mod 4 waves), in batches of kBatch rows whose
+      // loads are all issued before their products (the rows past r1 of the
+      // last batch re-read row r1 - 1 and are dropped).
+      for (int r = max(blk.row0, g.c0) + wave; r < r1; r += kWaves * kBatch) {
+        Acc yr[kBatch][3];
+        Real v[kBatch][4];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+          const int rr = min(r + kWaves * k, r1 - 1);
+          load_vec4(y, static_cast<size_t>(rr), yr[k]);
+          load_quad<false>(col + width * (rr - blk.row0), v[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+          if (r + kWaves * k < r1) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const Acc w = on ? static_cast<Acc>(v[k][u]) : Acc(0);
+              acc[u][0] += w * yr[k][0];
+              acc[u][1] += w * yr[k][1];
+              acc[u][2] += w * yr[k][2];
+            }
+          }
+        }
+      }
+      blk = nxt;
+    }
+    if (wave > 0) {
+#pragma unroll
+      for (int t = 0; t < 12; ++t) part[wave - 1][t][lane] = acc[t / 3][t % 3];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int w = 0; w < kBlock / kWave - 1; ++w)
+#pragma unroll
+        for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] += part[w][t][lane];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int colx = c + u;
+        if (colx < g.end) {
+          const int o = gorig[colx];
+          if (o >= 0) {
+            V3 r{static_cast<double>(acc[u][0]), static_cast<double>(acc[u][1]), static_cast<double>(acc[u][2])};
+            if (base) r = r + ld3(base, o);
+            st3(out, o, r);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  });
+}
+
+// rhs [N*3] FP64 (original order) -> the 4-wide postorder right-hand side.
+template <typename In>
+__global__ __launch_bounds__(kBlock) void k_pack4(int n, const int* gpost, const double* rhs, In* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  store_vec4(out, static_cast<size_t>(gpost[i]), ld3(rhs, i));
 }
 
 // ---------------------------------------------------------------------------
@@ -1295,35 +1482,32 @@ __global__ void k_eval_jvps(int count, const double* fs, const double* dfs, cons
 
 size_t prefix_lds(const DeviceState& d) { return sizeof(int) * static_cast<size_t>(d.B + 1); }
 
-// One CSR product of S (transpose = false) or S^T over the work list (PRED)
-// or over every task.
-template <typename Real, typename Col, typename In, typename Out, bool PRED>
-void spmv_one(const DeviceState& d, bool transpose, const In* in, Out* out, const double* base, const int* active,
-              hipStream_t stream) {
-  const int ntasks = transpose ? d.t_ntasks : d.s_ntasks;
-  if (!ntasks) return;
-  const void* col = sizeof(Col) == 2 ? static_cast<const void*>(transpose ? d.t_col16 : d.s_col16)
-                                     : static_cast<const void*>(transpose ? d.t_col : d.s_col);
-  const int grid = PRED ? (transpose ? d.g_ttask : d.g_stask) : ntasks;
-  hipLaunchKernelGGL((k_spmv<Real, Col, In, Out, PRED>), dim3(grid), dim3(kBlock), prefix_lds(d), stream,
-                     transpose ? d.t_tasks : d.s_tasks, transpose ? d.t_rows : d.s_rows,
-                     transpose ? d.t_off : d.s_off, static_cast<const Col*>(col),
-                     static_cast<const Real*>(transpose ? d.t_val : d.s_val), in, out, base, active,
-                     transpose ? d.ttb : d.stb, d.B);
+// The pair x = S^T (S rhs4) (+ base) over the items of the active bodies:
+// the S product writes the 4-wide postorder y4, the S^T product the FP64
+// [N*3] out in original order.
+template <typename Real, typename Acc, typename In>
+void spmv_pair(const DeviceState& d, double* out, const double* base, const int* active, hipStream_t stream) {
+  const Real* val = static_cast<const Real*>(d.fval);
+  In* y4 = static_cast<In*>(d.y4);
+  if (d.s_nitems)
+    hipLaunchKernelGGL((k_factor_rows<Real, Acc, In>), dim3(d.g_s), dim3(kBlock), prefix_lds(d), stream, d.s_items, val,
+                       static_cast<const In*>(d.rhs4), y4, active, d.s_tb, d.B);
+  if (d.t_nitems)
+    hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_t), dim3(kBlock), prefix_lds(d), stream, d.t_items,
+                       d.t_covers, val, static_cast<const In*>(y4), out, d.gorig, base, active, d.t_tb, d.B);
+}
+
+// Dispatch on the stored precision.
+void spmv_dispatch(const DeviceState& d, double* out, const double* base, const int* active, hipStream_t stream) {
+  if (d.f32)
+    spmv_pair<float, float, float>(d, out, base, active, stream);
+  else
+    spmv_pair<double, double, double>(d, out, base, active, stream);
 }
 
 // Iteration products: x_raw = S^T (S rhs) (+ base, the defect-correction form).
 void spmv_iteration(const DeviceState& d, const double* base, const int* active, hipStream_t stream) {
-  if (!d.f32) {
-    spmv_one<double, int, double, double, true>(d, false, d.rhs, d.y, nullptr, active, stream);
-    spmv_one<double, int, double, double, true>(d, true, d.y, d.xr, base, active, stream);
-  } else if (d.col16) {
-    spmv_one<float, uint16_t, float, float, true>(d, false, d.rhs32, d.y32, nullptr, active, stream);
-    spmv_one<float, uint16_t, float, double, true>(d, true, d.y32, d.xr, base, active, stream);
-  } else {
-    spmv_one<float, int, float, float, true>(d, false, d.rhs32, d.y32, nullptr, active, stream);
-    spmv_one<float, int, float, double, true>(d, true, d.y32, d.xr, base, active, stream);
-  }
+  spmv_dispatch(d, d.xr, base, active, stream);
 }
 
 }  // namespace
@@ -1400,16 +1584,14 @@ void launch_material_finish(const DeviceState& d, double* gmat, hipStream_t stre
 }
 
 void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, hipStream_t stream) {
-  if (!d.f32) {
-    spmv_one<double, int, double, double, false>(d, false, rhs, d.y, nullptr, d.ones, stream);
-    spmv_one<double, int, double, double, false>(d, true, d.y, out, nullptr, d.ones, stream);
-  } else if (d.col16) {
-    spmv_one<float, uint16_t, double, float, false>(d, false, rhs, d.y32, nullptr, d.ones, stream);
-    spmv_one<float, uint16_t, float, double, false>(d, true, d.y32, out, nullptr, d.ones, stream);
-  } else {
-    spmv_one<float, int, double, float, false>(d, false, rhs, d.y32, nullptr, d.ones, stream);
-    spmv_one<float, int, float, double, false>(d, true, d.y32, out, nullptr, d.ones, stream);
-  }
+  const int grid = (d.N + kBlock - 1) / kBlock;
+  if (d.f32)
+    hipLaunchKernelGGL(k_pack4<float>, dim3(grid), dim3(kBlock), 0, stream, d.N, d.gpost, rhs,
+                       static_cast<float*>(d.rhs4));
+  else
+    hipLaunchKernelGGL(k_pack4<double>, dim3(grid), dim3(kBlock), 0, stream, d.N, d.gpost, rhs,
+                       static_cast<double*>(d.rhs4));
+  spmv_dispatch(d, out, nullptr, d.ones, stream);
 }
 
 void launch_eval_projections(int count, const double* f, const double* dirs, double mu, double lame, double penalty,

@@ -1,0 +1,149 @@
+// Device layout of the inverse factor: postorder renumbering and supernode
+// blocks.
+//
+// The factor S = D^{-1/2} L^{-1} P comes from factor.cpp with rows in
+// elimination order and original column indices (the restatement of
+// DP/sparse.py:291-320). For the products on the device it is renumbered:
+// with the structural elimination tree of P A P^T (Liu's algorithm) and a
+// postorder of that tree, (L^{-1})_{ij} can be nonzero only when j is a
+// descendant of i (every path in the graph of L climbs the tree), and the
+// descendants of i are the postorder range ending at i. So row r of S is the
+// dense range [r - subtree(r) + 1, r]. Where a row's parent has it as its
+// only child, the parent's range is the child's plus one column, so a chain
+// of such rows (a supernode) is one dense trapezoid, stored as a block. The
+// values are the factor's own numbers, only placed differently.
+#include <algorithm>
+#include <stdexcept>
+
+#include "setup.h"
+
+namespace dphd {
+
+namespace {
+
+// Structural elimination tree of the symmetric pattern of `a` in the order
+// `perm` (perm[position] = original index). parent[i] = -1 for roots.
+std::vector<int> elimination_tree(const Csr& a, const std::vector<int>& perm, const std::vector<int>& pos) {
+  const int n = a.n;
+  std::vector<int> parent(n, -1), ancestor(n, -1);
+  for (int i = 0; i < n; ++i) {
+    const int o = perm[i];
+    for (int p = a.off[o]; p < a.off[o + 1]; ++p) {
+      int j = pos[a.col[p]];
+      while (j != -1 && j < i) {
+        const int next = ancestor[j];
+        ancestor[j] = i;
+        if (next == -1) parent[j] = i;
+        j = next;
+      }
+    }
+  }
+  return parent;
+}
+
+// Postorder of a forest given by parent pointers: children visited in
+// ascending order, roots in ascending order. Returns order[k] = node.
+std::vector<int> postorder(const std::vector<int>& parent) {
+  const int n = static_cast<int>(parent.size());
+  std::vector<int> head(n, -1), next(n, -1), roots;
+  for (int j = n - 1; j >= 0; --j) {
+    if (parent[j] >= 0) {
+      next[j] = head[parent[j]];
+      head[parent[j]] = j;
+    }
+  }
+  for (int j = 0; j < n; ++j)
+    if (parent[j] < 0) roots.push_back(j);
+  std::vector<int> order;
+  order.reserve(n);
+  std::vector<int> stack, cursor(n, -2);
+  for (const int root : roots) {
+    stack.push_back(root);
+    cursor[root] = head[root];
+    while (!stack.empty()) {
+      const int v = stack.back();
+      const int c = cursor[v];
+      if (c >= 0) {
+        cursor[v] = next[c];
+        stack.push_back(c);
+        cursor[c] = head[c];
+      } else {
+        order.push_back(v);
+        stack.pop_back();
+      }
+    }
+  }
+  if (static_cast<int>(order.size()) != n) throw std::logic_error("elimination tree is not a forest");
+  return order;
+}
+
+}  // namespace
+
+FactorLayout factor_layout(const Csr& a, const Factor& f) {
+  const int n = f.n;
+  if (a.n != n) throw std::logic_error("factor and matrix sizes differ");
+  std::vector<int> pos(n);
+  for (int i = 0; i < n; ++i) pos[f.permutation[i]] = i;
+  const std::vector<int> parent = elimination_tree(a, f.permutation, pos);
+  const std::vector<int> order = postorder(parent);  // order[k] = elimination index
+  std::vector<int> size(n, 1), children(n, 0);
+  for (int i = 0; i < n; ++i) {
+    if (parent[i] >= 0) {
+      size[parent[i]] += size[i];  // children precede parents
+      ++children[parent[i]];
+    }
+  }
+
+  // Slots: postorder positions, each tree starting at a multiple of 4.
+  FactorLayout out;
+  std::vector<int> slot(n);  // postorder position -> slot
+  int next_slot = 0;
+  for (int k = 0; k < n;) {
+    // The tree holding position k spans [k, k + size(root)); its root is the
+    // first position whose node has no parent.
+    int end = k;
+    while (parent[order[end]] >= 0) ++end;
+    next_slot = (next_slot + 3) & ~3;
+    for (int j = k; j <= end; ++j) slot[j] = next_slot + (j - k);
+    next_slot += end - k + 1;
+    k = end + 1;
+  }
+  out.slots = (next_slot + 3) & ~3;
+  std::vector<int> post_of(n);  // elimination index -> postorder position
+  for (int k = 0; k < n; ++k) post_of[order[k]] = k;
+  out.post.resize(n);
+  for (int o = 0; o < n; ++o) out.post[o] = slot[post_of[pos[o]]];
+
+  // Supernode chains and their blocks.
+  for (int k = 0; k < n;) {
+    const int k0 = k;
+    while (k + 1 < n && parent[order[k]] == order[k + 1] && children[order[k + 1]] == 1) ++k;
+    const int k1 = k + 1;  // rows [k0, k1)
+    FactorBlock blk;
+    blk.row0 = slot[k0];
+    blk.rows = k1 - k0;
+    blk.lo = slot[k0 - size[order[k0]] + 1];
+    blk.col0 = blk.lo & ~3;
+    const int end_col = blk.row0 + blk.rows;  // exclusive; rows end at themselves
+    blk.quads = (end_col - blk.col0 + 3) / 4;
+    blk.offset = static_cast<int64_t>(out.val.size());
+    const size_t width = 4 * static_cast<size_t>(blk.quads);
+    out.val.resize(out.val.size() + width * blk.rows, 0.0);
+    for (int j = 0; j < blk.rows; ++j) {
+      const int i = order[k0 + j];
+      const int row = blk.row0 + j;
+      double* dst = out.val.data() + blk.offset + width * j;
+      for (int p = f.s.off[i]; p < f.s.off[i + 1]; ++p) {
+        const int c = out.post[f.s.col[p]];
+        if (c < blk.lo || c > row) throw std::logic_error("inverse factor entry outside its descendant range");
+        dst[c - blk.col0] = f.s.val[p];
+      }
+      out.entries += row - blk.lo + 1;
+    }
+    out.blocks.push_back(blk);
+    k = k1;
+  }
+  return out;
+}
+
+}  // namespace dphd

@@ -34,6 +34,37 @@ struct Factor {
   int64_t matrix_nnz = 0;         // nnz(A) after dropping exact zeros
 };
 
+// Device layout of one body's inverse factor. Rows and columns are
+// renumbered in a postorder of the structural elimination tree of A (taken
+// in the factor's elimination order), each tree of the forest starting at a
+// multiple of 4 (the unused slots in between are gaps). Row i of L^{-1} is
+// nonzero only at the descendants of i, which a postorder numbers
+// contiguously, so row r of S is the dense range [lo(r), r]. A chain of rows
+// whose parents have a single child (a supernode: the rows of one
+// separator, typically) shares lo, so the chain is one dense block: rows
+// [row0, row0 + rows) by columns [lo, row0 + rows), stored row-major from
+// the 4-aligned column lo & ~3 with `quads` quads (4 columns) per row. Block
+// entries the factor does not hold (above the diagonal, left and right
+// alignment, exact zeros of the factor) are zero.
+struct FactorBlock {
+  int row0 = 0, rows = 0;  // postorder rows
+  int lo = 0;              // first column of every row (postorder)
+  int col0 = 0;            // lo & ~3: column of the first stored value
+  int quads = 0;           // stored quads per row
+  int64_t offset = 0;      // first value
+};
+
+struct FactorLayout {
+  std::vector<int> post;  // original local index -> postorder slot
+  int slots = 0;          // postorder slots including gaps (a multiple of 4)
+  std::vector<FactorBlock> blocks;  // ascending row0
+  std::vector<double> val;
+  int64_t entries = 0;    // dense-range entries (sum of row lengths)
+};
+
+// Throws std::logic_error if a factor entry falls outside the structure.
+FactorLayout factor_layout(const Csr& a, const Factor& f);
+
 // Validates like DP/sparse.py:73-95 (finite, positive diagonal, exactly
 // symmetric) and drops exact zeros. Throws std::runtime_error.
 Csr validate_rows(const Csr& a);

@@ -62,41 +62,13 @@ T* dzeros(std::vector<void*>& owned, size_t count) {
   return p;
 }
 
-int pick_lanes(int len) {
-  if (len > 2048) return kBlock;
-  int lanes = 1;
-  while (lanes < 64 && lanes * 4 < len) lanes <<= 1;
-  return lanes;
-}
-
-// Row binning of one CSR, body by body: within a body, rows are grouped by
-// lanes-per-row (ascending row id within a group); one task per workgroup.
-// task_base[b] is the first task of body b.
-void build_tasks(const std::vector<int>& off, const std::vector<int>& voff, std::vector<int>& rows,
-                 std::vector<SpmvTask>& tasks, std::vector<int>& task_base) {
-  const int classes[] = {1, 2, 4, 8, 16, 32, 64, kBlock};
-  const int B = static_cast<int>(voff.size()) - 1;
-  rows.clear();
-  tasks.clear();
-  task_base.assign(B + 1, 0);
-  for (int b = 0; b < B; ++b) {
-    task_base[b] = static_cast<int>(tasks.size());
-    for (int L : classes) {
-      const int first = static_cast<int>(rows.size());
-      for (int r = voff[b]; r < voff[b + 1]; ++r)
-        if (pick_lanes(off[r + 1] - off[r]) == L) rows.push_back(r);
-      const int count = static_cast<int>(rows.size()) - first;
-      const int per = L == kBlock ? 1 : kBlock / L;
-      for (int k = 0; k < count; k += per)
-        tasks.push_back(SpmvTask{first + k, std::min(per, count - k), L, voff[b]});
-    }
-  }
-  task_base[B] = static_cast<int>(tasks.size());
-}
-
 // Workgroups of a work-list kernel: enough to fill the device, never more
-// than there are items.
-int grid_for(int items) { return std::max(1, std::min(items, 2048)); }
+// than there are items, a multiple of the 8 XCDs once there are 8 or more.
+int grid_for(int items) {
+  int g = std::max(1, std::min(items, 2048));
+  if (g >= 8) g &= ~7;
+  return g;
+}
 
 }  // namespace
 }  // namespace dphd
@@ -273,8 +245,9 @@ void build_solver(dphd_solver* s, double dt) {
       mw[e] = k * svol[ebody[e]];
     }
   }
-  // Assembly and factor per body, in parallel over bodies.
-  std::vector<Factor> factors(B);
+  // Assembly, factor and device layout per body, in parallel over bodies.
+  std::vector<FactorLayout> layouts(B);
+  std::vector<int64_t> factor_entries(B, 0);
   std::vector<Csr> amats(B);
   std::vector<std::string> errors(B);
   {
@@ -305,7 +278,9 @@ void build_solver(dphd_solver* s, double dt) {
             in.free = free.data() + v0;
             in.inertia = inertia[b];
             amats[b] = assemble_matrix(in);
-            factors[b] = build_factor(amats[b]);
+            const Factor factor = build_factor(amats[b]);
+            factor_entries[b] = static_cast<int64_t>(factor.s.col.size());
+            layouts[b] = factor_layout(amats[b], factor);
           } catch (const std::exception& ex) {
             errors[b] = ex.what();
           }
@@ -315,38 +290,75 @@ void build_solver(dphd_solver* s, double dt) {
   }
   for (int b = 0; b < B; ++b)
     if (!errors[b].empty()) throw std::invalid_argument(errors[b]);
-  // Global CSR (rows/columns offset by the body's first vertex).
+  // Global factor blocks: body b's postorder slots are offset by slot0[b]
+  // (a multiple of 4), its values by the values before it.
   int64_t znnz = 0, annz = 0;
   s->body_nnz.assign(B, 0);
+  std::vector<int> slot0(B + 1, 0);
   for (int b = 0; b < B; ++b) {
-    s->body_nnz[b] = static_cast<int64_t>(factors[b].s.col.size());
-    znnz += static_cast<int64_t>(factors[b].s.col.size());
+    s->body_nnz[b] = factor_entries[b];
+    znnz += factor_entries[b];
     annz += static_cast<int64_t>(amats[b].col.size());
+    if (static_cast<int64_t>(slot0[b]) + layouts[b].slots > INT32_MAX - 8)
+      throw std::invalid_argument("postorder slots exceed int32 indexing");
+    slot0[b + 1] = slot0[b] + layouts[b].slots;
   }
-  if (znnz > INT32_MAX) throw std::invalid_argument("inverse factor fill exceeds int32 sparse indexing");
-  std::vector<int> s_off(N + 1, 0), s_col(znnz), t_off(N + 1, 0), t_col(znnz);
-  std::vector<double> s_val(znnz), t_val(znnz);
-  {
-    int64_t zs = 0, zt = 0;
-    for (int b = 0; b < B; ++b) {
-      const int v0 = s->voff[b];
-      const Factor& f = factors[b];
-      for (int r = 0; r < f.n; ++r) {
-        for (int p = f.s.off[r]; p < f.s.off[r + 1]; ++p) {
-          s_col[zs] = f.s.col[p] + v0;
-          s_val[zs] = f.s.val[p];
-          ++zs;
-        }
-        s_off[v0 + r + 1] = static_cast<int>(zs);
-        for (int p = f.st.off[r]; p < f.st.off[r + 1]; ++p) {
-          t_col[zt] = f.st.col[p] + v0;
-          t_val[zt] = f.st.val[p];
-          ++zt;
-        }
-        t_off[v0 + r + 1] = static_cast<int>(zt);
+  const int slots = slot0[B];
+  std::vector<int> gpost(N), gorig(slots, -1);
+  std::vector<double> fval;
+  std::vector<RowItem> s_items;
+  std::vector<ColGroup> t_items;
+  std::vector<ColCover> t_covers;
+  std::vector<int> s_tb(B + 1, 0), t_tb(B + 1, 0);
+  for (int b = 0; b < B; ++b) {
+    const int v0 = s->voff[b], n = s->voff[b + 1] - v0, c0 = slot0[b];
+    FactorLayout& L = layouts[b];
+    for (int o = 0; o < n; ++o) {
+      gpost[v0 + o] = c0 + L.post[o];
+      gorig[c0 + L.post[o]] = v0 + o;
+    }
+    const long long voff_b = static_cast<long long>(fval.size());
+    s_tb[b] = static_cast<int>(s_items.size());
+    for (const FactorBlock& fb : L.blocks) {
+      const long long width = 4 * static_cast<long long>(fb.quads);
+      for (int r = 0; r < fb.rows; r += kRowsPerGroup) {
+        RowItem it{};
+        it.offset = voff_b + fb.offset + width * r;
+        it.col0 = c0 + fb.col0;
+        it.quads = fb.quads;
+        it.out_row = c0 + fb.row0 + r;
+        it.count = std::min(kRowsPerGroup, fb.rows - r);
+        s_items.push_back(it);
       }
     }
+    fval.insert(fval.end(), L.val.begin(), L.val.end());
+    // Column groups of 256 slots and the blocks that cover each of them.
+    const int groups = (L.slots + kColsPerGroup - 1) / kColsPerGroup;
+    std::vector<std::vector<int>> cover(groups);
+    for (size_t k = 0; k < L.blocks.size(); ++k) {
+      const FactorBlock& fb = L.blocks[k];
+      const int g0 = fb.col0 / kColsPerGroup, g1 = (fb.row0 + fb.rows - 1) / kColsPerGroup;
+      for (int g = g0; g <= g1; ++g) cover[g].push_back(static_cast<int>(k));
+    }
+    t_tb[b] = static_cast<int>(t_items.size());
+    for (int g = 0; g < groups; ++g) {
+      const int b0 = static_cast<int>(t_covers.size());
+      for (const int k : cover[g]) {
+        const FactorBlock& fb = L.blocks[k];
+        ColCover cv{};
+        cv.offset = voff_b + fb.offset;
+        cv.row0 = c0 + fb.row0;
+        cv.rows = fb.rows;
+        cv.col0 = c0 + fb.col0;
+        cv.quads = fb.quads;
+        t_covers.push_back(cv);
+      }
+      t_items.push_back(ColGroup{c0 + g * kColsPerGroup, b0, static_cast<int>(t_covers.size()), c0 + L.slots});
+    }
+    L = FactorLayout{};
   }
+  s_tb[B] = static_cast<int>(s_items.size());
+  t_tb[B] = static_cast<int>(t_items.size());
   std::vector<int> a_off, a_col;
   std::vector<double> a_val;
   const bool defect = s->cfg.factor_precision == 32;
@@ -364,10 +376,6 @@ void build_solver(dphd_solver* s, double dt) {
       }
     }
   }
-  std::vector<int> s_rows, t_rows, stb, ttb;
-  std::vector<SpmvTask> s_tasks, t_tasks;
-  build_tasks(s_off, s->voff, s_rows, s_tasks, stb);
-  build_tasks(t_off, s->voff, t_rows, t_tasks, ttb);
   // Chunks and incidence.
   std::vector<Chunk> vchunks, echunks;
   std::vector<int> vcb(B + 1, 0), ecb(B + 1, 0);
@@ -465,47 +473,37 @@ void build_solver(dphd_solver* s, double dt) {
   d.gravity = dupload(o, s->gravity);
   d.nh = dupload(o, nh);
   d.svol = dupload(o, svol);
-  d.s_off = dupload(o, s_off);
-  d.s_col = dupload(o, s_col);
-  d.t_off = dupload(o, t_off);
-  d.t_col = dupload(o, t_col);
+  d.slots = slots;
+  d.gpost = dupload(o, gpost);
+  d.gorig = dupload(o, gorig);
+  d.f32 = defect;
   if (defect) {
-    std::vector<float> sv(s_val.begin(), s_val.end()), tv(t_val.begin(), t_val.end());
-    d.s_val = dupload(o, sv);
-    d.t_val = dupload(o, tv);
-    d.f32 = true;
-    // Body-local 16-bit columns when every body fits (6 bytes per nonzero).
-    int largest = 0;
-    for (int b = 0; b < B; ++b) largest = std::max(largest, s->voff[b + 1] - s->voff[b]);
-    if (largest <= 65536) {
-      std::vector<uint16_t> sc16(znnz), tc16(znnz);
-      for (int b = 0; b < B; ++b) {
-        const int v0 = s->voff[b];
-        for (int64_t p = s_off[v0]; p < s_off[s->voff[b + 1]]; ++p) sc16[p] = static_cast<uint16_t>(s_col[p] - v0);
-        for (int64_t p = t_off[v0]; p < t_off[s->voff[b + 1]]; ++p) tc16[p] = static_cast<uint16_t>(t_col[p] - v0);
-      }
-      d.s_col16 = dupload(o, sc16);
-      d.t_col16 = dupload(o, tc16);
-      d.col16 = true;
-    }
-    d.rhs32 = dzeros<float>(o, static_cast<size_t>(N) * 3);
-    d.y32 = dzeros<float>(o, static_cast<size_t>(N) * 3);
+    const std::vector<float> v32(fval.begin(), fval.end());
+    d.fval = dupload(o, v32);
+  } else {
+    d.fval = dupload(o, fval);
+  }
+  std::vector<double>().swap(fval);
+  d.s_items = dupload(o, s_items);
+  d.s_nitems = static_cast<int>(s_items.size());
+  d.s_tb = dupload(o, s_tb);
+  d.t_items = dupload(o, t_items);
+  d.t_nitems = static_cast<int>(t_items.size());
+  d.t_tb = dupload(o, t_tb);
+  d.t_covers = dupload(o, t_covers);
+  d.g_s = grid_for((d.s_nitems + 3) / 4);
+  d.g_t = grid_for(d.t_nitems);
+  if (defect) {
     d.a_off = dupload(o, a_off);
     d.a_col = dupload(o, a_col);
     d.a_val = dupload(o, a_val);
     d.defect = 1;
-  } else {
-    d.s_val = dupload(o, s_val);
-    d.t_val = dupload(o, t_val);
   }
-  d.s_rows = dupload(o, s_rows);
-  d.t_rows = dupload(o, t_rows);
-  d.s_tasks = dupload(o, s_tasks);
-  d.t_tasks = dupload(o, t_tasks);
-  d.s_ntasks = static_cast<int>(s_tasks.size());
-  d.t_ntasks = static_cast<int>(t_tasks.size());
-  d.stb = dupload(o, stb);
-  d.ttb = dupload(o, ttb);
+  // 4-wide S-product vectors over the postorder slots (float in the FP32
+  // mode); gap slots stay zero.
+  const size_t v4bytes = (static_cast<size_t>(slots) + 4) * 4 * (d.f32 ? sizeof(float) : sizeof(double));
+  d.rhs4 = dzeros<unsigned char>(o, v4bytes);
+  d.y4 = dzeros<unsigned char>(o, v4bytes);
   const size_t v3 = static_cast<size_t>(N) * 3;
   d.q0 = dzeros<double>(o, v3);
   d.qt = dzeros<double>(o, v3);
@@ -513,8 +511,6 @@ void build_solver(dphd_solver* s, double dt) {
   d.x = dzeros<double>(o, v3);
   d.xr = dzeros<double>(o, v3);
   d.p = dzeros<double>(o, v3);
-  d.rhs = dzeros<double>(o, v3);
-  d.y = dzeros<double>(o, v3);
   d.contrib = dzeros<double>(o, static_cast<size_t>(E) * K * 3);
   d.W = s->cfg.anderson_window;
   if (d.W > 0) {
@@ -530,8 +526,6 @@ void build_solver(dphd_solver* s, double dt) {
   d.ecb = dupload(o, ecb);
   d.g_vert = grid_for(d.nvc);
   d.g_elem = grid_for(d.nec);
-  d.g_stask = grid_for(d.s_ntasks);
-  d.g_ttask = grid_for(d.t_ntasks);
   d.gram_part = dzeros<double>(o, static_cast<size_t>(d.nvc) * kGramEntries);
   d.conv_part = dzeros<double>(o, static_cast<size_t>(d.nvc) * 4);
   d.clip_part = dzeros<double>(o, d.nvc);

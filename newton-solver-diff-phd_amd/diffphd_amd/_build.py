@@ -1,6 +1,6 @@
 """Build recipe of the native library ``libdiffphd.so`` (gfx950).
 
-Host setup sources (``factor.cpp``, ``capi_setup.cpp``) are compiled by g++
+Host setup sources (``factor.cpp``, ``layout.cpp``, ``capi_setup.cpp``) are compiled by g++
 with ``-ffp-contract=off`` so their arithmetic rounds where the reference's
 NumPy rounds; HIP sources are compiled by hipcc for gfx950 only. The library
 is written in-tree next to this file.
@@ -23,7 +23,7 @@ BUILD = ROOT / "_build"
 LIBRARY = PACKAGE / "libdiffphd.so"
 ARCH = os.environ.get("DPHD_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["factor.cpp", "capi_setup.cpp"]
+HOST_SOURCES = ["factor.cpp", "layout.cpp", "capi_setup.cpp"]
 HIP_SOURCES = ["kernels.hip", "solver.hip"]
 
 
