@@ -11,9 +11,10 @@ One "step" is one forward PD step of every body plus its adjoint step.
 Timed: K forward steps (autograd-recorded) followed by the backward sweep
 through them, all bodies in one batched solver (`value`, steps/s), and the
 same bodies as 256 independent single-body solvers run one after another
-(the plain per-body GPU path). Roofline: the CSR product pair
-A^{-1} = S^T S of all bodies (the dominant kernel), timed live with HIP
-events. CPU baseline: the FP64 C oracle on a sample of bodies.
+(the plain per-body GPU path). Roofline: the factor products S and S^T
+of all bodies (A^{-1} = S^T S, the dominant kernels), timed live with HIP
+events; algorithmic bytes count the reference's CSR form. CPU baseline: the
+FP64 C oracle, one step of every body on the host cores.
 """
 
 from __future__ import annotations
@@ -113,27 +114,27 @@ def rollout(solver, model, steps, dt, idx, tgt):
 
 
 # ---------------------------------------------------------------------------
-# Algorithmic bytes (compulsory traffic of each kernel at stored widths).
+# Algorithmic bytes: the compulsory traffic of the reference's algorithm at
+# stored widths, counted once per logical pass (s = factor value bytes, V =
+# 24-byte FP64 vertex vectors, CSR column indices 4 bytes).
+
+
+def bytes_product(n, z, s=8, v_in=24, v_out=24):
+    """One factor product (S or S^T) in the reference's CSR form: values and column indices, row offsets,
+    the input and output vectors."""
+    return z * (s + 4) + 8 * (n + 1) + n * (v_in + v_out)
 
 
 def bytes_forward_iteration(n, e, z, s=8, window=5):
-    spmv = 2 * (z * (s + 4) + n * (4 + 4 + 24 + 24) + 4)
-    local = e * (16 + 36 + 24 + 1 + 96) + n * 24
-    gather = e * (16 + 96) + n * (4 + 24 + 48 + 24)
-    push = n * (24 + 24 + 48 + 24 * window)
-    apply = n * (24 + 24 + 24 * (window + 1) + 24)
-    return spmv + local + gather + push + apply
+    """Per forward iteration per body: 2 Z (s+4) + 8 (n+1) + E (4K + G_e + 3s) + c_f n V, Tet4 (K = 4,
+    G_e = 12 s), c_f = 6 + (3m + 5) + 4."""
+    c_f = 6 + (3 * window + 5) + 4
+    return 2 * z * (s + 4) + 8 * (n + 1) + e * (4 * 4 + 12 * s + 3 * s) + c_f * n * 24
 
 
-def bytes_adjoint_iteration(n, e, z, s=8, window=5):
-    spmv = 2 * (z * (s + 4) + n * (4 + 4 + 24 + 24) + 4)
-    local = e * (16 + 36 + 648 + 96) + n * 24
-    gather = e * (16 + 96) + n * (4 + 24 + 24)
-    clip = n * 48
-    push = n * (48 + 24 + 48 + 24 * window)
-    apply = n * (24 + 24 * (window + 1) + 24)
-    finish = n * 48
-    return spmv + local + gather + clip + push + apply + finish
+def bytes_adjoint_iteration(n, e, z, s=8, window=5, s_j=8):
+    """Per adjoint iteration per body: 2 Z (s+4) + 8 (n+1) + E (4K + 81 s_J) + c_a n V, c_a = 34."""
+    return 2 * z * (s + 4) + 8 * (n + 1) + e * (4 * 4 + 81 * s_j) + 34 * n * 24
 
 
 # ---------------------------------------------------------------------------
@@ -152,7 +153,7 @@ def main():
     ap.add_argument("--plain-bodies", type=int, default=None, help="bodies timed on the plain path (default all)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-bodies", type=int, default=6)
+    ap.add_argument("--cpu-sample-bodies", type=int, default=0, help="0 = every body")
     ap.add_argument("--probe-reps", type=int, default=20)
     args = ap.parse_args()
 
@@ -214,8 +215,9 @@ def main():
     elem_iters = float((fwd_iters * e_b).sum() + (adj_iters * e_b).sum())
     rollout_gbps = total_bytes / elapsed / 1e9
 
-    # Roofline probe: S and S^T products of every body, timed with HIP events
-    # on the stream they are launched on (torch's current stream).
+    # Roofline probe: the S and S^T products of every body (the dominant
+    # kernels), timed with HIP events on the stream they are launched on
+    # (torch's current stream). One apply_inverse = pack + S + S^T.
     rhs = torch.randn((n_total, 3), dtype=torch.float64, device=dev)
     for _ in range(3):
         solver.apply_inverse(rhs)
@@ -228,9 +230,16 @@ def main():
     torch.cuda.synchronize()
     per_launch_s = ev0.elapsed_time(ev1) / 1e3 / (2 * args.probe_reps)
     z_total = float(z_b.sum())
-    launch_bytes = z_total * (s_bytes + 4) + n_total * (4 + 4 + 24 + 24) + 4
+    v_half = 24 if args.precision == 64 else 12  # the S-product vectors are FP32 in the FP32-factor mode
+    # Algorithmic bytes of the pair (S: in V/2.. out; S^T: in, out FP64), per launch.
+    launch_bytes = 0.5 * (bytes_product(n_total, z_total, s_bytes, v_half, v_half) + bytes_product(n_total, z_total, s_bytes, v_half, 24))
     probe_gbps = launch_bytes / per_launch_s / 1e9
-    log(f"[bench] spmv probe {per_launch_s * 1e6:.1f} us/launch, {probe_gbps:.0f} GB/s")
+    info = solver.info
+    slots = float(info.factor_slots)
+    vec4 = 32 if args.precision == 64 else 16
+    physical_bytes = float(info.factor_values) * s_bytes + 0.5 * (2 * slots * vec4 + slots * vec4 + n_total * 24)
+    log(f"[bench] factor product probe {per_launch_s * 1e6:.1f} us/launch, {probe_gbps:.0f} GB/s algorithmic, "
+        f"{physical_bytes / per_launch_s / 1e9:.0f} GB/s of stored bytes")
     del rhs
 
     # Copy-kernel ceiling: a plain 256 MiB device copy (also the reference
@@ -280,11 +289,11 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(bodies, z_b, dt, args.cpu_sample_bodies, solver)
 
-    # HBM traffic per launch of the probe kernel from the committed rocprofv3
+    # HBM traffic per launch of the probe kernels from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs (calibrated against a plain copy kernel).
     traffic = None
-    traffic_file = ROOT / "profiles" / "traffic_k_spmv_batched256.json"
-    if traffic_file.exists() and args.precision == 64 and len(bodies) == 256 and (args.kmin, args.kmax) == (4, 16):
+    traffic_file = ROOT / "profiles" / f"traffic_factor_products_batched256_fp{args.precision}.json"
+    if traffic_file.exists() and len(bodies) == 256 and (args.kmin, args.kmax) == (4, 16):
         traffic = json.loads(traffic_file.read_text())["traffic_bytes_per_launch"]
 
     value = args.steps / elapsed
@@ -324,7 +333,7 @@ def main():
         "gradients_finite": grad_ok,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_spmv (S and S^T CSR products, all bodies)",
+            "kernel": "k_factor_rows + k_factor_cols (S and S^T products of all bodies), per launch",
             "achieved": probe_gbps,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
@@ -332,6 +341,8 @@ def main():
             "traffic": traffic,
             "launch_us": per_launch_s * 1e6,
             "bytes_per_launch": launch_bytes,
+            "stored_bytes_per_launch": physical_bytes,
+            "stored_gbps": physical_bytes / per_launch_s / 1e9,
             "copy_ceiling_gbps": copy_gbps,
             "frac_of_copy": probe_gbps / copy_gbps,
         },
@@ -341,13 +352,17 @@ def main():
 
 
 def cpu_baseline(bodies, z_b, dt, sample_count, solver):
-    """FP64 C oracle, one fwd+adj step on a sample of bodies, scaled by factor size to the whole batch."""
+    """FP64 C oracle on the host cores: one fwd+adj step from rest of every body (or of a sample scaled by
+    factor size to the whole batch)."""
     from diffphd_amd import _lib
     from oracle import oracle
 
     threads = min(16, os.cpu_count() or 1)
-    order = np.argsort([b["k"] for b in bodies])
-    pick = order[np.linspace(0, len(order) - 1, sample_count).astype(int)]
+    if sample_count <= 0 or sample_count >= len(bodies):
+        pick = np.arange(len(bodies))
+    else:
+        order = np.argsort([b["k"] for b in bodies])
+        pick = order[np.linspace(0, len(order) - 1, sample_count).astype(int)]
     seconds = 0.0
     for i in pick:
         b = bodies[i]
@@ -366,13 +381,13 @@ def cpu_baseline(bodies, z_b, dt, sample_count, solver):
         seconds += time.perf_counter() - t0
     scale = float(z_b.sum() / z_b[pick].sum())
     per_step = seconds * scale
+    what = "all" if len(pick) == len(bodies) else f"{len(pick)} (k={[bodies[i]['k'] for i in pick]}), scaled by nnz(S) x{scale:.1f}"
     return {
         "value": 1.0 / per_step,
         "unit": "steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{sample_count} bodies (k={[bodies[i]['k'] for i in pick]}), 1 fwd+adj step each from rest, "
-        f"{seconds:.1f}s, scaled by total/sample nnz(S) = {scale:.1f}",
+        "sample": f"{what} of {len(bodies)} bodies, one fwd+adj step each from rest, {seconds:.1f} s of oracle time",
     }
 
 

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPHD_ABI_VERSION 1
+#define DPHD_ABI_VERSION 2
 
 enum {
   DPHD_OK = 0,
@@ -99,6 +99,8 @@ typedef struct {
   int64_t matrix_nnz;       /* nnz(A) per coordinate summed over bodies */
   int64_t jacobian_bytes;   /* device bytes of the adjoint's Jacobian, element blocks and assembled operator */
   double setup_seconds;     /* host time of the last dphd_build */
+  int64_t factor_values;    /* values stored on the device for S (dense supernode blocks, zeros included) */
+  int64_t factor_slots;     /* postorder slots of the device vectors (vertices plus alignment gaps) */
 } dphd_info;
 
 int dphd_create(const dphd_config* config, int device, dphd_solver** out);

@@ -51,23 +51,25 @@ struct RowItem {
   int pad[2];
 };
 
-// One block as the S^T column groups that it covers see it.
-struct ColCover {
-  long long offset;
-  int row0;
-  int rows;
+// One row of one block as an S^T column group sees it: the row's values
+// start at `rowoff` with `quads` quads from column col0, and its S-product
+// value is y[yrow].
+struct ColEntry {
+  long long rowoff;
   int col0;
   int quads;
-  int pad[2];
+  int yrow;
+  int pad;
 };
 
 // S^T product work item (one workgroup): the 256 columns [c0, c0 + 256) of
-// one body (columns from `end` on belong to the next body), over the blocks
-// covers[b0 .. b1) that cover them.
+// one body (columns from `end` on belong to the next body), over the rows
+// entries[e0 .. e1) of the blocks that cover them (block by block, rows
+// ascending).
 struct ColGroup {
   int c0;
-  int b0;
-  int b1;
+  int e0;
+  int e1;
   int end;
 };
 
@@ -113,7 +115,7 @@ struct DeviceState {
   ColGroup* t_items = nullptr;
   int t_nitems = 0;
   int* t_tb = nullptr;        // [B+1] S^T items of each body
-  ColCover* t_covers = nullptr;  // blocks of the column groups
+  ColEntry* t_entries = nullptr;  // block rows of the column groups
   int g_s = 0, g_t = 0;       // grids of the two products
   bool f32 = false;           // FP32 factor values
   // constant matrix for the defect correction (per coordinate)
@@ -165,6 +167,7 @@ struct DeviceState {
   double* ke = nullptr;      // [E*K*K*9] element blocks sum_s G_s^T J_s G_s
   double* kval = nullptr;    // [kblocks*9]
   int kblocks = 0;
+  int kmaxwidth = 0;         // the widest chunk's slots
   double *arhs = nullptr, *sol = nullptr;  // [N*3]
   int* aiter = nullptr;      // [B]
   int* aactive = nullptr;    // [B]

@@ -525,7 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_factor_rows(const RowItem* __restric
 
 template <typename Real, typename Acc, typename In>
 __global__ __launch_bounds__(kBlock) void k_factor_cols(const ColGroup* __restrict__ items,
-                                                         const ColCover* __restrict__ covers,
+                                                         const ColEntry* __restrict__ entries,
                                                          const Real* __restrict__ val, const In* __restrict__ y,
                                                          double* __restrict__ out, const int* __restrict__ gorig,
                                                          const double* base, const int* active, const int* tb, int B) {
@@ -538,42 +538,35 @@ __global__ __launch_bounds__(kBlock) void k_factor_cols(const ColGroup* __restri
     Acc acc[4][3];
 #pragma unroll
     for (int u = 0; u < 4; ++u) acc[u][0] = acc[u][1] = acc[u][2] = Acc(0);
+    // Wavefront w takes the w-th quarter of the group's rows, in batches of
+    // kBatch rows (from any number of blocks) whose loads are all issued
+    // before their products; rows past the quarter re-read its last row and
+    // are dropped.
     constexpr int kWaves = kBlock / kWave, kBatch = 8;
-    ColCover blk = g.b0 < g.b1 ? covers[g.b0] : ColCover{};
-    for (int bi = g.b0; bi < g.b1; ++bi) {
-      const ColCover nxt = bi + 1 < g.b1 ? covers[bi + 1] : ColCover{};
-      const int qi = (c - blk.col0) >> 2;
-      const bool on = c >= blk.col0 && qi < blk.quads;
-      const size_t width = 4 * static_cast<size_t>(blk.quads);
-      // Lanes outside the block read its first quad and drop it.
-      const Real* col = val + blk.offset + 4 * static_cast<size_t>(on ? qi : 0);
-      const int r1 = blk.row0 + blk.rows;
-      // Rows r = first + wave (mod 4 waves), in batches of kBatch rows whose
-      // loads are all issued before their products (the rows past r1 of the
-      // last batch re-read row r1 - 1 and are dropped).
-      for (int r = max(blk.row0, g.c0) + wave; r < r1; r += kWaves * kBatch) {
-        Acc yr[kBatch][3];
-        Real v[kBatch][4];
+    const int count = g.e1 - g.e0, share = (count + kWaves - 1) / kWaves;
+    const int w0 = g.e0 + min(wave * share, count), w1 = g.e0 + min((wave + 1) * share, count);
+    for (int e = w0; e < w1; e += kBatch) {
+      Acc yr[kBatch][3];
+      Real v[kBatch][4];
+      bool on[kBatch];
 #pragma unroll
-        for (int k = 0; k < kBatch; ++k) {
-          const int rr = min(r + kWaves * k, r1 - 1);
-          load_vec4(y, static_cast<size_t>(rr), yr[k]);
-          load_quad<false>(col + width * (rr - blk.row0), v[k]);
-        }
+      for (int k = 0; k < kBatch; ++k) {
+        const ColEntry en = entries[min(e + k, w1 - 1)];
+        const int qi = (c - en.col0) >> 2;
+        on[k] = e + k < w1 && c >= en.col0 && qi < en.quads;
+        load_vec4(y, static_cast<size_t>(en.yrow), yr[k]);
+        load_quad<false>(val + en.rowoff + 4 * static_cast<size_t>(on[k] ? qi : 0), v[k]);
+      }
 #pragma unroll
-        for (int k = 0; k < kBatch; ++k) {
-          if (r + kWaves * k < r1) {
+      for (int k = 0; k < kBatch; ++k) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const Acc w = on ? static_cast<Acc>(v[k][u]) : Acc(0);
-              acc[u][0] += w * yr[k][0];
-              acc[u][1] += w * yr[k][1];
-              acc[u][2] += w * yr[k][2];
-            }
-          }
+        for (int u = 0; u < 4; ++u) {
+          const Acc w = on[k] ? static_cast<Acc>(v[k][u]) : Acc(0);
+          acc[u][0] += w * yr[k][0];
+          acc[u][1] += w * yr[k][1];
+          acc[u][2] += w * yr[k][2];
         }
       }
-      blk = nxt;
     }
     if (wave > 0) {
 #pragma unroll
@@ -656,18 +649,31 @@ __device__ __forceinline__ void push_and_gram(const DeviceState& d, int b, int i
   }
 }
 
+// Mixing coefficients of a ring of N differences (anderson.py:150-215):
+// the regularized Gram system solved in registers; returns |gamma|^2. The
+// Gram entries are read from the reduced partials with compile-time indices.
 template <int N>
-__device__ __forceinline__ void solve_fixed(const double* gram, const double* rhs, double* coef) {
-  double a[N * N], b[N], x[N];
+__device__ __forceinline__ double anderson_coefficients(const double* red, double* gamma) {
+  double a[N][N], b[N], x[N];
+  double trace = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) trace += red[gram_index(i, i)];
+  const double reg = fmax(1.0e-6 * trace / static_cast<double>(N), 1.0e-12);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    b[i] = rhs[i];
+    b[i] = red[kRhsIndex + i];
 #pragma unroll
-    for (int j = 0; j < N; ++j) a[i * N + j] = gram[i * kMaxWindow + j];
+    for (int j = 0; j < N; ++j) a[i][j] = red[i <= j ? gram_index(i, j) : gram_index(j, i)];
+    a[i][i] = a[i][i] + reg;
   }
-  ldlt_solve<N>(a, N, b, N, x);
+  ldlt_solve_regs<N>(a, b, x);
+  double norm = 0.0;
 #pragma unroll
-  for (int i = 0; i < N; ++i) coef[i] = x[i];
+  for (int i = 0; i < N; ++i) {
+    gamma[i] = x[i];
+    norm += x[i] * x[i];
+  }
+  return norm;
 }
 
 // Reduce the body's Gram partials in chunk order and run the reference
@@ -691,32 +697,16 @@ __device__ void gram_solve(const DeviceState& d, int b, const int* iters, bool w
     meta[2] = start;
     int accept = 0;
     if (count >= 1) {
-      double gram[kMaxWindow * kMaxWindow], rhs[kMaxWindow], coef[kMaxWindow];
-      for (int a = 0; a < count; ++a) {
-        rhs[a] = red[kRhsIndex + a];
-        for (int bb = a; bb < count; ++bb) gram[a * kMaxWindow + bb] = red[gram_index(a, bb)];
-      }
-      double trace = 0.0;
-      for (int a = 0; a < count; ++a) {
-        trace += gram[a * kMaxWindow + a];
-        for (int bb = a + 1; bb < count; ++bb) gram[bb * kMaxWindow + a] = gram[a * kMaxWindow + bb];
-      }
-      const double reg = fmax(1.0e-6 * trace / static_cast<double>(count), 1.0e-12);
-      for (int a = 0; a < count; ++a) gram[a * kMaxWindow + a] = gram[a * kMaxWindow + a] + reg;
-      switch (count) {
-        case 1: solve_fixed<1>(gram, rhs, coef); break;
-        case 2: solve_fixed<2>(gram, rhs, coef); break;
-        case 3: solve_fixed<3>(gram, rhs, coef); break;
-        case 4: solve_fixed<4>(gram, rhs, coef); break;
-        case 5: solve_fixed<5>(gram, rhs, coef); break;
-        case 6: solve_fixed<6>(gram, rhs, coef); break;
-        case 7: solve_fixed<7>(gram, rhs, coef); break;
-        default: solve_fixed<8>(gram, rhs, coef); break;
-      }
       double norm = 0.0;
-      for (int a = 0; a < count; ++a) {
-        gamma[a] = coef[a];
-        norm += coef[a] * coef[a];
+      switch (count) {
+        case 1: norm = anderson_coefficients<1>(red, gamma); break;
+        case 2: norm = anderson_coefficients<2>(red, gamma); break;
+        case 3: norm = anderson_coefficients<3>(red, gamma); break;
+        case 4: norm = anderson_coefficients<4>(red, gamma); break;
+        case 5: norm = anderson_coefficients<5>(red, gamma); break;
+        case 6: norm = anderson_coefficients<6>(red, gamma); break;
+        case 7: norm = anderson_coefficients<7>(red, gamma); break;
+        default: norm = anderson_coefficients<8>(red, gamma); break;
       }
       if (sqrt(norm) >= 10.0) {
         count = 0;
@@ -1030,45 +1020,44 @@ __global__ __launch_bounds__(kBlock) void k_element_operator(DeviceState d) {
   }
 }
 
-// Rows of the assembled operator, one thread per row of a vertex chunk:
-// block (i, j) = sum over the incident elements of i, in incidence order, of
-// K_e[k][l] with idx[e][k] = i and idx[e][l] = j; minus A_ij I in the defect
-// mode.
+// Rows of the assembled operator: workgroup (chunk, slot), one thread per
+// row of the vertex chunk: block (i, j) = sum over the incident elements of
+// i, in incidence order, of K_e[k][l] with idx[e][k] = i and idx[e][l] = j;
+// minus A_ij I in the defect mode.
 __global__ __launch_bounds__(kBlock) void k_operator_assemble(DeviceState d) {
-  const int ci = blockIdx.x;
+  const int ci = blockIdx.x, s = blockIdx.y;
+  const int w = d.kwidth[ci];
+  if (s >= w) return;
   const Chunk c = d.vchunks[ci];
   const int i = c.begin + threadIdx.x;
-  const int w = d.kwidth[ci];
   const size_t base = static_cast<size_t>(d.kbase[ci]);
   const int K = d.K;
-  for (int s = 0; s < w; ++s) {
-    const size_t blk = base + static_cast<size_t>(s) * kBlock + threadIdx.x;
-    double acc[9];
+  const size_t blk = base + static_cast<size_t>(s) * kBlock + threadIdx.x;
+  double acc[9];
 #pragma unroll
-    for (int t = 0; t < 9; ++t) acc[t] = 0.0;
-    const int j = i < c.end ? d.kcol[blk] : -1;
-    if (j >= 0) {
-      for (int a = d.inc_off[i]; a < d.inc_off[i + 1]; ++a) {
-        const int ent = d.inc_ent[a];
-        const int e = ent / K;
-        for (int l = 0; l < K; ++l) {
-          if (d.idx[e * K + l] != j) continue;
-          const double* kb = d.ke + (static_cast<size_t>(ent) * K + l) * 9;
+  for (int t = 0; t < 9; ++t) acc[t] = 0.0;
+  const int j = i < c.end ? d.kcol[blk] : -1;
+  if (j >= 0) {
+    for (int a = d.inc_off[i]; a < d.inc_off[i + 1]; ++a) {
+      const int ent = d.inc_ent[a];
+      const int e = ent / K;
+      for (int l = 0; l < K; ++l) {
+        if (d.idx[e * K + l] != j) continue;
+        const double* kb = d.ke + (static_cast<size_t>(ent) * K + l) * 9;
 #pragma unroll
-          for (int t = 0; t < 9; ++t) acc[t] += kb[t];
-        }
-      }
-      if (d.defect) {
-        const double a = d.kdiag[blk];
-        acc[0] -= a;
-        acc[4] -= a;
-        acc[8] -= a;
+        for (int t = 0; t < 9; ++t) acc[t] += kb[t];
       }
     }
-    double* out = d.kval + 9 * base + static_cast<size_t>(s) * 9 * kBlock + threadIdx.x;
-#pragma unroll
-    for (int t = 0; t < 9; ++t) out[t * kBlock] = acc[t];
+    if (d.defect) {
+      const double a = d.kdiag[blk];
+      acc[0] -= a;
+      acc[4] -= a;
+      acc[8] -= a;
+    }
   }
+  double* out = d.kval + 9 * base + static_cast<size_t>(s) * 9 * kBlock + threadIdx.x;
+#pragma unroll
+  for (int t = 0; t < 9; ++t) out[t * kBlock] = acc[t];
 }
 
 // work = K mu + rhs (gather with accumulate, adjoint.py:939), or
@@ -1494,7 +1483,7 @@ void spmv_pair(const DeviceState& d, double* out, const double* base, const int*
                        static_cast<const In*>(d.rhs4), y4, active, d.s_tb, d.B);
   if (d.t_nitems)
     hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_t), dim3(kBlock), prefix_lds(d), stream, d.t_items,
-                       d.t_covers, val, static_cast<const In*>(y4), out, d.gorig, base, active, d.t_tb, d.B);
+                       d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base, active, d.t_tb, d.B);
 }
 
 // Dispatch on the stored precision.
@@ -1546,7 +1535,7 @@ void launch_adjoint_setup(const DeviceState& d, const StepScalars& s, const doub
   hipLaunchKernelGGL(k_jacobian, dim3((total + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d, x, targets);
   const int rows = d.E * d.K;
   hipLaunchKernelGGL(k_element_operator, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_operator_assemble, dim3(d.nvc), dim3(kBlock), 0, stream, d);
+  hipLaunchKernelGGL(k_operator_assemble, dim3(d.nvc, d.kmaxwidth), dim3(kBlock), 0, stream, d);
 }
 
 void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStream_t stream) {

@@ -10,8 +10,9 @@
 // descendants of i are the postorder range ending at i. So row r of S is the
 // dense range [r - subtree(r) + 1, r]. Where a row's parent has it as its
 // only child, the parent's range is the child's plus one column, so a chain
-// of such rows (a supernode) is one dense trapezoid, stored as a block. The
-// values are the factor's own numbers, only placed differently.
+// of such rows (a supernode) is one dense trapezoid, stored as a block; a
+// small subtree is stored whole as one block. The values are the factor's
+// own numbers, only placed differently.
 #include <algorithm>
 #include <stdexcept>
 
@@ -20,6 +21,10 @@
 namespace dphd {
 
 namespace {
+
+// Largest subtree stored as one block (a leaf of the dissection has at most
+// 24 vertices).
+constexpr int kAmalgamate = 32;
 
 // Structural elimination tree of the symmetric pattern of `a` in the order
 // `perm` (perm[position] = original index). parent[i] = -1 for roots.
@@ -114,15 +119,31 @@ FactorLayout factor_layout(const Csr& a, const Factor& f) {
   out.post.resize(n);
   for (int o = 0; o < n; ++o) out.post[o] = slot[post_of[pos[o]]];
 
-  // Supernode chains and their blocks.
+  // Blocks: a whole subtree of at most kAmalgamate vertices (the leaves of
+  // the dissection, whose many short chains would otherwise be tiny blocks)
+  // is one block over its own column range, zero above the diagonal and left
+  // of each row's range; elsewhere a block is one supernode chain.
   for (int k = 0; k < n;) {
     const int k0 = k;
-    while (k + 1 < n && parent[order[k]] == order[k + 1] && children[order[k + 1]] == 1) ++k;
-    const int k1 = k + 1;  // rows [k0, k1)
+    int k1 = k0 + 1;  // rows [k0, k1)
+    int lo_pos = k0 - size[order[k0]] + 1;
+    int top = -1;     // the largest subtree starting at k0 that is small enough
+    for (int e = order[k0]; post_of[e] - size[e] + 1 == k0 && size[e] <= kAmalgamate; e = parent[e]) {
+      top = e;
+      if (parent[e] < 0) break;
+    }
+    if (top >= 0 && size[top] > 1) {
+      k1 = k0 + size[top];
+      lo_pos = k0;
+    } else {
+      while (k + 1 < n && parent[order[k]] == order[k + 1] && children[order[k + 1]] == 1) ++k;
+      k1 = k + 1;
+    }
+    k = k1 - 1;
     FactorBlock blk;
     blk.row0 = slot[k0];
     blk.rows = k1 - k0;
-    blk.lo = slot[k0 - size[order[k0]] + 1];
+    blk.lo = slot[lo_pos];
     blk.col0 = blk.lo & ~3;
     const int end_col = blk.row0 + blk.rows;  // exclusive; rows end at themselves
     blk.quads = (end_col - blk.col0 + 3) / 4;
@@ -138,7 +159,7 @@ FactorLayout factor_layout(const Csr& a, const Factor& f) {
         if (c < blk.lo || c > row) throw std::logic_error("inverse factor entry outside its descendant range");
         dst[c - blk.col0] = f.s.val[p];
       }
-      out.entries += row - blk.lo + 1;
+      out.entries += size[i];  // the row's own dense range
     }
     out.blocks.push_back(blk);
     k = k1;

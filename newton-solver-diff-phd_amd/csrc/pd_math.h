@@ -292,6 +292,87 @@ DPHD_HD void ldlt_solve(const double* a_in, int ld, const double* b, int n, doub
   for (int i = 0; i < n; ++i) x[perm[i]] = y[i];
 }
 
+// ldlt_solve for a compile-time size, with the same operations in the same
+// order, written so that no local array is indexed by a run-time value (the
+// pivot swaps and the permutation are compare-and-select over compile-time
+// indices): on the device everything stays in registers.
+template <int N>
+DPHD_HD void ldlt_solve_regs(const double (&a_in)[N][N], const double (&b)[N], double (&x)[N]) {
+  double m[N][N];
+  int perm[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    perm[i] = i;
+#pragma unroll
+    for (int j = 0; j < N; ++j) m[i][j] = a_in[i][j];
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    int biggest = k;
+    double best = fabs(m[k][k]);
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      if (fabs(m[i][i]) > best) {
+        biggest = i;
+        best = fabs(m[i][i]);
+      }
+    }
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      if (i == biggest) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          const double t = m[k][j];
+          m[k][j] = m[i][j];
+          m[i][j] = t;
+        }
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+          const double t = m[r][k];
+          m[r][k] = m[r][i];
+          m[r][i] = t;
+        }
+        const int t = perm[k];
+        perm[k] = perm[i];
+        perm[i] = t;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < k; ++j) m[k][k] = m[k][k] - m[k][j] * m[k][j] * m[j][j];
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+#pragma unroll
+      for (int j = 0; j < k; ++j) m[i][k] = m[i][k] - m[i][j] * m[j][j] * m[k][j];
+      if (m[k][k] != 0.0) m[i][k] = m[i][k] / m[k][k];
+    }
+  }
+  double y[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double v = 0.0;
+#pragma unroll
+    for (int t = 0; t < N; ++t) v = perm[i] == t ? b[t] : v;
+    y[i] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < i; ++j) y[i] = y[i] - m[i][j] * y[j];
+#pragma unroll
+  for (int i = 0; i < N; ++i) y[i] = fabs(m[i][i]) > kDblMin ? y[i] / m[i][i] : 0.0;
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    const int i = N - 1 - r;
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) y[i] = y[i] - m[j][i] * y[j];
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int t = 0; t < N; ++t)
+      if (perm[i] == t) x[t] = y[i];
+}
+
 DPHD_HD M3 nh_prox_hessian(V3 sigma, double mu, double lame, double penalty) {
   const double product = sigma.x * sigma.y * sigma.z;
   const double log_j = log(fmax(product, 1.0e-12));

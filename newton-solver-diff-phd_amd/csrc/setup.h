@@ -42,10 +42,11 @@ struct Factor {
 // contiguously, so row r of S is the dense range [lo(r), r]. A chain of rows
 // whose parents have a single child (a supernode: the rows of one
 // separator, typically) shares lo, so the chain is one dense block: rows
-// [row0, row0 + rows) by columns [lo, row0 + rows), stored row-major from
-// the 4-aligned column lo & ~3 with `quads` quads (4 columns) per row. Block
-// entries the factor does not hold (above the diagonal, left and right
-// alignment, exact zeros of the factor) are zero.
+// [row0, row0 + rows) by columns [lo, row0 + rows); a small subtree is one
+// block as a whole. A block is stored row-major from the 4-aligned column
+// lo & ~3 with `quads` quads (4 columns) per row. Entries the factor does not
+// hold (above the diagonal, left of a row's range, alignment, exact zeros of
+// the factor) are zero.
 struct FactorBlock {
   int row0 = 0, rows = 0;  // postorder rows
   int lo = 0;              // first column of every row (postorder)

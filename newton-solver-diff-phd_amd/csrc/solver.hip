@@ -308,7 +308,7 @@ void build_solver(dphd_solver* s, double dt) {
   std::vector<double> fval;
   std::vector<RowItem> s_items;
   std::vector<ColGroup> t_items;
-  std::vector<ColCover> t_covers;
+  std::vector<ColEntry> t_entries;
   std::vector<int> s_tb(B + 1, 0), t_tb(B + 1, 0);
   for (int b = 0; b < B; ++b) {
     const int v0 = s->voff[b], n = s->voff[b + 1] - v0, c0 = slot0[b];
@@ -332,7 +332,8 @@ void build_solver(dphd_solver* s, double dt) {
       }
     }
     fval.insert(fval.end(), L.val.begin(), L.val.end());
-    // Column groups of 256 slots and the blocks that cover each of them.
+    // Column groups of 256 slots and, per group, the rows (from the group's
+    // first column on) of the blocks that cover it, block by block.
     const int groups = (L.slots + kColsPerGroup - 1) / kColsPerGroup;
     std::vector<std::vector<int>> cover(groups);
     for (size_t k = 0; k < L.blocks.size(); ++k) {
@@ -342,18 +343,23 @@ void build_solver(dphd_solver* s, double dt) {
     }
     t_tb[b] = static_cast<int>(t_items.size());
     for (int g = 0; g < groups; ++g) {
-      const int b0 = static_cast<int>(t_covers.size());
+      const int gc0 = g * kColsPerGroup;  // body-local first column of the group
+      const int e0 = static_cast<int>(t_entries.size());
       for (const int k : cover[g]) {
         const FactorBlock& fb = L.blocks[k];
-        ColCover cv{};
-        cv.offset = voff_b + fb.offset;
-        cv.row0 = c0 + fb.row0;
-        cv.rows = fb.rows;
-        cv.col0 = c0 + fb.col0;
-        cv.quads = fb.quads;
-        t_covers.push_back(cv);
+        const long long width = 4 * static_cast<long long>(fb.quads);
+        // Rows before the group's first column hold none of its columns.
+        for (int r = std::max(fb.row0, gc0); r < fb.row0 + fb.rows; ++r) {
+          ColEntry en{};
+          en.rowoff = voff_b + fb.offset + width * (r - fb.row0);
+          en.col0 = c0 + fb.col0;
+          en.quads = fb.quads;
+          en.yrow = c0 + r;
+          t_entries.push_back(en);
+        }
       }
-      t_items.push_back(ColGroup{c0 + g * kColsPerGroup, b0, static_cast<int>(t_covers.size()), c0 + L.slots});
+      if (t_entries.size() > static_cast<size_t>(INT32_MAX)) throw std::invalid_argument("too many factor rows");
+      t_items.push_back(ColGroup{c0 + gc0, e0, static_cast<int>(t_entries.size()), c0 + L.slots});
     }
     L = FactorLayout{};
   }
@@ -477,6 +483,7 @@ void build_solver(dphd_solver* s, double dt) {
   d.gpost = dupload(o, gpost);
   d.gorig = dupload(o, gorig);
   d.f32 = defect;
+  const int64_t factor_values = static_cast<int64_t>(fval.size());
   if (defect) {
     const std::vector<float> v32(fval.begin(), fval.end());
     d.fval = dupload(o, v32);
@@ -490,7 +497,7 @@ void build_solver(dphd_solver* s, double dt) {
   d.t_items = dupload(o, t_items);
   d.t_nitems = static_cast<int>(t_items.size());
   d.t_tb = dupload(o, t_tb);
-  d.t_covers = dupload(o, t_covers);
+  d.t_entries = dupload(o, t_entries);
   d.g_s = grid_for((d.s_nitems + 3) / 4);
   d.g_t = grid_for(d.t_nitems);
   if (defect) {
@@ -549,6 +556,7 @@ void build_solver(dphd_solver* s, double dt) {
   d.kcol = dupload(o, kcol);
   if (defect) d.kdiag = dupload(o, kdiag);
   d.kblocks = static_cast<int>(kcol.size());
+  d.kmaxwidth = std::max(1, *std::max_element(kwidth.begin(), kwidth.end()));
   d.material = pd ? 0 : 1;
   d.has_muscles = muscles ? 1 : 0;
   d.integrate = s->cfg.integrate_pinned_velocities ? 1 : 0;
@@ -579,6 +587,8 @@ void build_solver(dphd_solver* s, double dt) {
   s->info.factor_nnz = znnz;
   s->info.matrix_nnz = annz;
   s->info.jacobian_bytes = jac_bytes;
+  s->info.factor_values = factor_values;
+  s->info.factor_slots = slots;
   s->info.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

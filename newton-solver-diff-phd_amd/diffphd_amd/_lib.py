@@ -47,6 +47,8 @@ class Info(ctypes.Structure):
         ("matrix_nnz", ctypes.c_int64),
         ("jacobian_bytes", ctypes.c_int64),
         ("setup_seconds", ctypes.c_double),
+        ("factor_values", ctypes.c_int64),
+        ("factor_slots", ctypes.c_int64),
     ]
 
 
@@ -90,6 +92,9 @@ SIGNATURES = {
 _lib = None
 
 
+ABI_VERSION = 2  # DPHD_ABI_VERSION of include/diffphd.h
+
+
 def lib():
     """Load the native library once; raise loudly when it is absent."""
     global _lib
@@ -101,6 +106,10 @@ def lib():
             fn = getattr(handle, name)
             fn.argtypes = args
             fn.restype = res
+        if handle.dphd_version() != ABI_VERSION:
+            raise NativeLibraryMissing(
+                f"{LIBRARY} has ABI {handle.dphd_version()}, this package needs {ABI_VERSION}; rebuild it"
+            )
         _lib = handle
     return _lib
 
