@@ -140,6 +140,57 @@ def bytes_adjoint_iteration(n, e, z, s=8, window=5, s_j=8):
 # ---------------------------------------------------------------------------
 
 
+def factor_probe(solver, dev, n_total, z_total, s_bytes, args):
+    """The S and S^T products of every body (the dominant kernels), timed with HIP events on the stream they
+    are launched on (torch's current stream). One apply_inverse = pack + S + S^T; per launch = half."""
+    import torch
+
+    rhs = torch.randn((n_total, 3), dtype=torch.float64, device=dev)
+    for _ in range(3):
+        solver.apply_inverse(rhs)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(args.probe_reps):
+        solver.apply_inverse(rhs)
+    ev1.record()
+    torch.cuda.synchronize()
+    per_launch_s = ev0.elapsed_time(ev1) / 1e3 / (2 * args.probe_reps)
+    v_half = 24 if s_bytes == 8 else 12  # the S-product vectors are FP32 in the FP32-factor mode
+    # Algorithmic bytes of the pair (S: FP64 or FP32 in and out; S^T: in, FP64 out), per launch.
+    launch_bytes = 0.5 * (
+        bytes_product(n_total, z_total, s_bytes, v_half, v_half) + bytes_product(n_total, z_total, s_bytes, v_half, 24)
+    )
+    info = solver.info
+    slots = float(info.factor_slots)
+    vec4 = 4 * v_half // 3
+    stored_bytes = float(info.factor_values) * s_bytes + 0.5 * (3 * slots * vec4 + n_total * 24)
+    log(f"[bench] factor product probe {per_launch_s * 1e6:.1f} us/launch, {launch_bytes / per_launch_s / 1e9:.0f} "
+        f"GB/s algorithmic, {stored_bytes / per_launch_s / 1e9:.0f} GB/s of stored bytes")
+    return {"launch_s": per_launch_s, "launch_bytes": launch_bytes, "stored_bytes": stored_bytes}
+
+
+def copy_ceiling(dev):
+    """A plain 256 MiB device copy: the achievable streaming rate, and the reference kernel that calibrates
+    the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    src = torch.empty(2**25, dtype=torch.float64, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(10):
+        dst.copy_(src)
+    ev1.record()
+    torch.cuda.synchronize()
+    gbps = 2 * src.numel() * 8 * 10 / (ev0.elapsed_time(ev1) / 1e3) / 1e9
+    log(f"[bench] copy ceiling {gbps:.0f} GB/s")
+    return gbps
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,13 +199,16 @@ def main():
     ap.add_argument("--bodies", type=int, default=256)
     ap.add_argument("--kmin", type=int, default=4)
     ap.add_argument("--kmax", type=int, default=16)
-    ap.add_argument("--precision", type=int, default=64, choices=(32, 64))
+    ap.add_argument("--precision", type=int, default=32, choices=(32, 64),
+                    help="factor precision: 32 (the workload's FP32, FP64 defect correction) or 64")
     ap.add_argument("--chunk", type=int, default=0, help="iterations per captured graph (0 = default)")
     ap.add_argument("--plain-bodies", type=int, default=None, help="bodies timed on the plain path (default all)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-bodies", type=int, default=0, help="0 = every body")
     ap.add_argument("--probe-reps", type=int, default=20)
+    ap.add_argument("--probe-only", action="store_true",
+                    help="only the factor-product probe and the copy calibration (for counter runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -184,6 +238,13 @@ def main():
     )
     solver.prepare(dt)
     log(f"[bench] batched setup {time.time() - t0:.1f}s, nnz(S) {solver.info.factor_nnz}")
+    s_bytes = 8 if args.precision == 64 else 4
+    if args.probe_only:
+        z_all = float(np.sum(solver.body_factor_nnz()))
+        probe = factor_probe(solver, dev, n_total, z_all, s_bytes, args)
+        copy_gbps = copy_ceiling(dev)
+        print(json.dumps({"probe_launch_us": probe["launch_s"] * 1e6, "copy_ceiling_gbps": copy_gbps}))
+        return
 
     for _ in range(args.warmup):
         rollout(solver, model, 1, dt, idx, tgt)
@@ -208,55 +269,15 @@ def main():
     n_b = np.diff(voff)
     e_b = np.diff(eoff)
     z_b = np.array(solver.body_factor_nnz(), dtype=np.float64)
-    s_bytes = 8 if args.precision == 64 else 4
+    z_total = float(z_b.sum())
     per_fwd = bytes_forward_iteration(n_b, e_b, z_b, s=s_bytes)
     per_adj = bytes_adjoint_iteration(n_b, e_b, z_b, s=s_bytes)
     total_bytes = float((fwd_iters * per_fwd).sum() + (adj_iters * per_adj).sum())
     elem_iters = float((fwd_iters * e_b).sum() + (adj_iters * e_b).sum())
     rollout_gbps = total_bytes / elapsed / 1e9
 
-    # Roofline probe: the S and S^T products of every body (the dominant
-    # kernels), timed with HIP events on the stream they are launched on
-    # (torch's current stream). One apply_inverse = pack + S + S^T.
-    rhs = torch.randn((n_total, 3), dtype=torch.float64, device=dev)
-    for _ in range(3):
-        solver.apply_inverse(rhs)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(args.probe_reps):
-        solver.apply_inverse(rhs)
-    ev1.record()
-    torch.cuda.synchronize()
-    per_launch_s = ev0.elapsed_time(ev1) / 1e3 / (2 * args.probe_reps)
-    z_total = float(z_b.sum())
-    v_half = 24 if args.precision == 64 else 12  # the S-product vectors are FP32 in the FP32-factor mode
-    # Algorithmic bytes of the pair (S: in V/2.. out; S^T: in, out FP64), per launch.
-    launch_bytes = 0.5 * (bytes_product(n_total, z_total, s_bytes, v_half, v_half) + bytes_product(n_total, z_total, s_bytes, v_half, 24))
-    probe_gbps = launch_bytes / per_launch_s / 1e9
-    info = solver.info
-    slots = float(info.factor_slots)
-    vec4 = 32 if args.precision == 64 else 16
-    physical_bytes = float(info.factor_values) * s_bytes + 0.5 * (2 * slots * vec4 + slots * vec4 + n_total * 24)
-    log(f"[bench] factor product probe {per_launch_s * 1e6:.1f} us/launch, {probe_gbps:.0f} GB/s algorithmic, "
-        f"{physical_bytes / per_launch_s / 1e9:.0f} GB/s of stored bytes")
-    del rhs
-
-    # Copy-kernel ceiling: a plain 256 MiB device copy (also the reference
-    # kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters).
-    src = torch.empty(2**25, dtype=torch.float64, device=dev).fill_(1.0)
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    ev0.record()
-    for _ in range(10):
-        dst.copy_(src)
-    ev1.record()
-    torch.cuda.synchronize()
-    copy_gbps = 2 * src.numel() * 8 * 10 / (ev0.elapsed_time(ev1) / 1e3) / 1e9
-    del src, dst
-    log(f"[bench] copy ceiling {copy_gbps:.0f} GB/s")
+    probe = factor_probe(solver, dev, n_total, z_total, s_bytes, args)
+    copy_gbps = copy_ceiling(dev)
 
     # Plain per-body path: one single-body solver per body, run one after another.
     plain = None
@@ -334,17 +355,17 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": "k_factor_rows + k_factor_cols (S and S^T products of all bodies), per launch",
-            "achieved": probe_gbps,
+            "achieved": probe["launch_bytes"] / probe["launch_s"] / 1e9,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
-            "frac": probe_gbps / HBM_PEAK_GBPS,
+            "frac": probe["launch_bytes"] / probe["launch_s"] / 1e9 / HBM_PEAK_GBPS,
             "traffic": traffic,
-            "launch_us": per_launch_s * 1e6,
-            "bytes_per_launch": launch_bytes,
-            "stored_bytes_per_launch": physical_bytes,
-            "stored_gbps": physical_bytes / per_launch_s / 1e9,
+            "launch_us": probe["launch_s"] * 1e6,
+            "bytes_per_launch": probe["launch_bytes"],
+            "stored_bytes_per_launch": probe["stored_bytes"],
+            "stored_gbps": probe["stored_bytes"] / probe["launch_s"] / 1e9,
             "copy_ceiling_gbps": copy_gbps,
-            "frac_of_copy": probe_gbps / copy_gbps,
+            "frac_of_copy": probe["launch_bytes"] / probe["launch_s"] / 1e9 / copy_gbps,
         },
         "cpu_baseline": cpu,
     }
