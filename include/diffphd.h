@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPHD_ABI_VERSION 2
+#define DPHD_ABI_VERSION 3
 
 enum {
   DPHD_OK = 0,
@@ -119,6 +119,12 @@ int dphd_set_particles(dphd_solver* s, const float* inverse_mass, const double* 
 int dphd_set_materials(dphd_solver* s, const double* materials);
 /* Optional fibres (DP/solver.py:429-462): directions [E,3], stiffness [E]. */
 int dphd_set_muscles(dphd_solver* s, const double* directions, const double* stiffness);
+/* Optional planar vertex energies (PlanarEnergies, DP/hex.py:204-260, and
+ * their matrix diagonal, DP/solver.py:542-545): `count` distinct vertices
+ * [count] int32 with stiffness [count] float64 >= 0 against one plane
+ * (nx, ny, nz, offset), scaled to a unit normal. count = 0 removes them.
+ * Forward only: dphd_backward rejects a solver that has them. */
+int dphd_set_planar(dphd_solver* s, int count, const int* vertices, const double* stiffness, const double* plane);
 /* Mutable between steps (DP/solver.py:206-209). */
 int dphd_set_tolerances(dphd_solver* s, int max_iterations, double abs_tol, double rel_tol);
 /* Element weights, assembly and factor per body (DP/solver.py:491-552). */
@@ -138,11 +144,14 @@ int dphd_step(dphd_solver* s, const float* q_in, const float* qd_in, const float
 /* Backward of one recorded step (Adjoint.backward, DP/adjoint.py:1088-1259).
  * Seeds from gq_out/gqd_out (NULL = zero); accumulates (+=) into gq_in,
  * gqd_in, gf_in [N,3] float32, gmaterials [E,2] float64 (mu, lambda; NULL
- * skips material gradients) and gdirichlet [N,3] float32 (non-NULL when the
- * step used explicit Dirichlet targets). */
+ * skips material gradients), gdirichlet [N,3] float32 (non-NULL when the
+ * step used explicit Dirichlet targets) and gactivations [E] float64
+ * (d/du of the Tet4 fibre contraction u = 1 - target, DP/adjoint.py:595-620;
+ * NULL skips them, non-NULL needs Tet4 fibres). Material and activation
+ * gradients pass the reference's aggregate guard (DP/adjoint.py:622-659). */
 int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle_targets, const float* gq_out,
                   const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
-                  float* gdirichlet, void* stream);
+                  float* gdirichlet, double* gactivations, void* stream);
 
 /* Status word (DP/solver.py:380-390): OR of all bodies since the last clear. */
 int dphd_read_status(dphd_solver* s, int clear, int* bits);
@@ -154,6 +163,11 @@ int dphd_synchronize(dphd_solver* s);
 /* Device-side application of A^{-1} = S^T S of every body to [N,3] float64
  * device arrays (SparseInverse.apply, DP/sparse.py:350-352). */
 int dphd_apply_inverse(dphd_solver* s, const double* rhs, double* out, void* stream);
+
+/* det(F) at the eight Gauss samples of every Hex8 cell, out [E,8] float64,
+ * from float32 positions q [N,3] (SolverDiffPhD.deformation_determinants,
+ * DP/solver.py:360-378; DP/hex.py:190-201). Device pointers; Hex8 only. */
+int dphd_deformation_determinants(dphd_solver* s, const float* q, double* out, void* stream);
 
 /* Evaluate the pointwise projections of `count` float64 3x3 matrices on the
  * device (the reference's _evaluate_projections test kernel,

@@ -174,7 +174,8 @@ struct DeviceState {
   double* ascale = nullptr;  // [B*2] clip scales
   double* route = nullptr;   // [B*2] routing scale, diverged flag
   double* gmat_tmp = nullptr;  // [E*2]
-  int* valid = nullptr;        // [B]
+  double* gact_tmp = nullptr;  // [E]
+  int* valid = nullptr;        // [2*B] material, activation gradients pass the guard
   int* ones = nullptr;         // [B] all ones (every body active)
   int* eoff = nullptr;         // [B+1] element offsets
   // options
@@ -182,6 +183,10 @@ struct DeviceState {
   int has_muscles = 0;
   int integrate = 0;
   int defect = 0;  // FP32 factor: x_raw = x + S^T S (b - A x)
+  // planar vertex energies: stiffness per vertex (0 = none) or null, and the
+  // plane (unit normal, offset)
+  double* planar_w = nullptr;
+  double plane[4] = {0, 0, 1, 0};
 };
 
 struct StepScalars {
@@ -201,12 +206,13 @@ void launch_write_state(const DeviceState& d, const StepScalars& s, float* q, fl
 void launch_adjoint_setup(const DeviceState& d, const StepScalars& s, const double* x, const double* targets,
                           const float* gq_out, const float* gqd_out, hipStream_t stream);
 void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStream_t stream);
+// State, Dirichlet, material (gmat) and activation (gact, Tet4 fibres)
+// gradients of a finished adjoint solve; gmat/gact may be null.
 void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const double* x, const float* gqd_out,
-                            float* gq_in, float* gqd_in, float* gf_in, double* gmat, float* gdir,
+                            float* gq_in, float* gqd_in, float* gf_in, double* gmat, double* gact, float* gdir,
                             bool any_pinned, hipStream_t stream);
-
-void launch_material_finish(const DeviceState& d, double* gmat, hipStream_t stream);
 void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, hipStream_t stream);
+void launch_hex_determinants(const DeviceState& d, const float* q, double* out, hipStream_t stream);
 
 void launch_eval_projections(int count, const double* f, const double* dirs, double mu, double lame, double penalty,
                              double target, double* sigma, double* recon, double* rotation, double* volume,

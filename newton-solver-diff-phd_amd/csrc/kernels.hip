@@ -380,6 +380,16 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d) {
     if (i < c.end) {
       V3 t{0, 0, 0};
       for (int a = d.inc_off[i]; a < d.inc_off[i + 1]; ++a) t = t + ld3(d.contrib, d.inc_ent[a]);
+      if (d.planar_w) {
+        // planar_projection (hex.py:246-260): k (x - min(0, n.x + d) n - x_D).
+        const double kw = d.planar_w[i];
+        if (kw != 0.0) {
+          const V3 xi = ld3(d.x, i), nrm{d.plane[0], d.plane[1], d.plane[2]};
+          const double dist = dot(nrm, xi) + d.plane[3];
+          const V3 projected = xi - (dist < 0.0 ? dist : 0.0) * nrm;
+          t = t + kw * (projected - ld3(d.xd, i));
+        }
+      }
       const V3 old = ld3(d.p, i);
       V3 r = d.inertia[b] * ld3(d.qt, i) + t;
       if (d.defect) {
@@ -1387,30 +1397,88 @@ __global__ __launch_bounds__(kBlock) void k_adj_material_hex8(DeviceState d, con
   }
 }
 
-// Aggregate guard per body (adjoint.py:622-644), parallel over the body's elements.
-__global__ __launch_bounds__(kBlock) void k_adj_material_guard(DeviceState d) {
-  const int b = blockIdx.x;
-  __shared__ double smem[2 * kBlock / kWave];
-  __shared__ double res[2];
-  double part[2] = {0.0, 0.0};
-  for (int e = d.eoff[b] + threadIdx.x; e < d.eoff[b + 1]; e += kBlock) {
-    part[0] += d.gmat_tmp[2 * e];
-    part[1] += d.gmat_tmp[2 * e + 1];
+// Activation gradients of Tet4 fibres (adjoint.py:595-620): d/du of the
+// contraction u with target length a = 1 - u is
+// -sum_k z_k . (w_m / |F m|) (m . g_k) F m, zero for a disabled or collapsed
+// fibre.
+__global__ __launch_bounds__(kBlock) void k_adj_activation_tet4(DeviceState d, const double* x) {
+  const Chunk c = d.echunks[blockIdx.x];
+  const int e = c.begin + threadIdx.x;
+  if (e >= c.end) return;
+  const double wm = d.mw[e];
+  double value = 0.0;
+  if (wm != 0.0) {
+    V3 g[4];
+    tet_grads(d.poses, e, g);
+    M3 f = zero3();
+    for (int k = 0; k < 4; ++k) add_outer(f, ld3(x, d.idx[4 * e + k]), g[k]);
+    const V3 m = ld3(d.mdir, e);
+    const V3 fibre = mul(f, m);
+    const double len = sqrt(dot(fibre, fibre));
+    if (len > 2.220446049250313e-16) {
+      for (int k = 0; k < 4; ++k) {
+        const V3 z = ld3(d.sol, d.idx[4 * e + k]);
+        value += dot(z, (wm / len * dot(m, g[k])) * fibre);
+      }
+      value = -value;
+    }
   }
-  block_sum<2>(part, smem, res);
-  if (threadIdx.x != 0) return;
-  const bool zero_energy = d.material == 0 && d.nh[3 * b + 1] == 0.0;
-  const bool ok = !zero_energy && isfinite(res[0]) && isfinite(res[1]);
-  if (!ok) atomicOr(d.status, 16);
-  d.valid[b] = (ok && d.route[2 * b + 1] == 0.0) ? 1 : 0;
+  d.gact_tmp[e] = value;
 }
 
-__global__ __launch_bounds__(kBlock) void k_adj_material_add(DeviceState d, double* gmat) {
+// Aggregate guard per body (adjoint.py:622-644), parallel over the body's
+// elements: valid[2b] gates the material gradients, valid[2b+1] the
+// activation gradients; a non-finite aggregate of either sets status 16.
+__global__ __launch_bounds__(kBlock) void k_adj_parameter_guard(DeviceState d, int materials, int activations) {
+  const int b = blockIdx.x;
+  __shared__ double smem[3 * kBlock / kWave];
+  __shared__ double res[3];
+  double part[3] = {0.0, 0.0, 0.0};
+  for (int e = d.eoff[b] + threadIdx.x; e < d.eoff[b + 1]; e += kBlock) {
+    if (materials) {
+      part[0] += d.gmat_tmp[2 * e];
+      part[1] += d.gmat_tmp[2 * e + 1];
+    }
+    // Any non-finite value makes the sum non-finite (a NaN stays NaN, and
+    // infinities of both signs give NaN), which is the per-element test.
+    if (activations) part[2] += isfinite(d.gact_tmp[e]) ? 0.0 : NAN;
+  }
+  block_sum<3>(part, smem, res);
+  if (threadIdx.x != 0) return;
+  const bool zero_energy = materials && d.material == 0 && d.nh[3 * b + 1] == 0.0;
+  const bool material_ok = !zero_energy && isfinite(res[0]) && isfinite(res[1]);
+  const bool activation_ok = isfinite(res[2]);
+  if (!material_ok || !activation_ok) atomicOr(d.status, 16);
+  const bool live = d.route[2 * b + 1] == 0.0;
+  d.valid[2 * b] = (material_ok && live) ? 1 : 0;
+  d.valid[2 * b + 1] = (activation_ok && live) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_parameter_add(DeviceState d, double* gmat, double* gact) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= d.E) return;
-  if (!d.valid[d.ebody[e]]) return;
-  gmat[2 * e] += d.gmat_tmp[2 * e];
-  gmat[2 * e + 1] += d.gmat_tmp[2 * e + 1];
+  const int b = d.ebody[e];
+  if (gmat && d.valid[2 * b]) {
+    gmat[2 * e] += d.gmat_tmp[2 * e];
+    gmat[2 * e + 1] += d.gmat_tmp[2 * e + 1];
+  }
+  if (gact && d.valid[2 * b + 1]) gact[e] += d.gact_tmp[e];
+}
+
+// det(F) at the eight Gauss samples of every Hex8 cell from float32
+// positions (deformation_determinants, hex.py:190-201); one lane per sample.
+__global__ __launch_bounds__(kBlock) void k_hex_determinants(DeviceState d, const float* q, double* out) {
+  const int sample = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sample >= d.E * 8) return;
+  const int e = sample >> 3, s = sample & 7;
+  const int b = d.ebody[e];
+  M3 f = zero3();
+  for (int j = 0; j < 8; ++j) {
+    const int n = d.idx[8 * e + j];
+    const V3 x{static_cast<double>(q[3 * n]), static_cast<double>(q[3 * n + 1]), static_cast<double>(q[3 * n + 2])};
+    add_outer(f, x, hex_grad(d, b, s, j));
+  }
+  out[sample] = det3(f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1550,8 +1618,8 @@ void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStr
 }
 
 void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const double* x, const float* gqd_out,
-                            float* gq_in, float* gqd_in, float* gf_in, double* gmat, float* gdir, bool any_pinned,
-                            hipStream_t stream) {
+                            float* gq_in, float* gqd_in, float* gf_in, double* gmat, double* gact, float* gdir,
+                            bool any_pinned, hipStream_t stream) {
   hipLaunchKernelGGL(k_adj_route, dim3(d.B), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_adj_state_grads, dim3(d.nvc), dim3(kBlock), 0, stream, d, s.dt, gqd_out, gq_in, gqd_in,
                      gf_in);
@@ -1565,11 +1633,13 @@ void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const do
     else
       hipLaunchKernelGGL(k_adj_material_hex8, dim3(d.nec), dim3(kBlock), 0, stream, d, x);
   }
-}
-
-void launch_material_finish(const DeviceState& d, double* gmat, hipStream_t stream) {
-  hipLaunchKernelGGL(k_adj_material_guard, dim3(d.B), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_adj_material_add, dim3((d.E + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d, gmat);
+  if (gact) hipLaunchKernelGGL(k_adj_activation_tet4, dim3(d.nec), dim3(kBlock), 0, stream, d, x);
+  if (gmat || gact) {
+    hipLaunchKernelGGL(k_adj_parameter_guard, dim3(d.B), dim3(kBlock), 0, stream, d, gmat != nullptr ? 1 : 0,
+                       gact != nullptr ? 1 : 0);
+    hipLaunchKernelGGL(k_adj_parameter_add, dim3((d.E + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d, gmat,
+                       gact);
+  }
 }
 
 void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, hipStream_t stream) {
@@ -1581,6 +1651,11 @@ void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, 
     hipLaunchKernelGGL(k_pack4<double>, dim3(grid), dim3(kBlock), 0, stream, d.N, d.gpost, rhs,
                        static_cast<double*>(d.rhs4));
   spmv_dispatch(d, out, nullptr, d.ones, stream);
+}
+
+void launch_hex_determinants(const DeviceState& d, const float* q, double* out, hipStream_t stream) {
+  if (d.E == 0) return;
+  hipLaunchKernelGGL(k_hex_determinants, dim3((d.E * 8 + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d, q, out);
 }
 
 void launch_eval_projections(int count, const double* f, const double* dirs, double mu, double lame, double penalty,

@@ -86,6 +86,9 @@ struct dphd_solver {
   std::vector<double> gravity, density;
   std::vector<double> materials;
   std::vector<double> mdir_in, mstiff_in;
+  std::vector<int> planar_idx;    // planar vertex energies (hex.py:204-243)
+  std::vector<double> planar_w;
+  double plane[4] = {0, 0, 1, 0};  // unit normal, offset
   bool have_topology = false, have_particles = false, have_materials = false;
   // build products
   bool built = false;
@@ -277,6 +280,16 @@ void build_solver(dphd_solver* s, double dt) {
             in.fibre_w = muscles ? mw.data() + e0 : nullptr;
             in.free = free.data() + v0;
             in.inertia = inertia[b];
+            std::vector<int> plane_local;
+            std::vector<double> plane_w;
+            for (size_t j = 0; j < s->planar_idx.size(); ++j)
+              if (s->planar_idx[j] >= v0 && s->planar_idx[j] < v1) {
+                plane_local.push_back(s->planar_idx[j] - v0);
+                plane_w.push_back(s->planar_w[j]);
+              }
+            in.plane_vertices = plane_local.data();
+            in.plane_stiffness = plane_w.data();
+            in.plane_count = static_cast<int>(plane_local.size());
             amats[b] = assemble_matrix(in);
             const Factor factor = build_factor(amats[b]);
             factor_entries[b] = static_cast<int64_t>(factor.s.col.size());
@@ -547,7 +560,8 @@ void build_solver(dphd_solver* s, double dt) {
   d.ascale = dzeros<double>(o, 2 * static_cast<size_t>(B));
   d.route = dzeros<double>(o, 2 * static_cast<size_t>(B));
   d.gmat_tmp = dzeros<double>(o, 2 * static_cast<size_t>(E));
-  d.valid = dzeros<int>(o, B);
+  d.gact_tmp = dzeros<double>(o, static_cast<size_t>(E));
+  d.valid = dzeros<int>(o, 2 * static_cast<size_t>(B));
   d.ones = dupload(o, std::vector<int>(B, 1));
   d.eoff = dupload(o, s->eoff);
   d.jac = nullptr;  // allocated on the first backward, with ke and kval
@@ -559,6 +573,12 @@ void build_solver(dphd_solver* s, double dt) {
   d.kmaxwidth = std::max(1, *std::max_element(kwidth.begin(), kwidth.end()));
   d.material = pd ? 0 : 1;
   d.has_muscles = muscles ? 1 : 0;
+  if (!s->planar_idx.empty()) {
+    std::vector<double> pw(N, 0.0);
+    for (size_t j = 0; j < s->planar_idx.size(); ++j) pw[s->planar_idx[j]] = s->planar_w[j];
+    d.planar_w = dupload(o, pw);
+    for (int t = 0; t < 4; ++t) d.plane[t] = s->plane[t];
+  }
   d.integrate = s->cfg.integrate_pinned_velocities ? 1 : 0;
   s->targets = muscles ? dupload(o, std::vector<double>(E, 1.0)) : nullptr;
   void* host = nullptr;
@@ -650,9 +670,9 @@ int run_loop(dphd_solver* s, bool adjoint) {
 
 using dphd::set_error;
 
-#define DPHD_TRY(body)                                    \
+#define DPHD_TRY(...)                                     \
   try {                                                   \
-    body;                                                 \
+    __VA_ARGS__;                                          \
     return DPHD_OK;                                       \
   } catch (const std::invalid_argument& e) {              \
     set_error(e.what());                                  \
@@ -817,6 +837,35 @@ int dphd_set_muscles(dphd_solver* s, const double* directions, const double* sti
   })
 }
 
+int dphd_set_planar(dphd_solver* s, int count, const int* vertices, const double* stiffness, const double* plane) {
+  DPHD_TRY({
+    if (!s || s->N == 0 || count < 0 || (count > 0 && (!vertices || !stiffness || !plane)))
+      throw std::invalid_argument("dphd_set_planar: invalid arguments");
+    std::vector<int> idx(vertices, vertices + count);
+    std::vector<char> seen(s->N, 0);
+    for (const int i : idx) {
+      if (i < 0 || i >= s->N || seen[i]) throw std::invalid_argument("planar_energies needs distinct valid particle indices");
+      seen[i] = 1;
+    }
+    std::vector<double> w(stiffness, stiffness + count);
+    for (const double k : w)
+      if (!std::isfinite(k) || k < 0.0) throw std::invalid_argument("planar_energies stiffness must be finite and nonnegative");
+    double unit[4] = {0, 0, 1, 0};
+    if (count > 0) {
+      const double len = std::sqrt(plane[0] * plane[0] + plane[1] * plane[1] + plane[2] * plane[2]);
+      bool finite = std::isfinite(len);
+      for (int t = 0; t < 4; ++t) finite = finite && std::isfinite(plane[t]);
+      if (!finite || len <= 1.0e-5)
+        throw std::invalid_argument("planar_energies needs a finite (nx,ny,nz,offset) with nonzero normal");
+      for (int t = 0; t < 4; ++t) unit[t] = plane[t] / len;
+    }
+    s->planar_idx = std::move(idx);
+    s->planar_w = std::move(w);
+    for (int t = 0; t < 4; ++t) s->plane[t] = unit[t];
+    s->release();
+  })
+}
+
 int dphd_set_tolerances(dphd_solver* s, int max_iterations, double abs_tol, double rel_tol) {
   DPHD_TRY({
     if (!s || max_iterations < 1 || !std::isfinite(abs_tol) || abs_tol < 0 || !std::isfinite(rel_tol) || rel_tol < 0)
@@ -888,13 +937,21 @@ int dphd_step(dphd_solver* s, const float* q_in, const float* qd_in, const float
 
 int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle_targets, const float* gq_out,
                   const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
-                  float* gdirichlet, void* stream) {
+                  float* gdirichlet, double* gactivations, void* stream) {
   if (!s || !s->built) {
     set_error("dphd_backward: call dphd_build first");
     return DPHD_ESTATE;
   }
   if (!checkpoint || !gq_in || !gqd_in || !gf_in) {
     set_error("dphd_backward: null pointer");
+    return DPHD_EINVAL;
+  }
+  if (!s->planar_idx.empty()) {
+    set_error("dphd_backward: planar vertex energies have no adjoint (the reference records none for Hex8)");
+    return DPHD_ENOTIMPL;
+  }
+  if (gactivations && (s->Q != 1 || !s->d.has_muscles)) {
+    set_error("dphd_backward: activation gradients need Tet4 elements with muscle fibres");
     return DPHD_EINVAL;
   }
   DPHD_TRY({
@@ -917,9 +974,8 @@ int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle
     dphd::launch_adjoint_setup(s->d, sc, checkpoint, s->targets, gq_out, gqd_out, s->stream);
     const int chunks = dphd::run_loop(s, true);
     s->last_adj_iters = chunks * dphd::chunk_size(s);
-    dphd::launch_adjoint_outputs(s->d, sc, checkpoint, gqd_out, gq_in, gqd_in, gf_in, gmaterials, gdirichlet,
-                                 s->any_pinned, s->stream);
-    if (gmaterials) dphd::launch_material_finish(s->d, gmaterials, s->stream);
+    dphd::launch_adjoint_outputs(s->d, sc, checkpoint, gqd_out, gq_in, gqd_in, gf_in, gmaterials, gactivations,
+                                 gdirichlet, s->any_pinned, s->stream);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(s->ev_out, s->stream));
     HIP_CHECK(hipStreamWaitEvent(user, s->ev_out, 0));
@@ -959,6 +1015,19 @@ int dphd_apply_inverse(dphd_solver* s, const double* rhs, double* out, void* str
     HIP_CHECK(hipSetDevice(s->device));
     hipStream_t user = static_cast<hipStream_t>(stream);
     dphd::launch_apply_inverse(s->d, rhs, out, user);
+    HIP_CHECK(hipGetLastError());
+  })
+}
+
+int dphd_deformation_determinants(dphd_solver* s, const float* q, double* out, void* stream) {
+  if (s && s->Q != 8) {
+    set_error("deformation_determinants currently requires hex_mesh");
+    return DPHD_ENOTIMPL;
+  }
+  DPHD_TRY({
+    if (!s || !s->built || !q || !out) throw std::invalid_argument("dphd_deformation_determinants: invalid arguments");
+    HIP_CHECK(hipSetDevice(s->device));
+    dphd::launch_hex_determinants(s->d, q, out, static_cast<hipStream_t>(stream));
     HIP_CHECK(hipGetLastError());
   })
 }

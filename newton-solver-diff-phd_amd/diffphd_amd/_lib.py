@@ -75,16 +75,18 @@ SIGNATURES = {
     "dphd_set_particles": ([_P, _P, _P, _P], _I),
     "dphd_set_materials": ([_P, _P], _I),
     "dphd_set_muscles": ([_P, _P, _P], _I),
+    "dphd_set_planar": ([_P, _I, _P, _P, _P], _I),
     "dphd_set_tolerances": ([_P, _I, _D, _D], _I),
     "dphd_build": ([_P, _D], _I),
     "dphd_get_info": ([_P, ctypes.POINTER(Info)], _I),
     "dphd_body_nnz": ([_P, _P], _I),
     "dphd_step": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
-    "dphd_backward": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "dphd_backward": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "dphd_read_status": ([_P, _I, ctypes.POINTER(_I)], _I),
     "dphd_iterations": ([_P, _I, _P], _I),
     "dphd_synchronize": ([_P], _I),
     "dphd_apply_inverse": ([_P, _P, _P, _P], _I),
+    "dphd_deformation_determinants": ([_P, _P, _P, _P], _I),
     "dphd_eval_projections": ([_I, _P, _P, _D, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P], _I),
     "dphd_eval_jvps": ([_I, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P, _P], _I),
 }
@@ -92,7 +94,7 @@ SIGNATURES = {
 _lib = None
 
 
-ABI_VERSION = 2  # DPHD_ABI_VERSION of include/diffphd.h
+ABI_VERSION = 3  # DPHD_ABI_VERSION of include/diffphd.h
 
 
 def lib():

@@ -5,7 +5,8 @@ the checkpoint, like ``StepRecord`` (DP/adjoint.py:809-841). Backward runs
 the native adjoint (``Adjoint.backward``, DP/adjoint.py:1088-1259): it
 recomputes the projection Jacobian at the checkpoint, solves
 ``(A - K) mu = dL/dq+ + dL/dv+ / h`` with the solver's factor, and returns
-the state, external-force, material and Dirichlet-target gradients.
+the state, external-force, material, Dirichlet-target and fibre-activation
+gradients (the last with respect to the contraction ``u``, DP/adjoint.py:595-620).
 """
 
 from __future__ import annotations
@@ -15,7 +16,7 @@ import torch
 
 class StepFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, solver, dt, q, qd, f, materials, dirichlet, targets):
+    def forward(ctx, solver, dt, q, qd, f, materials, dirichlet, targets, activations):
         n = q.shape[0]
         q_out = torch.empty_like(q)
         qd_out = torch.empty_like(qd)
@@ -38,6 +39,8 @@ class StepFunction(torch.autograd.Function):
         ctx.materials_dtype = materials.dtype
         ctx.save_for_backward(checkpoint, targets if targets is not None else torch.empty(0, device=q.device))
         ctx.has_targets = targets is not None
+        ctx.activations_shape = None if activations is None else tuple(activations.shape)
+        ctx.activations_dtype = None if activations is None else activations.dtype
         return q_out, qd_out
 
     @staticmethod
@@ -54,6 +57,8 @@ class StepFunction(torch.autograd.Function):
         want_materials = ctx.needs_input_grad[5]
         gmat = torch.zeros((ctx.materials_shape[0], 2), dtype=torch.float64, device=dev) if want_materials else None
         gdir = torch.zeros_like(gq) if ctx.has_dirichlet else None
+        want_activations = ctx.activations_shape is not None and ctx.needs_input_grad[8]
+        gact = torch.zeros(ctx.activations_shape, dtype=torch.float64, device=dev) if want_activations else None
         solver._native_backward(
             checkpoint,
             targets if ctx.has_targets else None,
@@ -64,9 +69,11 @@ class StepFunction(torch.autograd.Function):
             gf,
             gmat,
             gdir,
+            gact,
         )
         gmaterials = None
         if want_materials:
             gmaterials = torch.zeros(ctx.materials_shape, dtype=ctx.materials_dtype, device=dev)
             gmaterials[:, :2] = gmat.to(ctx.materials_dtype)
-        return None, None, gq, gqd, gf, gmaterials, gdir, None
+        gactivations = None if gact is None else gact.to(ctx.activations_dtype)
+        return None, None, gq, gqd, gf, gmaterials, gdir, None, gactivations

@@ -8,8 +8,9 @@ contacts, dt, *, observables=None, dirichlet_positions=None)`` (:689-786),
 Gradients: where the reference records a ``wp.Tape`` closure
 (:801-827), this solver's steps are ``torch.autograd`` nodes whose backward
 runs the native adjoint (``autograd.StepFunction``); gradients reach
-``state_in.particle_q/qd/f``, ``model.tet_materials[:, :2]`` and
-``dirichlet_positions``.
+``state_in.particle_q/qd/f``, ``model.tet_materials[:, :2]``,
+``dirichlet_positions`` and the fibre activations (``control.tet_activations``
+or ``model.tet_activations``, whichever the step read).
 
 Additions, not in the reference: ``independent_bodies`` solves every
 builder body as its own system with its own factor, convergence gate and
@@ -86,12 +87,14 @@ class SolverDiffPhD:
             raise ValueError("hex_mesh and hex_materials must be supplied together")
         if hex_mesh is not None and model.tet_count:
             raise ValueError("hex_mesh cannot be combined with model tetrahedra")
-        if frictional_boundary is not None or mesh_contact is not None or planar_energies is not None:
-            raise NotImplementedError("frictional, mesh and planar contact are not part of this solver path yet")
+        if hex_mesh is None and (muscle_targets is not None or planar_energies is not None):
+            raise ValueError("Explicit muscle_targets and planar_energies currently require hex_mesh")
+        if hex_mesh is not None and any(x is not None for x in (frictional_boundary, mesh_contact)):
+            raise NotImplementedError("Hex8 currently supports planar_energies, not frictional or mesh contact")
+        if frictional_boundary is not None or mesh_contact is not None:
+            raise NotImplementedError("frictional and mesh contact are not part of this solver path yet")
         if frictional_vertices is not None or friction or restitution:
             raise ValueError("frictional_vertices, friction and restitution need a frictional_boundary")
-        if muscle_targets is not None:
-            raise NotImplementedError("Hex8 muscle_targets are not supported; Tet4 fibres use tet_activations")
         density = np.broadcast_to(np.asarray(density, dtype=np.float64), (1,) if np.ndim(density) == 0 else np.shape(density))
         if not np.all(np.isfinite(density)) or np.any(density <= 0):
             raise ValueError("density must be finite and positive")
@@ -161,6 +164,49 @@ class SolverDiffPhD:
                 raise ValueError("Muscle directions and stiffness must be finite, with nonnegative stiffness")
             self._has_muscles = bool(np.any(k > 0))
             self._muscles = (np.ascontiguousarray(d), np.ascontiguousarray(k))
+        # Hex8 fibre target lengths (DP/solver.py:245-260): fixed per element,
+        # ones by default; tet_activations cannot control Hex8 fibres.
+        self.muscle_targets = None
+        if self._hex:
+            if muscle_targets is not None and not self._has_muscles:
+                raise ValueError("muscle_targets requires enabled muscle_directions and muscle_stiffness")
+            targets = np.ones(self._element_count) if muscle_targets is None else muscle_targets
+            if isinstance(targets, torch.Tensor):
+                if targets.dtype != torch.float64 or targets.device != self.device:
+                    raise ValueError("muscle_targets tensor must be float64 on the model device")
+                self.muscle_targets = targets
+            else:
+                self.muscle_targets = torch.as_tensor(np.asarray(targets, dtype=np.float64), device=self.device)
+            if tuple(self.muscle_targets.shape) != (self._element_count,):
+                raise ValueError("muscle_targets must have shape (element_count,)")
+            values = self.muscle_targets.detach().cpu().numpy()
+            if not np.isfinite(values).all() or np.any(values < 0):
+                raise ValueError("muscle_targets must be finite nonnegative native target lengths")
+        # Planar vertex energies (DP/hex.py:204-243), validated like the reference.
+        self._planar = None
+        if planar_energies is not None:
+            p_idx, p_stiff, plane = planar_energies
+            p_idx = np.asarray(p_idx)
+            if (
+                p_idx.ndim != 1
+                or not np.issubdtype(p_idx.dtype, np.integer)
+                or np.any(p_idx < 0)
+                or np.any(p_idx >= self._n)
+                or len(np.unique(p_idx)) != len(p_idx)
+            ):
+                raise ValueError("planar_energies needs distinct valid particle indices")
+            p_stiff = np.broadcast_to(np.asarray(p_stiff, dtype=np.float64), p_idx.shape)
+            plane = np.asarray(plane, dtype=np.float64)
+            normal_length = np.linalg.norm(plane[:3]) if plane.shape == (4,) else 0.0
+            if plane.shape != (4,) or not np.isfinite(plane).all() or not np.isfinite(normal_length) or normal_length <= 1.0e-5:
+                raise ValueError("planar_energies needs a finite (nx,ny,nz,offset) with nonzero normal")
+            if not np.isfinite(p_stiff).all() or np.any(p_stiff < 0):
+                raise ValueError("planar_energies stiffness must be finite and nonnegative")
+            self._planar = (
+                np.ascontiguousarray(p_idx, dtype=np.int32),
+                np.ascontiguousarray(p_stiff, dtype=np.float64),
+                np.ascontiguousarray(plane),
+            )
         cfg = _lib.Config(
             element_kind=1 if self._hex else 0,
             material=MATERIALS[material],
@@ -266,6 +312,9 @@ class SolverDiffPhD:
         check(lib().dphd_set_materials(self._h, ptr(materials)), "materials")
         if self._muscles is not None:
             check(lib().dphd_set_muscles(self._h, ptr(self._muscles[0]), ptr(self._muscles[1])), "muscles")
+        if self._planar is not None:
+            idx, stiff, plane = self._planar
+            check(lib().dphd_set_planar(self._h, len(idx), ptr(idx), ptr(stiff), ptr(plane)), "planar")
         check(lib().dphd_build(self._h, float(dt)), "build")
         info = _lib.Info()
         check(lib().dphd_get_info(self._h, ctypes.byref(info)))
@@ -279,6 +328,15 @@ class SolverDiffPhD:
     def _muscle_targets(self, control):
         if not self._has_muscles:
             return None
+        if self._hex:
+            if control is not None and control.tet_activations is not None and control.tet_activations.numel():
+                raise ValueError("Hex8 uses solver.muscle_targets; tet_activations cannot control Hex8 fibres")
+            t = self.muscle_targets
+            if t.dtype != torch.float64 or tuple(t.shape) != (self._element_count,):
+                raise ValueError("muscle_targets must have shape (element_count,)")
+            if bool((~torch.isfinite(t) | (t < 0)).any()):
+                self._status_word |= STATUS_ACTIVATION
+            return t.detach().contiguous(), None
         source = self.model.tet_activations
         if control is not None and control.tet_activations is not None:
             source = control.tet_activations
@@ -322,19 +380,23 @@ class SolverDiffPhD:
             self._build(dt)
         targets = self._muscle_targets(control)
         target_values = None if targets is None else targets[0]
+        activations = None if targets is None else targets[1]
         f = state_in.particle_f if state_in.particle_f is not None else torch.zeros_like(state_in.particle_q)
         materials = self.hex_materials if self._hex else self.model.tet_materials
         tracked = torch.is_grad_enabled() and any(
             t is not None and t.requires_grad
-            for t in (state_in.particle_q, state_in.particle_qd, f, materials, dirichlet_positions)
+            for t in (state_in.particle_q, state_in.particle_qd, f, materials, dirichlet_positions, activations)
         )
         if tracked:
             if self._hex and materials.requires_grad:
                 raise NotImplementedError("Hex8 material gradients need Tet4 materials on the model")
+            if self._planar is not None:
+                raise NotImplementedError("planar_energies are forward only: the reference records no Hex8 adjoint")
             from .autograd import StepFunction
 
             q_out, qd_out = StepFunction.apply(
-                self, dt, state_in.particle_q, state_in.particle_qd, f, materials, dirichlet_positions, target_values
+                self, dt, state_in.particle_q, state_in.particle_qd, f, materials, dirichlet_positions, target_values,
+                activations,
             )
             state_out.particle_q = q_out
             state_out.particle_qd = qd_out
@@ -379,11 +441,11 @@ class SolverDiffPhD:
         if self.iteration_log is not None:
             self.iteration_log.append(("forward", self.iterations.numpy()))
 
-    def _native_backward(self, checkpoint, targets, gq_out, gqd_out, gq, gqd, gf, gmat, gdir):
+    def _native_backward(self, checkpoint, targets, gq_out, gqd_out, gq, gqd, gf, gmat, gdir, gact=None):
         check(
             lib().dphd_backward(
                 self._h, ptr(checkpoint), ptr(targets), ptr(gq_out), ptr(gqd_out), ptr(gq), ptr(gqd), ptr(gf),
-                ptr(gmat), ptr(gdir), _stream(),
+                ptr(gmat), ptr(gdir), ptr(gact), _stream(),
             ),
             "backward",
         )
@@ -418,6 +480,8 @@ class SolverDiffPhD:
         if bits & STATUS_SINGULAR:
             raise ValueError("DiffPhD volume projection met a singular deformation gradient (reference 'Singular F.')")
         if bits & STATUS_ACTIVATION:
+            if self._hex:
+                raise ValueError("Hex8 muscle target lengths must be finite and nonnegative")
             raise ValueError("Muscle activations must be finite and <= 1, and zero for disabled fibres")
         if bits & STATUS_NOT_CONVERGED:
             raise ConvergenceError(f"DiffPhD did not converge within {self._max_iterations} iterations")
@@ -425,6 +489,20 @@ class SolverDiffPhD:
             raise ConvergenceError("A DiffPhD backward fixed point did not converge")
         if bits & STATUS_ADJOINT_SAFEGUARD:
             raise ConvergenceError("A reference DiffPhD backward safeguard modified the gradient")
+
+    def deformation_determinants(self, state) -> torch.Tensor:
+        """det(F) at all eight Hex8 Gauss samples as ``(cells, 8)`` float64 (DP/solver.py:360-378)."""
+        if not self._hex:
+            raise NotImplementedError("deformation_determinants currently requires hex_mesh")
+        q = state.particle_q
+        if q.device != self.device or tuple(q.shape) != (self._n, 3) or q.dtype != torch.float32:
+            raise ValueError("state must have this solver's particle count and device")
+        if self._factor_key is None:
+            raise RuntimeError("deformation_determinants needs a built solver; step or prepare first")
+        q = q.detach().contiguous()
+        out = torch.empty((self._element_count, 8), dtype=torch.float64, device=self.device)
+        check(lib().dphd_deformation_determinants(self._h, ptr(q), ptr(out), _stream()), "deformation_determinants")
+        return out
 
     def apply_inverse(self, rhs: torch.Tensor) -> torch.Tensor:
         """A^{-1} rhs for every body through the two device SpMVs (FP64 factor only)."""

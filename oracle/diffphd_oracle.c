@@ -11,11 +11,13 @@
  *   ldlt_small ............... projections.py:178-228, anderson.py:97-147
  *   nh_stretches ............. projections.py:263-305
  *   muscle ................... projections.py:308-316
+ *   planar ................... hex.py:246-260
  *   dsvd / *_jvp ............. projections.py:323-433
  *   forward step ............. solver.py:689-786 with kernels.py:155-414,
  *                              hex.py:122-187, sparse.py:194-206 (CPU SpMV),
  *                              anderson.py:38-317
  *   backward ................. adjoint.py:70-321, 474-659, 1088-1259
+ *                              (activation gradients :595-620)
  * Sums run in the reference's CPU order: sequential CSR rows, incidence
  * entries in stored order, 16-node chunk partials reduced in chunk order.
  * Compiled with -ffp-contract=off.
@@ -512,6 +514,21 @@ static void gather(const oracle_problem* p, const double* contrib, int accumulat
   }
 }
 
+/* planar_projection (hex.py:246-260), after the element gather. */
+static void planar(const oracle_problem* p, const double* x, const double* bnd, double* proj) {
+  const double* nrm = p->plane;
+  for (int j = 0; j < p->planar_count; ++j) {
+    const int i = p->planar_idx[j];
+    const double* xi = x + 3 * (size_t)i;
+    const double d = v_dot(nrm, xi) + nrm[3];
+    const double active = d < 0.0 ? d : 0.0;
+    for (int c = 0; c < 3; ++c) {
+      const double projected = xi[c] - active * nrm[c];
+      proj[3 * (size_t)i + c] = proj[3 * (size_t)i + c] + p->planar_w[j] * (projected - bnd[3 * (size_t)i + c]);
+    }
+  }
+}
+
 static void spmv(int n, const int* off, const int* col, const double* val, const double* in, double* out) {
 #pragma omp parallel for schedule(dynamic, 64)
   for (int r = 0; r < n; ++r) {
@@ -721,6 +738,7 @@ int oracle_forward(const oracle_problem* p, double dt, const float* q, const flo
     memcpy(pprev, proj, vb);
     st |= local_step(p, x, bnd, targets, contrib);
     gather(p, contrib, 0, proj);
+    planar(p, x, bnd, proj);
     for (int t = 0; t < 3 * n; ++t) rhs[t] = inertia * tgt[t] + proj[t];
     spmv(n, p->s_off, p->s_col, p->s_val, rhs, work);
     spmv(n, p->t_off, p->t_col, p->t_val, work, x);
@@ -815,7 +833,9 @@ static int safe_clip(const oracle_problem* p, double* v, int chunks, double* par
 
 int oracle_backward(const oracle_problem* p, double dt, const double* x, const double* targets, const float* gq_out,
                     const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
-                    float* gdirichlet, double* mu_out, int* iterations, int* status) {
+                    float* gdirichlet, double* gactivations, double* mu_out, int* iterations, int* status) {
+  /* The reference records no adjoint with planar energies (Hex8 only). */
+  if (p->planar_count > 0) return 1;
   set_threads(p);
   const int n = p->n, K = p->corners, Q = p->samples, E = p->elements;
   const size_t vb = sizeof(double) * 3 * (size_t)n;
@@ -1011,11 +1031,13 @@ int oracle_backward(const oracle_problem* p, double dt, const double* x, const d
       }
     }
   }
-  /* Material gradients (adjoint.py:556-591, 622-652). */
-  if (gmaterials) {
+  /* Material and activation gradients (adjoint.py:556-620) and their
+   * aggregate guard (adjoint.py:622-659). */
+  if (gmaterials || gactivations) {
     double* g = calloc((size_t)E * 2, sizeof(double));
+    double* ga = calloc((size_t)E, sizeof(double));
     const double svol = p->sample_volume;
-    for (int e = 0; e < E; ++e) {
+    for (int e = 0; gmaterials && e < E; ++e) {
       double first_total = 0.0, second_total = 0.0;
       for (int s = 0; s < Q; ++s) {
         double f[9], first[9], second[9] = {0};
@@ -1058,18 +1080,43 @@ int oracle_backward(const oracle_problem* p, double dt, const double* x, const d
       g[2 * e] = 2.0 * svol * first_total;
       g[2 * e + 1] = p->material == 0 ? svol * second_total : 0.0;
     }
-    int ok = !diverged;
+    /* d/du of the fibre energy with target length a = 1 - u (Tet4 only):
+     * -sum_k z_k . (w_m / |F m|) (m . g_k) F m, zero for a collapsed fibre. */
+    for (int e = 0; gactivations && p->mw && Q == 1 && e < E; ++e) {
+      const double wm = p->mw[e];
+      if (wm == 0.0) continue;
+      const double* m = p->mdir + 3 * (size_t)e;
+      double f[9], fibre[3];
+      deformation(p, x, e, 0, f);
+      m_vec(f, m, fibre);
+      const double length = sqrt(v_dot(fibre, fibre));
+      if (length <= EPS) continue;
+      const double* gr = sample_grads(p, e, 0);
+      double value = 0.0;
+      for (int k = 0; k < K; ++k) {
+        const double* zk = sol + 3 * (size_t)p->indices[(size_t)e * K + k];
+        const double c = wm / length * v_dot(m, gr + 3 * k);
+        const double t[3] = {c * fibre[0], c * fibre[1], c * fibre[2]};
+        value += v_dot(zk, t);
+      }
+      ga[e] = -value;
+    }
     double s0 = 0.0, s1 = 0.0;
+    int activation_ok = 1;
     for (int e = 0; e < E; ++e) {
       s0 += g[2 * e];
       s1 += g[2 * e + 1];
+      activation_ok = activation_ok && isfinite(ga[e]);
     }
-    const int zero_energy = p->material == 0 && p->nh_lame == 0.0;
+    const int zero_energy = gmaterials && p->material == 0 && p->nh_lame == 0.0;
     const int material_ok = !zero_energy && isfinite(s0) && isfinite(s1);
-    if (!material_ok) st |= 16;
-    if (ok && material_ok)
+    if (!material_ok || !activation_ok) st |= 16;
+    if (gmaterials && !diverged && material_ok)
       for (int t = 0; t < 2 * E; ++t) gmaterials[t] += g[t];
+    if (gactivations && !diverged && activation_ok)
+      for (int e = 0; e < E; ++e) gactivations[e] += ga[e];
     free(g);
+    free(ga);
   }
   if (mu_out) memcpy(mu_out, sol, vb);
   *iterations = iteration;

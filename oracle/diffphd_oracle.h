@@ -47,6 +47,12 @@ typedef struct {
   double abs_tol, rel_tol;
   int integrate_pinned;
   int threads;            /* OpenMP threads, <= 0 = default */
+  /* Planar vertex energies (hex.py:204-260): k_j (x - min(0, n.x + d) n - x_D)
+   * added to the projection of vertex idx[j]; plane = (n, d) with |n| = 1. */
+  int planar_count;
+  const int* planar_idx;
+  const double* planar_w;
+  double plane[4];
 } oracle_problem;
 
 int oracle_forward(const oracle_problem* p, double dt, const float* q, const float* qd, const float* f,
@@ -55,7 +61,7 @@ int oracle_forward(const oracle_problem* p, double dt, const float* q, const flo
 
 int oracle_backward(const oracle_problem* p, double dt, const double* x, const double* targets, const float* gq_out,
                     const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
-                    float* gdirichlet, double* mu_out, int* iterations, int* status);
+                    float* gdirichlet, double* gactivations, double* mu_out, int* iterations, int* status);
 
 void oracle_projections(int count, const double* f, const double* dirs, double mu, double lame, double penalty,
                         double fibre_target, double* sigma, double* recon, double* rotation, double* volume,

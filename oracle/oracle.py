@@ -75,6 +75,10 @@ class _Problem(ctypes.Structure):
         ("rel_tol", ctypes.c_double),
         ("integrate_pinned", ctypes.c_int),
         ("threads", ctypes.c_int),
+        ("planar_count", ctypes.c_int),
+        ("planar_idx", ctypes.c_void_p),
+        ("planar_w", ctypes.c_void_p),
+        ("plane", ctypes.c_double * 4),
     ]
 
 
@@ -122,6 +126,7 @@ class OracleBody:
         jacobian_mode="reference",
         muscle_directions=None,
         muscle_stiffness=None,
+        planar_energies=None,
         factor=None,
         threads=0,
     ):
@@ -167,9 +172,19 @@ class OracleBody:
             self.mdir, self.mw = d, k * svol
         self.material = 0 if material == "pd" else 1
         self.inertia = self.mass / dt**2
+        # PlanarEnergies (hex.py:204-243): distinct vertices, one plane with its
+        # normal scaled to unit length (the offset with it).
+        self.planar = None
+        if planar_energies is not None:
+            idx, stiffness, plane = planar_energies
+            idx = np.ascontiguousarray(idx, dtype=np.int32)
+            stiffness = np.ascontiguousarray(np.broadcast_to(np.asarray(stiffness, np.float64), idx.shape))
+            plane = np.asarray(plane, dtype=np.float64)
+            self.planar = (idx, stiffness, plane / np.linalg.norm(plane[:3]))
         weights = self.wc + self.wv + self.wn
         self.rows = setup_ref.assemble_rows(
-            self.n, self.indices, self.grads, shared, weights, self.free.astype(bool), self.inertia, self.mdir, self.mw
+            self.n, self.indices, self.grads, shared, weights, self.free.astype(bool), self.inertia, self.mdir, self.mw,
+            None if self.planar is None else self.planar[:2],
         )
         if factor is None:
             f = sparse_ref.sparse_inverse(self.rows)
@@ -235,6 +250,11 @@ class OracleBody:
         p.window, p.max_iterations = o["window"], o["max_iterations"]
         p.abs_tol, p.rel_tol = o["abs_tol"], o["rel_tol"]
         p.integrate_pinned, p.threads = o["integrate"], o["threads"]
+        if self.planar is not None:
+            p.planar_count = len(self.planar[0])
+            p.planar_idx, p.planar_w = _ptr(self.planar[0]), _ptr(self.planar[1])
+            for d in range(4):
+                p.plane[d] = self.planar[2][d]
         return p
 
     def forward(self, dt, q, qd, f=None, dirichlet=None, targets=None):
@@ -265,17 +285,20 @@ class OracleBody:
         )
         return {"x": x, "q": qo, "qd": qdo, "iterations": it.value, "status": st.value}
 
-    def backward(self, dt, x, gq_out=None, gqd_out=None, materials=True, explicit_dirichlet=False, targets=None):
+    def backward(
+        self, dt, x, gq_out=None, gqd_out=None, materials=True, explicit_dirichlet=False, targets=None, activations=False
+    ):
         n, E = self.n, len(self.indices)
         gq_in = np.zeros((n, 3), np.float32)
         gqd_in = np.zeros((n, 3), np.float32)
         gf_in = np.zeros((n, 3), np.float32)
         gmat = np.zeros((E, 2)) if materials else None
         gdir = np.zeros((n, 3), np.float32) if explicit_dirichlet else None
+        gact = np.zeros(E) if activations else None
         mu = np.zeros((n, 3))
         it, st = ctypes.c_int(), ctypes.c_int()
         p = self._struct()
-        lib().oracle_backward(
+        code = lib().oracle_backward(
             ctypes.byref(p),
             ctypes.c_double(dt),
             _ptr(_c(x, np.float64)),
@@ -287,16 +310,20 @@ class OracleBody:
             _ptr(gf_in),
             _ptr(gmat),
             _ptr(gdir),
+            _ptr(gact),
             _ptr(mu),
             ctypes.byref(it),
             ctypes.byref(st),
         )
+        if code != 0:
+            raise NotImplementedError("the oracle has no adjoint with planar energies")
         return {
             "q": gq_in,
             "qd": gqd_in,
             "f": gf_in,
             "materials": gmat,
             "dirichlet": gdir,
+            "activations": gact,
             "mu": mu,
             "iterations": it.value,
             "status": st.value,

@@ -15,11 +15,14 @@ reference's own tests:
   (``:275-279``), fibres (``:281-307``);
 * ``tet_bwd_*``   ``reference_backward`` (mean-weight K, FD projection
   Jacobians) at the oracle's converged iterate (``tests/test_diff_phd_adjoint.py:138-153``);
+  ``tet_bwd_muscle_pd`` adds fibres and their activation gradients;
 * ``hex_fwd_*``   ``ReferenceStep.step`` with an isoparametric 2x2x2-Gauss
   Hex8 integration written here from ``HexOracle``
   (``tests/test_diff_phd_hex.py:39-109``) on its two-cube fixture (``:25-36``);
 * ``projections`` the oracle projections on the F set of
-  ``tests/test_diff_phd_solver.py:124-161``.
+  ``tests/test_diff_phd_solver.py:124-161``;
+* ``cantilever_geometry`` the Hex8 cells, reference points, root vertices and
+  x-cell indices of ``examples/assets/cantilever.usda``, parsed as text.
 """
 
 import argparse
@@ -36,6 +39,12 @@ sys.path.insert(0, str(ROOT / "newton-solver-diff-phd_amd"))
 from diffphd_amd import meshes  # noqa: E402
 
 DENSITY = 1000.0
+ONLY = None  # file-name prefix filter (--only)
+
+
+def save(path, **arrays):
+    if ONLY is None or path.name.startswith(ONLY):
+        np.savez_compressed(path, **arrays)
 
 
 def load_reference_oracle(reference):
@@ -127,10 +136,32 @@ def make_hex_oracle(ro):
     return HexQuadrature
 
 
+def cantilever_geometry(reference):
+    """Arrays of the reference's baked cantilever asset, read as text (USD ASCII)."""
+    import re
+
+    path = Path(reference) / "src" / "newton_solver_diff_phd" / "examples" / "assets" / "cantilever.usda"
+    text = path.read_text()
+
+    def attribute(name):
+        match = re.search(re.escape(name) + r" = \[(.*?)\]\n", text, re.S)
+        return re.findall(r"[-+0-9.eE]+", match.group(1))
+
+    return {
+        "hexes": np.array(attribute("newton:hexVertexIndices"), dtype=np.int32).reshape(-1, 8),
+        "points": np.array(attribute("newton:referencePoints"), dtype=np.float64).reshape(-1, 3),
+        "root": np.array(attribute("diffphd:rootVertexIndices"), dtype=np.int32),
+        "cells": np.array(attribute("diffphd:xCellIndices"), dtype=np.int32),
+    }
+
+
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--reference", default="/root/reference")
+    parser.add_argument("--only", default=None, help="write only the files whose names start with this")
     args = parser.parse_args()
+    global ONLY
+    ONLY = args.only
     ro = load_reference_oracle(args.reference)
     dt = 0.004
     gravity = np.array([0.0, 0.0, -9.81])
@@ -150,7 +181,7 @@ def main():
             x, iterations, converged = step.step(
                 q.astype(np.float64), v.astype(np.float64), f.astype(np.float64), gravity, dt, **opts
             )
-            np.savez_compressed(
+            save(
                 HERE / f"tet_fwd_{name}_{'pd' if material == 'pd' else 'nh'}.npz",
                 tets=tets, poses=poses, materials=materials, free=free, q=q, qd=v, f=f, gravity=gravity,
                 dt=dt, density=DENSITY, max_iterations=opts["max_iterations"], abs_tol=opts["abs_tol"],
@@ -174,7 +205,7 @@ def main():
     x, iterations, converged = step.step(
         q.astype(np.float64), v.astype(np.float64), f.astype(np.float64), gravity, dt, targets=targets, **opts
     )
-    np.savez_compressed(
+    save(
         HERE / "tet_fwd_muscle_pd.npz", tets=tets, poses=poses, materials=materials, free=free, q=q, qd=v, f=f,
         gravity=gravity, dt=dt, density=DENSITY, directions=directions, stiffness=stiffness, targets=targets,
         x=x, iterations=iterations, converged=converged, **opts,
@@ -196,11 +227,37 @@ def main():
         gq = np.random.default_rng(49).standard_normal(q.shape).astype(np.float32)
         gqd = (0.01 * np.random.default_rng(50).standard_normal(q.shape)).astype(np.float32) if seed_v else np.zeros_like(gq)
         out = ro.reference_backward(step, x, dt, gq.astype(np.float64), gqd.astype(np.float64))
-        np.savez_compressed(
+        save(
             HERE / f"tet_bwd_{name}.npz", tets=tets, poses=poses, materials=materials, free=free, q=q, qd=v, f=f,
             gravity=gravity, dt=dt, density=DENSITY, material=material, x=x, iterations=iterations,
             gq=gq, gqd=gqd, **{f"out_{k}": np.asarray(val) for k, val in out.items()}, **tight,
         )
+
+    # Backward with fibres: activation gradients (d/du of the contraction).
+    vertices, tets, poses, materials, free = tet_scene()
+    q, v = deformed_state(vertices, free, stretch=(1.2, 0.85, 1.05), speed=0.3)
+    rng = np.random.default_rng(5)
+    directions = rng.standard_normal((len(tets), 3))
+    stiffness = np.where(np.arange(len(tets)) % 2 == 0, 4.0e4, 0.0)
+    contraction = np.where(stiffness > 0, 0.25, 0.0).astype(np.float32)
+    targets = 1.0 - contraction.astype(np.float64)
+    f = np.random.default_rng(21).uniform(-1.0, 1.0, q.shape).astype(np.float32)
+    step = ro.ReferenceStep(
+        tets, poses, materials[:, 0].astype(np.float64), materials[:, 1].astype(np.float64), DENSITY, free, "pd",
+        muscle_directions=directions, muscle_stiffness=stiffness,
+    )
+    x, iterations, converged = step.step(
+        q.astype(np.float64), v.astype(np.float64), f.astype(np.float64), gravity, dt, targets=targets, **tight
+    )
+    gq = np.random.default_rng(49).standard_normal(q.shape).astype(np.float32)
+    gqd = (0.01 * np.random.default_rng(50).standard_normal(q.shape)).astype(np.float32)
+    out = ro.reference_backward(step, x, dt, gq.astype(np.float64), gqd.astype(np.float64), targets=targets)
+    save(
+        HERE / "tet_bwd_muscle_pd.npz", tets=tets, poses=poses, materials=materials, free=free, q=q, qd=v, f=f,
+        gravity=gravity, dt=dt, density=DENSITY, material="pd", directions=directions, stiffness=stiffness,
+        targets=targets, x=x, iterations=iterations, gq=gq, gqd=gqd,
+        **{f"out_{k}": np.asarray(val) for k, val in out.items()}, **tight,
+    )
 
     # Hex8: two cubes, four pinned vertices, heterogeneous materials, no Anderson.
     HexQuadrature = make_hex_oracle(ro)
@@ -222,11 +279,14 @@ def main():
             q32.astype(np.float64), v32.astype(np.float64), f32.astype(np.float64), np.zeros(3), 0.003, **opts
         )
         inverse = np.linalg.inv(oracle.matrix(0.003))
-        np.savez_compressed(
+        save(
             HERE / f"hex_fwd_{'pd' if material == 'pd' else 'nh'}.npz", vertices=vertices, cells=cells,
             materials=materials, free=free, q=q32, qd=v32, f=f32, dt=0.003, density=DENSITY, x=x,
             iterations=iterations, converged=converged, inverse=inverse, mass=oracle.mass, **opts,
         )
+
+    # The Hex8 cantilever geometry of the reference example (text of its USD asset).
+    save(HERE / "cantilever_geometry.npz", **cantilever_geometry(args.reference))
 
     # Pointwise projections.
     rng = np.random.default_rng(7)
@@ -237,7 +297,7 @@ def main():
     dirs = rng.standard_normal((len(fs), 3))
     dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
     mu, lame = 3.0e4, 8.0e4
-    np.savez_compressed(
+    save(
         HERE / "projections.npz", f=fs, directions=dirs, mu=mu, lame=lame,
         sigma=np.array([np.linalg.svd(m, compute_uv=False) for m in fs]),
         rotation=np.array([ro.corotated_projection(m) for m in fs]),

@@ -40,3 +40,50 @@ def cube_body(k, cell=0.005, seed=0, log_e=(np.log(1e5), np.log(1e7)), poisson=0
     materials = np.stack([mu, lam, np.zeros_like(mu)], axis=1).astype(np.float32)
     free = vertices[:, 2] > vertices[:, 2].min()
     return vertices.astype(np.float32), tets, poses, materials, free
+
+
+# The Hex8 cantilever of the reference example (examples/_diff_phd_cantilever.py:83-127):
+# 20x5x5 cells of 0.016 m, x = 0 clamped, E = 1e7 in x-cells 7..12 and 1e6
+# elsewhere, nu 0.45, density 1000, 20 fibres along +X (k 1e5, target 0) in the
+# cells j = k = 0, planar energies at vertices 720 and 750 (k 100, plane z = -0.1).
+CANTILEVER = {
+    "axial": {"dt": 0.01, "steps": 25, "rel_tol": 1.0e-4},
+    "twist": {"dt": 0.001, "steps": 100, "rel_tol": 1.0e-6},
+}
+
+
+def cantilever_scene(motion="axial"):
+    vertices, hexes = meshes.hex_grid(20, 5, 5, 0.016)
+    cells = np.arange(len(hexes)) // 25
+    stiff = (cells > 20 // 3) & (cells <= (20 // 3) * 2)
+    mu, lam = meshes.lame(1.0e6 * np.where(stiff, 10.0, 1.0), 0.45)
+    stiffness = np.zeros(len(hexes))
+    stiffness[np.arange(20) * 25] = 1.0e5
+    q0 = vertices.copy() if motion == "axial" else meshes.twisted_positions(vertices, np.pi / 6.0)
+    return {
+        "vertices": vertices,
+        "hexes": hexes,
+        "materials": np.column_stack([mu, lam]),
+        "free": vertices[:, 0] != 0.0,
+        "directions": np.tile([1.0, 0.0, 0.0], (len(hexes), 1)),
+        "stiffness": stiffness,
+        "targets": np.zeros(len(hexes)),
+        "planar": (np.array([720, 750], dtype=np.int32), 100.0, (0.0, 0.0, 1.0, 0.1)),
+        "q0": q0.astype(np.float32),
+        "tip": np.flatnonzero(vertices[:, 0] == vertices[:, 0].max()),
+        **CANTILEVER[motion],
+    }
+
+
+def cantilever_force(scene, step):
+    """The axial load: +X 80 t / 25 N on every tip vertex at step t (examples/_diff_phd_cantilever.py:168-171)."""
+    f = np.zeros_like(scene["q0"])
+    f[scene["tip"], 0] = 80.0 * step / 25
+    return f
+
+
+def hex_determinants(grads, hexes, q):
+    """det(F) at the eight Gauss samples of every cell from float32 positions (DP/hex.py:190-201)."""
+    x = np.asarray(q, dtype=np.float32).astype(np.float64)[hexes]  # [E, 8, 3]
+    f = np.einsum("ejb,sjn->esbn", x, grads)
+    return np.linalg.det(f)

@@ -164,12 +164,19 @@ def test_forward_matches_reference_oracle(cuda, name):
     np.testing.assert_array_equal(s_out.particle_q.cpu().numpy(), x.astype(np.float32))
 
 
-@pytest.mark.parametrize("name", ["tet_bwd_pd_c2", "tet_bwd_pd_qd", "tet_bwd_nh"])
+@pytest.mark.parametrize("name", ["tet_bwd_pd_c2", "tet_bwd_pd_qd", "tet_bwd_nh", "tet_bwd_muscle_pd"])
 def test_backward_matches_reference_oracle(cuda, name):
     g = golden(name)
     print(f"[case] backward golden {name}", flush=True)
     material = str(g["material"])
     model = tet_model(cuda, g["tets"], g["poses"], g["materials"], g["free"], g["q"], g["gravity"], requires_grad=True)
+    kw, control = {}, None
+    if "targets" in g:
+        kw = dict(muscle_directions=g["directions"], muscle_stiffness=g["stiffness"])
+        control = model.control()
+        control.tet_activations = torch.as_tensor(
+            (1.0 - g["targets"]).astype(np.float32), device=cuda
+        ).requires_grad_(True)
     solver = _solver(
         model,
         density=float(g["density"]),
@@ -177,9 +184,13 @@ def test_backward_matches_reference_oracle(cuda, name):
         max_iterations=int(g["max_iterations"]),
         abs_tol=float(g["abs_tol"]),
         rel_tol=float(g["rel_tol"]),
+        **kw,
     )
     s_in, s_out = _states(model, g["q"], g["qd"], g["f"], requires_grad=True)
-    solver.step(s_in, s_out, None, None, float(g["dt"]))
+    solver.step(s_in, s_out, control, None, float(g["dt"]))
+    if control is not None:
+        scale = np.abs(g["x"]).max()
+        np.testing.assert_allclose(s_out.particle_q.detach().cpu().numpy(), g["x"], rtol=0, atol=1e-6 * scale)
     torch.autograd.backward(
         [s_out.particle_q, s_out.particle_qd],
         [torch.as_tensor(g["gq"], device=cuda), torch.as_tensor(g["gqd"], device=cuda)],
@@ -198,6 +209,12 @@ def test_backward_matches_reference_oracle(cuda, name):
         ref_m = g["out_materials"]
         got_m = model.tet_materials.grad.cpu().numpy()[:, :2]
         np.testing.assert_allclose(got_m, ref_m, rtol=0, atol=2e-5 * np.abs(ref_m).max())
+    if control is not None:
+        # d/du of the contraction; the golden's disabled fibres are exactly zero.
+        ref_a = g["out_activations"]
+        got_a = control.tet_activations.grad.cpu().numpy()
+        assert np.all(got_a[g["stiffness"] == 0] == 0)
+        np.testing.assert_allclose(got_a, ref_a, rtol=0, atol=2e-5 * np.abs(ref_a).max())
 
 
 @pytest.mark.parametrize("name", ["hex_fwd_pd", "hex_fwd_nh"])

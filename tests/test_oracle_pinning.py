@@ -46,14 +46,20 @@ def test_forward_matches_golden(name):
     np.testing.assert_allclose(out["x"], g["x"], rtol=0, atol=1e-10 * np.abs(g["x"]).max())
 
 
-@pytest.mark.parametrize("name", ["tet_bwd_pd_c2", "tet_bwd_pd_qd", "tet_bwd_nh"])
+@pytest.mark.parametrize("name", ["tet_bwd_pd_c2", "tet_bwd_pd_qd", "tet_bwd_nh", "tet_bwd_muscle_pd"])
 def test_backward_matches_golden(name):
     g = golden(name)
     material = str(g["material"])
     body = _body(g, material)
-    fw = body.forward(float(g["dt"]), g["q"], g["qd"], g["f"])
+    targets = g.get("targets")
+    fw = body.forward(float(g["dt"]), g["q"], g["qd"], g["f"], targets=targets)
     assert fw["iterations"] == int(g["iterations"])
-    bw = body.backward(float(g["dt"]), g["x"], g["gq"], g["gqd"])
+    bw = body.backward(float(g["dt"]), g["x"], g["gq"], g["gqd"], targets=targets, activations=targets is not None)
+    if targets is not None:
+        # d/du of the contraction u = 1 - target; zero where the fibre is disabled.
+        ref = g["out_activations"]
+        assert np.abs(ref).max() > 0 and np.all(bw["activations"][g["stiffness"] == 0] == 0)
+        np.testing.assert_allclose(bw["activations"], ref, rtol=0, atol=2e-5 * np.abs(ref).max())
     assert bw["status"] & 8 == 0
     free = g["free"]
     # The golden mu comes from finite-difference projection Jacobians (eps 1e-6).
@@ -150,6 +156,33 @@ def test_element_mode_adjoint_matches_central_differences():
         assert abs(bw["materials"][e, col] - fd) <= 1e-4 * abs(fd), (e, col, bw["materials"][e, col], fd)
 
 
+def test_activation_gradients_match_central_differences():
+    """FP64 FD of the oracle's converged step in the fibre targets (u = 1 - target, so dL/du = -dL/dtarget)."""
+    vertices, tets, poses, materials, free = cube_body(2, seed=3, log_e=(np.log(2e5), np.log(2e5)))
+    dt = 1e-3
+    rng = np.random.default_rng(4)
+    directions = rng.standard_normal((len(tets), 3))
+    stiffness = np.where(np.arange(len(tets)) % 3 == 0, 0.0, 3.0e5)
+    q0 = (vertices + 3e-4 * rng.standard_normal(vertices.shape) * free[:, None]).astype(np.float32)
+    w = rng.standard_normal(vertices.shape)
+    body = O.OracleBody("tet4", tets, poses=poses, materials=materials, density=1000.0,
+                        inverse_mass=free.astype(np.float32), gravity=(0, 0, -9.81), dt=dt, rel_tol=1e-14,
+                        abs_tol=1e-18, max_iterations=3000, muscle_directions=directions, muscle_stiffness=stiffness)
+    targets = np.where(stiffness > 0, 0.8, 1.0)
+    fw = body.forward(dt, q0, np.zeros_like(q0), targets=targets)
+    bw = body.backward(dt, fw["x"], w.astype(np.float32), None, materials=False, targets=targets, activations=True)
+    assert bw["status"] == 0 and np.all(bw["activations"][stiffness == 0] == 0)
+    h = 1e-4
+    for e in (1, 5, 10):
+        vals = []
+        for sign in (1, -1):
+            t = targets.copy()
+            t[e] += sign * h
+            vals.append((body.forward(dt, q0, np.zeros_like(q0), targets=t)["x"] * w).sum())
+        fd = -(vals[0] - vals[1]) / (2 * h)
+        assert abs(bw["activations"][e] - fd) <= 1e-4 * abs(fd), (e, bw["activations"][e], fd)
+
+
 def test_library_exports_every_declared_entry_point():
     from diffphd_amd import _lib
 
@@ -159,7 +192,7 @@ def test_library_exports_every_declared_entry_point():
     lib = _lib.lib()
     for name in sorted(declared):
         assert hasattr(lib, name), name
-    assert lib.dphd_version() == 2
+    assert lib.dphd_version() == 3
     assert set(_lib.SIGNATURES) == declared
 
 
