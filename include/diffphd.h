@@ -125,6 +125,23 @@ int dphd_set_muscles(dphd_solver* s, const double* directions, const double* sti
  * (nx, ny, nz, offset), scaled to a unit normal. count = 0 removes them.
  * Forward only: dphd_backward rejects a solver that has them. */
 int dphd_set_planar(dphd_solver* s, int count, const int* vertices, const double* stiffness, const double* plane);
+/* Optional mesh contact state force between two bodies (MeshContact,
+ * DP/mesh_contact.py:180-272; solver.py:321-325, 762-770): contact radius [m],
+ * normal stiffnesses of A's vertices against B's faces (kn_left) and of B's
+ * against A's (kn_right) [N/m], friction bound kf [N s/m], Coulomb mu, and
+ * the surface triangles of A and B [count,3] int32 (host). Evaluated on the
+ * start-of-step state into q~ += h^2/m f; no gradient. All-zero/NULL faces
+ * remove it. */
+int dphd_set_mesh_contact(dphd_solver* s, double contact_radius, double kn_left, double kn_right, double kf,
+                          double mu, int faces_a_count, const int* faces_a, int faces_b_count, const int* faces_b);
+/* Mesh contact force [N,3] float64 of float64 positions/velocities [N,3]
+ * (MeshContact.evaluate, DP/mesh_contact.py:245-272); device pointers. */
+int dphd_mesh_contact_force(dphd_solver* s, const double* position, const double* velocity, double* force,
+                            void* stream);
+/* Vertex-face pairs within the contact radius in the last evaluation
+ * (SolverDiffPhD.mesh_contact_pairs, DP/solver.py:350-353); -1 without mesh
+ * contact. Synchronizes. */
+int dphd_mesh_contact_pairs(dphd_solver* s, int* pairs);
 /* Mutable between steps (DP/solver.py:206-209). */
 int dphd_set_tolerances(dphd_solver* s, int max_iterations, double abs_tol, double rel_tol);
 /* Element weights, assembly and factor per body (DP/solver.py:491-552). */

@@ -1,0 +1,60 @@
+// Contact between bodies on gfx950.
+//
+// Mesh contact (the reference's MeshContactStateForce,
+// src/newton_solver_diff_phd/_src/solvers/diff_phd/mesh_contact.py): a
+// penalty force between the surface triangles of two bodies, evaluated once
+// per step on the start-of-step state and added to the inertial target
+// q~ += h^2/m f; it has no gradient.
+//
+// Layout: the vertices of each face set (sorted, unique) and the faces are
+// device arrays; the pairs within the contact radius are found by a hit
+// bitmask per direction, row-major [vertex][face word] (one 64-bit word per
+// 64 faces, built by wavefront ballots) and its transpose [face][vertex word].
+// A particle's force then sums the forces of its own hit pairs in face order
+// and subtracts the reactions of the hit pairs on the faces it is a corner
+// of in vertex order; hit pairs are recomputed where they are summed, so no
+// per-pair force matrix is stored and no float atomics are used.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dphd {
+
+struct MeshContactSide {
+  int nv = 0;                 // vertices of this side's faces
+  int nf = 0;                 // faces of the other side
+  const int* verts = nullptr; // [nv] particle ids (sorted)
+  const int* faces = nullptr; // [nf*3] the other side's faces
+  double kn = 0.0;            // normal stiffness of this side's vertices
+  uint64_t* rows = nullptr;   // [nv * row_words] hits by vertex
+  uint64_t* cols = nullptr;   // [nf * col_words] hits by face
+  int row_words = 0;          // ceil(nf / 64)
+  int col_words = 0;          // ceil(nv / 64)
+};
+
+struct MeshContactDev {
+  int n = 0;                  // particles
+  double radius = 0.0, kf = 0.0, mu = 0.0;
+  MeshContactSide ab, ba;     // A's vertices against B's faces, and B's against A's
+  const int* slot_a = nullptr;  // [n] row of a particle among A's vertices, -1 if none
+  const int* slot_b = nullptr;
+  // Per particle, the (face, corner) entries of the faces of A (inc_a) and of
+  // B (inc_b) it is a corner of, in face order (CSR).
+  const int* inc_a_off = nullptr;
+  const int* inc_a = nullptr;  // [2 * entries]
+  const int* inc_b_off = nullptr;
+  const int* inc_b = nullptr;
+  double* force = nullptr;    // [n*3]
+  int* pairs = nullptr;       // pairs within the contact radius in the last evaluation
+};
+
+// Evaluate the force of positions/velocities given as float32 (state arrays,
+// pos64/vel64 null) or float64 (pos32/vel32 null) into mc.force; with
+// `target` non-null, also target[i] += dt^2 / mass[vbody[i]] * force[i].
+void launch_mesh_contact(const MeshContactDev& mc, const float* pos32, const float* vel32, const double* pos64,
+                         const double* vel64, double* target, const int* vbody, const double* mass, double dt,
+                         hipStream_t stream);
+
+}  // namespace dphd

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/diffphd.h"
+#include "contact.h"
 #include "error.h"
 #include "kernels.h"
 #include "setup.h"
@@ -89,6 +90,11 @@ struct dphd_solver {
   std::vector<int> planar_idx;    // planar vertex energies (hex.py:204-243)
   std::vector<double> planar_w;
   double plane[4] = {0, 0, 1, 0};  // unit normal, offset
+  // mesh contact state force (mesh_contact.py): radius, kn_left, kn_right, kf, mu
+  bool has_mesh_contact = false;
+  double mc_params[5] = {0, 0, 0, 0, 0};
+  std::vector<int> mc_faces_a, mc_faces_b;
+  dphd::MeshContactDev mc;
   bool have_topology = false, have_particles = false, have_materials = false;
   // build products
   bool built = false;
@@ -125,6 +131,8 @@ namespace dphd {
 namespace {
 
 int chunk_size(const dphd_solver* s) { return s->cfg.iteration_chunk > 0 ? s->cfg.iteration_chunk : 8; }
+
+void build_mesh_contact(dphd_solver* s);
 
 void build_solver(dphd_solver* s, double dt) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -581,6 +589,7 @@ void build_solver(dphd_solver* s, double dt) {
   }
   d.integrate = s->cfg.integrate_pinned_velocities ? 1 : 0;
   s->targets = muscles ? dupload(o, std::vector<double>(E, 1.0)) : nullptr;
+  if (s->has_mesh_contact) build_mesh_contact(s);
   void* host = nullptr;
   HIP_CHECK(hipHostMalloc(&host, sizeof(int), hipHostMallocMapped));
   s->any_host = static_cast<int*>(host);
@@ -610,6 +619,72 @@ void build_solver(dphd_solver* s, double dt) {
   s->info.factor_values = factor_values;
   s->info.factor_slots = slots;
   s->info.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Device arrays of the mesh contact (MeshContact.__init__, mesh_contact.py:180-243).
+void build_mesh_contact(dphd_solver* s) {
+  auto& o = s->owned;
+  const int n = s->N;
+  MeshContactDev& mc = s->mc;
+  mc = MeshContactDev{};
+  mc.n = n;
+  mc.radius = s->mc_params[0];
+  mc.kf = s->mc_params[3];
+  mc.mu = s->mc_params[4];
+  auto unique_vertices = [](const std::vector<int>& faces) {
+    std::vector<int> v(faces);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    return v;
+  };
+  const std::vector<int> va = unique_vertices(s->mc_faces_a), vb = unique_vertices(s->mc_faces_b);
+  std::vector<int> slot_a(n, -1), slot_b(n, -1);
+  for (size_t k = 0; k < va.size(); ++k) slot_a[va[k]] = static_cast<int>(k);
+  for (size_t k = 0; k < vb.size(); ++k) slot_b[vb[k]] = static_cast<int>(k);
+  // Per particle, its (face, corner) entries in face order.
+  auto incidence = [n](const std::vector<int>& faces, std::vector<int>& off, std::vector<int>& ent) {
+    off.assign(n + 1, 0);
+    for (const int v : faces) ++off[v + 1];
+    for (int i = 0; i < n; ++i) off[i + 1] += off[i];
+    ent.assign(2 * faces.size(), 0);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (size_t t = 0; t < faces.size(); ++t) {
+      const int k = fill[faces[t]]++;
+      ent[2 * k] = static_cast<int>(t / 3);
+      ent[2 * k + 1] = static_cast<int>(t % 3);
+    }
+  };
+  std::vector<int> ia_off, ia, ib_off, ib;
+  incidence(s->mc_faces_a, ia_off, ia);
+  incidence(s->mc_faces_b, ib_off, ib);
+  const int fa = static_cast<int>(s->mc_faces_a.size() / 3), fb = static_cast<int>(s->mc_faces_b.size() / 3);
+  const int* dva = dupload(o, va);
+  const int* dvb = dupload(o, vb);
+  const int* dfa = dupload(o, s->mc_faces_a);
+  const int* dfb = dupload(o, s->mc_faces_b);
+  auto side = [&](const int* verts, int nv, const int* faces, int nf, double kn) {
+    MeshContactSide sd;
+    sd.nv = nv;
+    sd.nf = nf;
+    sd.verts = verts;
+    sd.faces = faces;
+    sd.kn = kn;
+    sd.row_words = (nf + 63) / 64;
+    sd.col_words = (nv + 63) / 64;
+    sd.rows = dzeros<uint64_t>(o, static_cast<size_t>(nv) * sd.row_words);
+    sd.cols = dzeros<uint64_t>(o, static_cast<size_t>(nf) * sd.col_words);
+    return sd;
+  };
+  mc.ab = side(dva, static_cast<int>(va.size()), dfb, fb, s->mc_params[1]);
+  mc.ba = side(dvb, static_cast<int>(vb.size()), dfa, fa, s->mc_params[2]);
+  mc.slot_a = dupload(o, slot_a);
+  mc.slot_b = dupload(o, slot_b);
+  mc.inc_a_off = dupload(o, ia_off);
+  mc.inc_a = dupload(o, ia);
+  mc.inc_b_off = dupload(o, ib_off);
+  mc.inc_b = dupload(o, ib);
+  mc.force = dzeros<double>(o, 3 * static_cast<size_t>(n));
+  mc.pairs = dzeros<int>(o, 1);
 }
 
 StepScalars scalars(const dphd_solver* s) {
@@ -866,6 +941,60 @@ int dphd_set_planar(dphd_solver* s, int count, const int* vertices, const double
   })
 }
 
+int dphd_set_mesh_contact(dphd_solver* s, double contact_radius, double kn_left, double kn_right, double kf,
+                          double mu, int faces_a_count, const int* faces_a, int faces_b_count, const int* faces_b) {
+  DPHD_TRY({
+    if (!s || s->N == 0) throw std::invalid_argument("dphd_set_mesh_contact: invalid arguments");
+    if (faces_a_count == 0 && faces_b_count == 0 && !faces_a && !faces_b) {
+      s->has_mesh_contact = false;
+      s->release();
+      return DPHD_OK;
+    }
+    const double params[5] = {contact_radius, kn_left, kn_right, kf, mu};
+    const char* names[5] = {"contact_radius", "kn_left", "kn_right", "kf", "mu"};
+    for (int t = 0; t < 5; ++t)
+      if (!std::isfinite(params[t]) || params[t] < 0.0)
+        throw std::invalid_argument(std::string("mesh_contact ") + names[t] + " must be finite and nonnegative");
+    if (faces_a_count <= 0 || faces_b_count <= 0 || !faces_a || !faces_b)
+      throw std::invalid_argument("mesh_contact needs faces on both bodies");
+    std::vector<int> fa(faces_a, faces_a + 3 * static_cast<size_t>(faces_a_count));
+    std::vector<int> fb(faces_b, faces_b + 3 * static_cast<size_t>(faces_b_count));
+    for (const auto* faces : {&fa, &fb})
+      for (const int v : *faces)
+        if (v < 0 || v >= s->N) throw std::invalid_argument("mesh_contact faces must reference existing particles");
+    s->has_mesh_contact = true;
+    for (int t = 0; t < 5; ++t) s->mc_params[t] = params[t];
+    s->mc_faces_a = std::move(fa);
+    s->mc_faces_b = std::move(fb);
+    s->release();
+  })
+}
+
+int dphd_mesh_contact_force(dphd_solver* s, const double* position, const double* velocity, double* force,
+                            void* stream) {
+  DPHD_TRY({
+    if (!s || !s->built || !s->has_mesh_contact || !position || !velocity || !force)
+      throw std::invalid_argument("dphd_mesh_contact_force: needs a built solver with mesh contact");
+    HIP_CHECK(hipSetDevice(s->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    dphd::launch_mesh_contact(s->mc, nullptr, nullptr, position, velocity, nullptr, nullptr, nullptr, 0.0, st);
+    HIP_CHECK(hipMemcpyAsync(force, s->mc.force, sizeof(double) * 3 * static_cast<size_t>(s->N),
+                             hipMemcpyDeviceToDevice, st));
+    HIP_CHECK(hipGetLastError());
+  })
+}
+
+int dphd_mesh_contact_pairs(dphd_solver* s, int* pairs) {
+  DPHD_TRY({
+    if (!s || !pairs) throw std::invalid_argument("dphd_mesh_contact_pairs: invalid arguments");
+    *pairs = -1;
+    if (!s->built || !s->has_mesh_contact) return DPHD_OK;
+    HIP_CHECK(hipSetDevice(s->device));
+    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipMemcpy(pairs, s->mc.pairs, sizeof(int), hipMemcpyDeviceToHost));
+  })
+}
+
 int dphd_set_tolerances(dphd_solver* s, int max_iterations, double abs_tol, double rel_tol) {
   DPHD_TRY({
     if (!s || max_iterations < 1 || !std::isfinite(abs_tol) || abs_tol < 0 || !std::isfinite(rel_tol) || rel_tol < 0)
@@ -926,6 +1055,10 @@ int dphd_step(dphd_solver* s, const float* q_in, const float* qd_in, const float
     }
     const dphd::StepScalars sc = dphd::scalars(s);
     dphd::launch_begin_step(s->d, sc, q_in, qd_in, f_in, dirichlet, s->stream);
+    // q~ = q + h v + h^2/m (f_ext + ForwardStateForce(q, v)) (solver.py:762-770).
+    if (s->has_mesh_contact)
+      dphd::launch_mesh_contact(s->mc, q_in, qd_in, nullptr, nullptr, s->d.qt, s->d.vbody, s->d.mass, s->dt,
+                                s->stream);
     const int chunks = dphd::run_loop(s, false);
     s->last_fwd_iters = chunks * dphd::chunk_size(s);
     dphd::launch_write_state(s->d, sc, q_out, qd_out, checkpoint, s->stream);

@@ -24,7 +24,7 @@ LIBRARY = PACKAGE / "libdiffphd.so"
 ARCH = os.environ.get("DPHD_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["factor.cpp", "layout.cpp", "capi_setup.cpp"]
-HIP_SOURCES = ["kernels.hip", "solver.hip"]
+HIP_SOURCES = ["kernels.hip", "contact.hip", "solver.hip"]
 
 
 def _hipcc():

@@ -76,6 +76,9 @@ SIGNATURES = {
     "dphd_set_materials": ([_P, _P], _I),
     "dphd_set_muscles": ([_P, _P, _P], _I),
     "dphd_set_planar": ([_P, _I, _P, _P, _P], _I),
+    "dphd_set_mesh_contact": ([_P, _D, _D, _D, _D, _D, _I, _P, _I, _P], _I),
+    "dphd_mesh_contact_force": ([_P, _P, _P, _P, _P], _I),
+    "dphd_mesh_contact_pairs": ([_P, ctypes.POINTER(_I)], _I),
     "dphd_set_tolerances": ([_P, _I, _D, _D], _I),
     "dphd_build": ([_P, _D], _I),
     "dphd_get_info": ([_P, ctypes.POINTER(Info)], _I),

@@ -91,8 +91,10 @@ class SolverDiffPhD:
             raise ValueError("Explicit muscle_targets and planar_energies currently require hex_mesh")
         if hex_mesh is not None and any(x is not None for x in (frictional_boundary, mesh_contact)):
             raise NotImplementedError("Hex8 currently supports planar_energies, not frictional or mesh contact")
-        if frictional_boundary is not None or mesh_contact is not None:
-            raise NotImplementedError("frictional and mesh contact are not part of this solver path yet")
+        if frictional_boundary is not None:
+            raise NotImplementedError("frictional contact is not part of this solver path yet")
+        if mesh_contact is not None and len(mesh_contact) != 7:
+            raise ValueError("mesh_contact is (contact_radius, kn_left, kn_right, kf, mu, faces_a, faces_b)")
         if frictional_vertices is not None or friction or restitution:
             raise ValueError("frictional_vertices, friction and restitution need a frictional_boundary")
         density = np.broadcast_to(np.asarray(density, dtype=np.float64), (1,) if np.ndim(density) == 0 else np.shape(density))
@@ -229,6 +231,20 @@ class SolverDiffPhD:
             indices = np.ascontiguousarray(model.tet_indices.detach().cpu().numpy().reshape(-1, 4), dtype=np.int32)
             poses = np.ascontiguousarray(model.tet_poses.detach().cpu().numpy(), dtype=np.float32)
             check(lib().dphd_set_tet4(self._h, ptr(indices), ptr(poses)), "tet4")
+        # Mesh contact state force (DP/solver.py:321-325), validated by the library.
+        self._mesh_contact = None
+        if mesh_contact is not None:
+            radius, kn_left, kn_right, kf, mu, faces_a, faces_b = mesh_contact
+            fa = np.ascontiguousarray(np.asarray(faces_a).reshape(-1, 3), dtype=np.int32)
+            fb = np.ascontiguousarray(np.asarray(faces_b).reshape(-1, 3), dtype=np.int32)
+            check(
+                lib().dphd_set_mesh_contact(
+                    self._h, float(radius), float(kn_left), float(kn_right), float(kf), float(mu), len(fa), ptr(fa),
+                    len(fb), ptr(fb),
+                ),
+                "mesh_contact",
+            )
+            self._mesh_contact = (fa, fb)
         # The converged iterate of the last step in FP64 (the reference's solver._x).
         self._x = torch.zeros((self._n, 3), dtype=torch.float64, device=self.device)
         self._status_word = 0
@@ -489,6 +505,27 @@ class SolverDiffPhD:
             raise ConvergenceError("A DiffPhD backward fixed point did not converge")
         if bits & STATUS_ADJOINT_SAFEGUARD:
             raise ConvergenceError("A reference DiffPhD backward safeguard modified the gradient")
+
+    @property
+    def mesh_contact_pairs(self):
+        """Vertex-face pairs within the contact radius in the last step, as a (1,) tensor, or None."""
+        if self._mesh_contact is None:
+            return None
+        out = ctypes.c_int()
+        check(lib().dphd_mesh_contact_pairs(self._h, ctypes.byref(out)))
+        return torch.tensor([max(out.value, 0)], dtype=torch.int32)
+
+    def mesh_contact_force(self, position: torch.Tensor, velocity: torch.Tensor) -> torch.Tensor:
+        """The mesh contact force [N] of float64 positions and velocities (MeshContact.evaluate)."""
+        if self._mesh_contact is None:
+            raise ValueError("this solver has no mesh_contact")
+        if self._factor_key is None:
+            raise RuntimeError("mesh_contact_force needs a built solver; step or prepare first")
+        position = position.to(device=self.device, dtype=torch.float64).contiguous()
+        velocity = velocity.to(device=self.device, dtype=torch.float64).contiguous()
+        out = torch.empty_like(position)
+        check(lib().dphd_mesh_contact_force(self._h, ptr(position), ptr(velocity), ptr(out), _stream()), "mesh_contact")
+        return out
 
     def deformation_determinants(self, state) -> torch.Tensor:
         """det(F) at all eight Hex8 Gauss samples as ``(cells, 8)`` float64 (DP/solver.py:360-378)."""

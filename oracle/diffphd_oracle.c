@@ -12,6 +12,7 @@
  *   nh_stretches ............. projections.py:263-305
  *   muscle ................... projections.py:308-316
  *   planar ................... hex.py:246-260
+ *   mesh contact ............. mesh_contact.py:34-177
  *   dsvd / *_jvp ............. projections.py:323-433
  *   forward step ............. solver.py:689-786 with kernels.py:155-414,
  *                              hex.py:122-187, sparse.py:194-206 (CPU SpMV),
@@ -727,6 +728,8 @@ int oracle_forward(const oracle_problem* p, double dt, const float* q, const flo
       proj[3 * i + d] = 0.0;
     }
   }
+  if (p->state_force)
+    for (int t = 0; t < 3 * n; ++t) tgt[t] = tgt[t] + dt * dt / p->mass * p->state_force[t];
   anderson aa;
   aa_init(&aa, n, p->window);
   aa_reset(&aa);
@@ -1175,4 +1178,185 @@ void oracle_jvps(int count, const double* f, const double* df, const double* dir
     nh_jvp(u, s, v, st, D, mu, lame, pen, neo_hookean + 9 * i);
     muscle_jvp(F, dirs + 3 * i, fibre_target, D, fibre + 3 * i);
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mesh contact state force (mesh_contact.py).                              */
+
+/* closest_point_triangle (mesh_contact.py:34-87). */
+static void closest_point(const double* p, const double* a, const double* b, const double* c, double* point,
+                          double* bary, double* dist2) {
+  double ab[3], ac[3], ap[3], bp[3], cp[3], d[3];
+  for (int k = 0; k < 3; ++k) {
+    ab[k] = b[k] - a[k];
+    ac[k] = c[k] - a[k];
+    ap[k] = p[k] - a[k];
+    bp[k] = p[k] - b[k];
+    cp[k] = p[k] - c[k];
+  }
+  const double d1 = v_dot(ab, ap), d2 = v_dot(ac, ap);
+  const double d3 = v_dot(ab, bp), d4 = v_dot(ac, bp);
+  const double d5 = v_dot(ab, cp), d6 = v_dot(ac, cp);
+  double u0 = 0, u1 = 0, u2 = 0;
+  if (d1 <= 0.0 && d2 <= 0.0) {
+    u0 = 1.0;
+    memcpy(point, a, sizeof(double) * 3);
+  } else if (d3 >= 0.0 && d4 <= d3) {
+    u1 = 1.0;
+    memcpy(point, b, sizeof(double) * 3);
+  } else if (d1 * d4 - d3 * d2 <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+    const double w = d1 / (d1 - d3);
+    u0 = 1.0 - w;
+    u1 = w;
+    for (int k = 0; k < 3; ++k) point[k] = a[k] + w * ab[k];
+  } else if (d6 >= 0.0 && d5 <= d6) {
+    u2 = 1.0;
+    memcpy(point, c, sizeof(double) * 3);
+  } else if (d5 * d2 - d1 * d6 <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+    const double w = d2 / (d2 - d6);
+    u0 = 1.0 - w;
+    u2 = w;
+    for (int k = 0; k < 3; ++k) point[k] = a[k] + w * ac[k];
+  } else if (d3 * d6 - d5 * d4 <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+    const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    u1 = 1.0 - w;
+    u2 = w;
+    for (int k = 0; k < 3; ++k) point[k] = b[k] + w * (c[k] - b[k]);
+  } else {
+    const double va = d3 * d6 - d5 * d4, vb = d5 * d2 - d1 * d6, vc = d1 * d4 - d3 * d2;
+    const double denom = 1.0 / (va + vb + vc);
+    const double v = vb * denom, w = vc * denom;
+    u0 = (1.0 - v) - w;
+    u1 = v;
+    u2 = w;
+    for (int k = 0; k < 3; ++k) point[k] = (a[k] + ab[k] * v) + ac[k] * w;
+  }
+  bary[0] = u0;
+  bary[1] = u1;
+  bary[2] = u2;
+  for (int k = 0; k < 3; ++k) d[k] = p[k] - point[k];
+  *dist2 = v_dot(d, d);
+}
+
+/* _pair_forces (mesh_contact.py:90-137); returns 1 for a pair in contact. */
+static int pair_force(const double* q, const double* v, int vid, const int* face, double radius, double kn, double kf,
+                      double mu, double* force, double* bary) {
+  const double *p = q + 3 * (size_t)vid, *a = q + 3 * (size_t)face[0], *b = q + 3 * (size_t)face[1],
+               *c = q + 3 * (size_t)face[2];
+  double closest[3], dist2, n[3], d[3];
+  closest_point(p, a, b, c, closest, bary, &dist2);
+  if (dist2 >= radius * radius) return 0;
+  const double dist = sqrt(dist2);
+  for (int k = 0; k < 3; ++k) d[k] = p[k] - closest[k];
+  if (dist > EPS) {
+    for (int k = 0; k < 3; ++k) n[k] = d[k] / dist;
+  } else {
+    double e1[3], e2[3];
+    for (int k = 0; k < 3; ++k) {
+      e1[k] = b[k] - a[k];
+      e2[k] = c[k] - a[k];
+    }
+    n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    const double nn = sqrt(v_dot(n, n));
+    if (nn <= EPS) return 0;
+    for (int k = 0; k < 3; ++k) n[k] = n[k] / nn;
+  }
+  if (v_dot(n, d) < 0.0)
+    for (int k = 0; k < 3; ++k) n[k] = -n[k];
+  const double fn_mag = kn * (radius - dist);
+  double vrel[3], vt[3];
+  for (int k = 0; k < 3; ++k) {
+    const double vtri = (bary[0] * v[3 * (size_t)face[0] + k] + bary[1] * v[3 * (size_t)face[1] + k]) +
+                        bary[2] * v[3 * (size_t)face[2] + k];
+    vrel[k] = v[3 * (size_t)vid + k] - vtri;
+  }
+  const double vn = v_dot(vrel, n);
+  for (int k = 0; k < 3; ++k) vt[k] = vrel[k] - vn * n[k];
+  const double vt_mag = sqrt(v_dot(vt, vt));
+  double scale = 0.0;
+  if (vt_mag > EPS) {
+    const double bound = mu * fn_mag / (vt_mag + EPS);
+    scale = -(kf < bound ? kf : bound);
+  }
+  for (int k = 0; k < 3; ++k) force[k] = fn_mag * n[k] + scale * vt[k];
+  return 1;
+}
+
+static int unique_sorted(const int* faces, int count, int n, int* slot, int* verts) {
+  for (int i = 0; i < n; ++i) slot[i] = -1;
+  for (int t = 0; t < 3 * count; ++t) slot[faces[t]] = 0;
+  int m = 0;
+  for (int i = 0; i < n; ++i)
+    if (slot[i] == 0) {
+      slot[i] = m;
+      verts[m++] = i;
+    }
+  return m;
+}
+
+int oracle_mesh_contact_force(int n, const double* q, const double* v, int fa, const int* faces_a, int fb,
+                              const int* faces_b, double radius, double kn_left, double kn_right, double kf,
+                              double mu, double* force) {
+  int *slot_a = malloc(sizeof(int) * (size_t)n), *slot_b = malloc(sizeof(int) * (size_t)n);
+  int *verts_a = malloc(sizeof(int) * (size_t)n), *verts_b = malloc(sizeof(int) * (size_t)n);
+  const int na = unique_sorted(faces_a, fa, n, slot_a, verts_a);
+  const int nb = unique_sorted(faces_b, fb, n, slot_b, verts_b);
+  /* Per-pair forces and barycentric weights, zero outside the radius. */
+  double* f_ab = calloc((size_t)na * fb * 6, sizeof(double));
+  double* f_ba = calloc((size_t)nb * fa * 6, sizeof(double));
+  int pairs = 0;
+  for (int s = 0; s < na; ++s)
+    for (int f = 0; f < fb; ++f) {
+      double* e = f_ab + 6 * ((size_t)s * fb + f);
+      if (!pair_force(q, v, verts_a[s], faces_b + 3 * f, radius, kn_left, kf, mu, e, e + 3)) {
+        memset(e, 0, sizeof(double) * 6);
+      } else {
+        ++pairs;
+      }
+    }
+  for (int s = 0; s < nb; ++s)
+    for (int f = 0; f < fa; ++f) {
+      double* e = f_ba + 6 * ((size_t)s * fa + f);
+      if (!pair_force(q, v, verts_b[s], faces_a + 3 * f, radius, kn_right, kf, mu, e, e + 3)) {
+        memset(e, 0, sizeof(double) * 6);
+      } else {
+        ++pairs;
+      }
+    }
+  /* _gather_forces (mesh_contact.py:140-170): own pairs in face order, then
+   * the reactions on the faces the particle is a corner of, in face and
+   * vertex order. */
+  for (int i = 0; i < n; ++i) {
+    double t[3] = {0.0, 0.0, 0.0};
+    if (slot_a[i] >= 0)
+      for (int f = 0; f < fb; ++f)
+        for (int k = 0; k < 3; ++k) t[k] = t[k] + f_ab[6 * ((size_t)slot_a[i] * fb + f) + k];
+    if (slot_b[i] >= 0)
+      for (int f = 0; f < fa; ++f)
+        for (int k = 0; k < 3; ++k) t[k] = t[k] + f_ba[6 * ((size_t)slot_b[i] * fa + f) + k];
+    for (int f = 0; f < fb; ++f)
+      for (int corner = 0; corner < 3; ++corner)
+        if (faces_b[3 * f + corner] == i)
+          for (int s = 0; s < na; ++s) {
+            const double* e = f_ab + 6 * ((size_t)s * fb + f);
+            for (int k = 0; k < 3; ++k) t[k] = t[k] - e[k] * e[3 + corner];
+          }
+    for (int f = 0; f < fa; ++f)
+      for (int corner = 0; corner < 3; ++corner)
+        if (faces_a[3 * f + corner] == i)
+          for (int s = 0; s < nb; ++s) {
+            const double* e = f_ba + 6 * ((size_t)s * fa + f);
+            for (int k = 0; k < 3; ++k) t[k] = t[k] - e[k] * e[3 + corner];
+          }
+    memcpy(force + 3 * (size_t)i, t, sizeof(t));
+  }
+  free(f_ab);
+  free(f_ba);
+  free(slot_a);
+  free(slot_b);
+  free(verts_a);
+  free(verts_b);
+  return pairs;
 }

@@ -53,6 +53,9 @@ typedef struct {
   const int* planar_idx;
   const double* planar_w;
   double plane[4];
+  /* State force [n,3] or NULL, added to the inertial target as
+   * q~ += h^2/m f (mesh_contact.py:173-177). */
+  const double* state_force;
 } oracle_problem;
 
 int oracle_forward(const oracle_problem* p, double dt, const float* q, const float* qd, const float* f,
@@ -71,6 +74,13 @@ void oracle_jvps(int count, const double* f, const double* df, const double* dir
                  double fibre_target, double* rotation, double* volume, double* neo_hookean, double* fibre);
 
 void oracle_apply_inverse(const oracle_problem* p, const double* rhs, double* out);
+
+/* Mesh contact state force of positions q and velocities v [n,3]
+ * (MeshContact.evaluate, mesh_contact.py:90-170) into force [n,3]; returns
+ * the number of vertex-face pairs within the radius. */
+int oracle_mesh_contact_force(int n, const double* q, const double* v, int fa, const int* faces_a, int fb,
+                              const int* faces_b, double radius, double kn_left, double kn_right, double kf,
+                              double mu, double* force);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,7 @@ class _Problem(ctypes.Structure):
         ("planar_idx", ctypes.c_void_p),
         ("planar_w", ctypes.c_void_p),
         ("plane", ctypes.c_double * 4),
+        ("state_force", ctypes.c_void_p),
     ]
 
 
@@ -127,6 +128,7 @@ class OracleBody:
         muscle_directions=None,
         muscle_stiffness=None,
         planar_energies=None,
+        mesh_contact=None,
         factor=None,
         threads=0,
     ):
@@ -181,6 +183,16 @@ class OracleBody:
             stiffness = np.ascontiguousarray(np.broadcast_to(np.asarray(stiffness, np.float64), idx.shape))
             plane = np.asarray(plane, dtype=np.float64)
             self.planar = (idx, stiffness, plane / np.linalg.norm(plane[:3]))
+        # MeshContact (mesh_contact.py:180-243): evaluated on the start state.
+        self.mesh_contact = None
+        if mesh_contact is not None:
+            radius, kn_left, kn_right, kf, mu, faces_a, faces_b = mesh_contact
+            self.mesh_contact = (
+                tuple(float(x) for x in (radius, kn_left, kn_right, kf, mu)),
+                np.ascontiguousarray(faces_a, dtype=np.int32).reshape(-1, 3),
+                np.ascontiguousarray(faces_b, dtype=np.int32).reshape(-1, 3),
+            )
+        self._state_force = None
         weights = self.wc + self.wv + self.wn
         self.rows = setup_ref.assemble_rows(
             self.n, self.indices, self.grads, shared, weights, self.free.astype(bool), self.inertia, self.mdir, self.mw,
@@ -250,6 +262,7 @@ class OracleBody:
         p.window, p.max_iterations = o["window"], o["max_iterations"]
         p.abs_tol, p.rel_tol = o["abs_tol"], o["rel_tol"]
         p.integrate_pinned, p.threads = o["integrate"], o["threads"]
+        p.state_force = _ptr(self._state_force)
         if self.planar is not None:
             p.planar_count = len(self.planar[0])
             p.planar_idx, p.planar_w = _ptr(self.planar[0]), _ptr(self.planar[1])
@@ -261,6 +274,10 @@ class OracleBody:
         n = self.n
         q = _c(q, np.float32)
         qd = _c(qd, np.float32)
+        self._state_force = None
+        if self.mesh_contact is not None:
+            params, faces_a, faces_b = self.mesh_contact
+            self._state_force = mesh_contact_force(q, qd, faces_a, faces_b, *params)[0]
         f = np.zeros((n, 3), np.float32) if f is None else _c(f, np.float32)
         dirichlet = _c(dirichlet, np.float32)
         targets = _c(targets, np.float64)
@@ -358,3 +375,17 @@ def jvps(f, df, directions, mu, lame, target):
         *[_ptr(a) for a in out]
     )
     return dict(zip(("rotation", "volume", "neo_hookean", "fibre"), out))
+
+
+def mesh_contact_force(q, v, faces_a, faces_b, radius, kn_left, kn_right, kf, mu):
+    """Mesh contact state force [n,3] and the pair count (mesh_contact.py:90-170)."""
+    q = _c(np.asarray(q, dtype=np.float64), np.float64)
+    v = _c(np.asarray(v, dtype=np.float64), np.float64)
+    faces_a = np.ascontiguousarray(faces_a, dtype=np.int32).reshape(-1, 3)
+    faces_b = np.ascontiguousarray(faces_b, dtype=np.int32).reshape(-1, 3)
+    out = np.zeros_like(q)
+    pairs = lib().oracle_mesh_contact_force(
+        len(q), _ptr(q), _ptr(v), len(faces_a), _ptr(faces_a), len(faces_b), _ptr(faces_b), ctypes.c_double(radius),
+        ctypes.c_double(kn_left), ctypes.c_double(kn_right), ctypes.c_double(kf), ctypes.c_double(mu), _ptr(out),
+    )
+    return out, pairs

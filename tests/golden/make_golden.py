@@ -21,6 +21,8 @@ reference's own tests:
   (``tests/test_diff_phd_hex.py:39-109``) on its two-cube fixture (``:25-36``);
 * ``projections`` the oracle projections on the F set of
   ``tests/test_diff_phd_solver.py:124-161``;
+* ``mesh_contact``  ``mesh_contact_force`` on perturbed two-body states and a
+  ``ReferenceStep.step`` with that force (``tests/test_diff_phd_mesh_contact.py:56-98``);
 * ``cantilever_geometry`` the Hex8 cells, reference points, root vertices and
   x-cell indices of ``examples/assets/cantilever.usda``, parsed as text.
 """
@@ -284,6 +286,40 @@ def main():
             materials=materials, free=free, q=q32, qd=v32, f=f32, dt=0.003, density=DENSITY, x=x,
             iterations=iterations, converged=converged, inverse=inverse, mass=oracle.mass, **opts,
         )
+
+    # Mesh contact: the state force on perturbed two-body states
+    # (tests/test_diff_phd_mesh_contact.py:56-78) and one step with the force in the target (:80-98).
+    sys.path.insert(0, str(ROOT / "tests"))
+    from scenes import two_bodies
+
+    vertices, tets, poses, materials, free, faces_a, faces_b = two_bodies(-0.004)
+    options = (0.02, 5.0e3, 5.0e4, 50.0, 0.5)
+    rng = np.random.default_rng(7)
+    q = vertices.astype(np.float64)
+    positions, velocities, forces = [], [], []
+    for trial in range(4):
+        pos = q + 0.012 * rng.standard_normal(q.shape) * (trial > 0)
+        vel = rng.standard_normal(q.shape)
+        positions.append(pos)
+        velocities.append(vel)
+        forces.append(ro.mesh_contact_force(pos, vel, faces_a, faces_b, *options))
+    vertices, tets, poses, materials, free, faces_a, faces_b = two_bodies(-0.003)
+    step_options = (0.01, 5.0e3, 5.0e4, 50.0, 0.5)
+    rng = np.random.default_rng(3)
+    v = np.where(free[:, None], 0.2 * rng.standard_normal(vertices.shape), 0.0).astype(np.float32)
+    q = vertices.astype(np.float64)
+    force = ro.mesh_contact_force(q, v.astype(np.float64), faces_a, faces_b, *step_options)
+    step = ro.ReferenceStep(
+        tets, poses, materials[:, 0].astype(np.float64), materials[:, 1].astype(np.float64), DENSITY, free, "pd"
+    )
+    x, iterations, converged = step.step(
+        q, v.astype(np.float64), force, np.zeros(3), 0.004, max_iterations=3000, abs_tol=1.0e-12, rel_tol=1.0e-11
+    )
+    save(
+        HERE / "mesh_contact.npz", options=np.array(options), positions=np.array(positions),
+        velocities=np.array(velocities), forces=np.array(forces), step_options=np.array(step_options), step_qd=v,
+        step_force=force, step_x=x, step_iterations=iterations, step_converged=converged,
+    )
 
     # The Hex8 cantilever geometry of the reference example (text of its USD asset).
     save(HERE / "cantilever_geometry.npz", **cantilever_geometry(args.reference))

@@ -87,3 +87,34 @@ def hex_determinants(grads, hexes, q):
     x = np.asarray(q, dtype=np.float32).astype(np.float64)[hexes]  # [E, 8, 3]
     f = np.einsum("ejb,sjn->esbn", x, grads)
     return np.linalg.det(f)
+
+
+_TET_FACES = np.array(((1, 2, 3), (0, 3, 2), (0, 1, 3), (0, 2, 1)))
+
+
+def boundary_faces(tets):
+    """Faces used by one tetrahedron, outward, in sorted-key order (the reference tests' _boundary)."""
+    faces = np.asarray(tets)[:, _TET_FACES].reshape(-1, 3)
+    _, first, counts = np.unique(np.sort(faces, axis=1), axis=0, return_index=True, return_counts=True)
+    return faces[first[counts == 1]]
+
+
+def two_bodies(gap):
+    """A 3x2x2 grid of 0.1 m cells (bottom pinned) and one tetrahedron whose apex is ``gap`` above its top.
+
+    The scene of tests/test_diff_phd_mesh_contact.py:27-52 with the Kuhn split for the grid.
+    Returns vertices, tets, poses, materials, free, faces_a (grid), faces_b (tetrahedron).
+    """
+    vertices, tets, _ = meshes.kuhn_tet_grid(3, 2, 2, 0.1)
+    grid = len(vertices)
+    extra = np.array([(0.13, 0.11, 0.2 + gap), (0.33, 0.07, 0.5), (0.05, 0.18, 0.5), (0.17, 0.02, 0.55)])
+    vertices = np.concatenate([vertices, extra])
+    tets = np.concatenate([tets, np.arange(grid, grid + 4, dtype=np.int32)[None]]).astype(np.int32)
+    poses = meshes.tet_poses(vertices, tets)
+    materials = np.tile([3.0e4, 8.0e4, 0.0], (len(tets), 1)).astype(np.float32)
+    free = np.ones(len(vertices), bool)
+    free[:grid] = vertices[:grid, 2] > 0.0
+    faces = boundary_faces(tets)
+    faces_a = faces[np.all(faces < grid, axis=1)]
+    faces_b = faces[np.all(faces >= grid, axis=1)]
+    return vertices.astype(np.float32), tets, poses, materials, free, faces_a, faces_b

@@ -1,0 +1,96 @@
+"""Mesh contact state force on the GPU (MeshContactStateForce, DP/mesh_contact.py).
+
+Against the golden vectors of the reference's NumPy transcription and the
+oracle, on the two-body scene of tests/test_diff_phd_mesh_contact.py.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from scenes import golden, tet_model, two_bodies
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = {"max_iterations": 3000, "abs_tol": 1.0e-12, "rel_tol": 1.0e-11}
+
+
+def _solver(model, **kw):
+    from diffphd_amd import SolverDiffPhD
+
+    return SolverDiffPhD(model, density=1000.0, **kw)
+
+
+def _scene(cuda, gap, requires_grad=False):
+    vertices, tets, poses, materials, free, faces_a, faces_b = two_bodies(gap)
+    model = tet_model(cuda, tets, poses, materials, free, vertices, (0.0, 0.0, 0.0), requires_grad=requires_grad)
+    return model, vertices, faces_a, faces_b
+
+
+def test_force_matches_reference_transcription(cuda):
+    print("[case] mesh contact force", flush=True)
+    g = golden("mesh_contact")
+    model, vertices, faces_a, faces_b = _scene(cuda, -0.004)
+    solver = _solver(model, mesh_contact=(*g["options"], faces_a, faces_b))
+    solver.prepare(0.004)
+    for trial, (pos, vel, expected) in enumerate(zip(g["positions"], g["velocities"], g["forces"])):
+        got = solver.mesh_contact_force(torch.as_tensor(pos, device=cuda), torch.as_tensor(vel, device=cuda))
+        got = got.cpu().numpy()
+        scale = max(np.abs(expected).max(), 1e-300)
+        np.testing.assert_allclose(got, expected, rtol=0, atol=1e-12 * scale, err_msg=f"trial {trial}")
+        ref, pairs = O.mesh_contact_force(pos, vel, faces_a, faces_b, *g["options"])
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * scale, err_msg=f"trial {trial}")
+        assert int(solver.mesh_contact_pairs[0]) == pairs
+
+
+def test_step_matches_reference_with_state_force_in_the_target(cuda):
+    print("[case] mesh contact step", flush=True)
+    g = golden("mesh_contact")
+    model, vertices, faces_a, faces_b = _scene(cuda, -0.003)
+    solver = _solver(model, mesh_contact=(*g["step_options"], faces_a, faces_b), **TIGHT)
+    s_in, s_out = model.state(), model.state()
+    s_in.particle_qd = torch.as_tensor(g["step_qd"], device=cuda)
+    solver.step(s_in, s_out, None, None, 0.004)
+    assert solver.read_status() == 0
+    x = solver.solution().cpu().numpy()
+    np.testing.assert_allclose(x, g["step_x"], rtol=0, atol=1e-10 * np.abs(g["step_x"]).max())
+    assert int(solver.mesh_contact_pairs[0]) > 0
+
+
+def test_gradients_hold_the_state_force_fixed(cuda):
+    print("[case] mesh contact gradients", flush=True)
+    g = golden("mesh_contact")
+    weights = torch.as_tensor(np.random.default_rng(5).standard_normal((40, 3)), device=cuda)
+    results = []
+    for frozen in (False, True):
+        model, vertices, faces_a, faces_b = _scene(cuda, -0.003, requires_grad=True)
+        options = (*g["step_options"], faces_a, faces_b)
+        solver = _solver(model, mesh_contact=None if frozen else options, **TIGHT)
+        s_in, s_out = model.state(), model.state()
+        if frozen:
+            q = vertices.astype(np.float64)
+            force, _ = O.mesh_contact_force(q, np.zeros_like(q), faces_a, faces_b, *g["step_options"])
+            s_in.particle_f = torch.as_tensor(force.astype(np.float32), device=cuda)
+        for name in ("particle_q", "particle_qd", "particle_f"):
+            getattr(s_in, name).requires_grad_(True)
+        solver.step(s_in, s_out, None, None, 0.004)
+        loss = (s_out.particle_q.to(torch.float64) * weights).sum()
+        loss.backward()
+        assert solver.read_status() == 0
+        results.append([s_in.particle_q.grad.cpu().numpy(), s_in.particle_qd.grad.cpu().numpy()])
+    for actual, expected in zip(results[0], results[1]):
+        np.testing.assert_allclose(actual, expected, rtol=0, atol=1e-6 * max(np.abs(expected).max(), 1.0))
+
+
+def test_invalid_mesh_contact_is_rejected(cuda):
+    model, vertices, faces_a, faces_b = _scene(cuda, 0.0)
+    n = len(vertices)
+    for options in (
+        (0.01, 5.0e3, 5.0e4, 50.0, 0.5, faces_a),
+        (-0.01, 5.0e3, 5.0e4, 50.0, 0.5, faces_a, faces_b),
+        (0.01, 5.0e3, 5.0e4, 50.0, 0.5, faces_a, np.zeros((0, 3), dtype=np.int32)),
+        (0.01, 5.0e3, 5.0e4, 50.0, 0.5, faces_a, faces_b + n),
+    ):
+        with pytest.raises(ValueError):
+            _solver(model, mesh_contact=options)
