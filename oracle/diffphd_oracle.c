@@ -13,6 +13,7 @@
  *   muscle ................... projections.py:308-316
  *   planar ................... hex.py:246-260
  *   mesh contact ............. mesh_contact.py:34-177
+ *   frictional boundary ...... contact.py (diffphd_contact.c)
  *   dsvd / *_jvp ............. projections.py:323-433
  *   forward step ............. solver.py:689-786 with kernels.py:155-414,
  *                              hex.py:122-187, sparse.py:194-206 (CPU SpMV),
@@ -37,6 +38,8 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+#include "contact_oracle.h"
 
 #define EPS 2.220446049250313e-16
 #define DMIN 2.2250738585072014e-308
@@ -730,6 +733,16 @@ int oracle_forward(const oracle_problem* p, double dt, const float* q, const flo
   }
   if (p->state_force)
     for (int t = 0; t < 3 * n; ++t) tgt[t] = tgt[t] + dt * dt / p->mass * p->state_force[t];
+  contact_work cw;
+  double* vel = NULL;
+  double* solved = NULL;
+  if (p->contact) {
+    contact_prepare(p, &cw);
+    contact_begin(p, &cw, pos, dt);
+    vel = malloc(vb);
+    solved = malloc(vb);
+    for (int t = 0; t < 3 * n; ++t) vel[t] = (double)qd[t];
+  }
   anderson aa;
   aa_init(&aa, n, p->window);
   aa_reset(&aa);
@@ -744,7 +757,13 @@ int oracle_forward(const oracle_problem* p, double dt, const float* q, const flo
     planar(p, x, bnd, proj);
     for (int t = 0; t < 3 * n; ++t) rhs[t] = inertia * tgt[t] + proj[t];
     spmv(n, p->s_off, p->s_col, p->s_val, rhs, work);
-    spmv(n, p->t_off, p->t_col, p->t_val, work, x);
+    if (p->contact) {
+      /* x = A^{-1}(b + J^T lambda) after one Newton step on the multipliers (solver.py:628-630). */
+      spmv(n, p->t_off, p->t_col, p->t_val, work, solved);
+      contact_iterate(p, &cw, x, solved, pos, x);
+    } else {
+      spmv(n, p->t_off, p->t_col, p->t_val, work, x);
+    }
     for (int i = 0; i < n; ++i)
       if (!p->free[i])
         for (int d = 0; d < 3; ++d) x[3 * i + d] = bnd[3 * i + d];
@@ -780,6 +799,12 @@ int oracle_forward(const oracle_problem* p, double dt, const float* q, const flo
       active = 0;
       st |= 1;
     }
+  }
+  if (p->contact) {
+    contact_finish(p, &cw, pos, vel, dt, x);
+    contact_free(&cw);
+    free(vel);
+    free(solved);
   }
   for (int t = 0; t < 3 * n; ++t) {
     if (x_out) x_out[t] = x[t];

@@ -11,6 +11,28 @@
 extern "C" {
 #endif
 
+/* Frictional boundary of the step (contact.py FrictionalContact): planar
+ * (n, offset), spherical (center, radius) or mesh (MeshFrictionalBoundary)
+ * against `count` registered vertices. `multiplier` carries the normal
+ * multipliers between steps (warm start); omega/normal/candidate receive
+ * the last step's values for the adjoint's Woodbury rows. */
+typedef struct {
+  int kind;               /* 0 planar, 1 spherical, 2 mesh */
+  double params[4];
+  int count;
+  const int* nodes;
+  double friction, restitution;
+  double* multiplier;     /* [count] in/out */
+  double* omega;          /* [count] out */
+  double* normal;         /* [count,3] out */
+  int* candidate;         /* [count] out */
+  /* mesh: faces = left faces then right faces; a vertex below `split`
+   * touches the right faces, the others the left ones */
+  double radius;
+  int split, left_count, face_count;
+  const int* faces;
+} oracle_contact;
+
 typedef struct {
   int n;          /* vertices */
   int elements;   /* elements */
@@ -56,6 +78,8 @@ typedef struct {
   /* State force [n,3] or NULL, added to the inertial target as
    * q~ += h^2/m f (mesh_contact.py:173-177). */
   const double* state_force;
+  /* Frictional boundary or NULL. */
+  oracle_contact* contact;
 } oracle_problem;
 
 int oracle_forward(const oracle_problem* p, double dt, const float* q, const float* qd, const float* f,

@@ -26,15 +26,37 @@ LIB = BUILD / "liboracle.so"
 
 def build(force=False):
     """Compile the C oracle with gcc (-ffp-contract=off, OpenMP)."""
-    src = HERE / "diffphd_oracle.c"
-    if LIB.exists() and not force and LIB.stat().st_mtime >= src.stat().st_mtime:
+    sources = [HERE / "diffphd_oracle.c", HERE / "diffphd_contact.c"]
+    newest = max(f.stat().st_mtime for f in [*sources, HERE / "diffphd_oracle.h", HERE / "contact_oracle.h"])
+    if LIB.exists() and not force and LIB.stat().st_mtime >= newest:
         return LIB
     BUILD.mkdir(exist_ok=True)
     tmp = BUILD / f"liboracle.so.tmp{os.getpid()}"
-    cmd = ["gcc", "-O2", "-std=c99", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", str(src), "-o", str(tmp), "-lm"]
+    cmd = ["gcc", "-O2", "-std=c99", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", *map(str, sources), "-o",
+           str(tmp), "-lm"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB
+
+
+class _Contact(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("params", ctypes.c_double * 4),
+        ("count", ctypes.c_int),
+        ("nodes", ctypes.c_void_p),
+        ("friction", ctypes.c_double),
+        ("restitution", ctypes.c_double),
+        ("multiplier", ctypes.c_void_p),
+        ("omega", ctypes.c_void_p),
+        ("normal", ctypes.c_void_p),
+        ("candidate", ctypes.c_void_p),
+        ("radius", ctypes.c_double),
+        ("split", ctypes.c_int),
+        ("left_count", ctypes.c_int),
+        ("face_count", ctypes.c_int),
+        ("faces", ctypes.c_void_p),
+    ]
 
 
 class _Problem(ctypes.Structure):
@@ -80,6 +102,7 @@ class _Problem(ctypes.Structure):
         ("planar_w", ctypes.c_void_p),
         ("plane", ctypes.c_double * 4),
         ("state_force", ctypes.c_void_p),
+        ("contact", ctypes.POINTER(_Contact)),
     ]
 
 
@@ -129,6 +152,10 @@ class OracleBody:
         muscle_stiffness=None,
         planar_energies=None,
         mesh_contact=None,
+        frictional_boundary=None,
+        frictional_vertices=None,
+        friction=0.0,
+        restitution=0.0,
         factor=None,
         threads=0,
     ):
@@ -193,6 +220,29 @@ class OracleBody:
                 np.ascontiguousarray(faces_b, dtype=np.int32).reshape(-1, 3),
             )
         self._state_force = None
+        # FrictionalContact (contact.py:686-780); the multipliers persist between steps.
+        self.contact = None
+        if frictional_boundary is not None:
+            kind, params = frictional_boundary
+            c = {"kind": {"planar": 0, "spherical": 1, "mesh": 2}[kind], "friction": float(friction),
+                 "restitution": float(restitution)}
+            if kind == "mesh":
+                radius, split, left, right = params
+                left = np.asarray(left, dtype=np.int32).reshape(-1, 3)
+                right = np.asarray(right, dtype=np.int32).reshape(-1, 3)
+                c.update(radius=float(radius), split=int(split), left_count=len(left),
+                         faces=np.ascontiguousarray(np.concatenate([left, right])), params=np.zeros(4))
+                if frictional_vertices is None:
+                    frictional_vertices = np.unique(c["faces"])
+            else:
+                c.update(radius=0.0, split=0, left_count=0, faces=None, params=np.asarray(params, np.float64))
+                if frictional_vertices is None:
+                    frictional_vertices = setup_ref.surface_vertices(self.indices)
+            nodes = np.ascontiguousarray(frictional_vertices, dtype=np.int32)
+            k = len(nodes)
+            c.update(nodes=nodes, multiplier=np.zeros(k), omega=np.zeros(k), normal=np.zeros((k, 3)),
+                     candidate=np.zeros(k, np.int32))
+            self.contact = c
         weights = self.wc + self.wv + self.wn
         self.rows = setup_ref.assemble_rows(
             self.n, self.indices, self.grads, shared, weights, self.free.astype(bool), self.inertia, self.mdir, self.mw,
@@ -263,6 +313,20 @@ class OracleBody:
         p.abs_tol, p.rel_tol = o["abs_tol"], o["rel_tol"]
         p.integrate_pinned, p.threads = o["integrate"], o["threads"]
         p.state_force = _ptr(self._state_force)
+        if self.contact is not None:
+            c = self.contact
+            self._contact_struct = _Contact()
+            cs = self._contact_struct
+            cs.kind, cs.count = c["kind"], len(c["nodes"])
+            for d in range(4):
+                cs.params[d] = c["params"][d]
+            cs.nodes, cs.multiplier = _ptr(c["nodes"]), _ptr(c["multiplier"])
+            cs.omega, cs.normal, cs.candidate = _ptr(c["omega"]), _ptr(c["normal"]), _ptr(c["candidate"])
+            cs.friction, cs.restitution = c["friction"], c["restitution"]
+            cs.radius, cs.split, cs.left_count = c["radius"], c["split"], c["left_count"]
+            cs.face_count = 0 if c["faces"] is None else len(c["faces"])
+            cs.faces = _ptr(c["faces"])
+            p.contact = ctypes.pointer(cs)
         if self.planar is not None:
             p.planar_count = len(self.planar[0])
             p.planar_idx, p.planar_w = _ptr(self.planar[0]), _ptr(self.planar[1])

@@ -155,3 +155,11 @@ def rows_to_full_csr(rows):
             val.append(r[j])
         off.append(len(col))
     return np.asarray(off, np.int32), np.asarray(col, np.int32), np.asarray(val, np.float64)
+
+
+def surface_vertices(tets):
+    """Vertices of faces that belong to exactly one tetrahedron, ascending (contact.py:679-684)."""
+    tets = np.asarray(tets)
+    faces = np.concatenate([tets[:, [0, 1, 2]], tets[:, [0, 1, 3]], tets[:, [0, 2, 3]], tets[:, [1, 2, 3]]])
+    unique, counts = np.unique(np.sort(faces, axis=1), axis=0, return_counts=True)
+    return np.unique(unique[counts == 1]).astype(np.int32)

@@ -23,6 +23,10 @@ reference's own tests:
   ``tests/test_diff_phd_solver.py:124-161``;
 * ``mesh_contact``  ``mesh_contact_force`` on perturbed two-body states and a
   ``ReferenceStep.step`` with that force (``tests/test_diff_phd_mesh_contact.py:56-98``);
+* ``contact_*``    rollouts of ``ReferenceStep`` with ``ReferenceContact`` (plane,
+  stick/slide friction, sphere, chatter, restitution) and ``ReferenceMeshContact``
+  (two blocks: frictionless, sticking, restitution), as in
+  ``tests/test_diff_phd_contact.py`` and ``tests/test_diff_phd_mesh_boundary.py``;
 * ``cantilever_geometry`` the Hex8 cells, reference points, root vertices and
   x-cell indices of ``examples/assets/cantilever.usda``, parsed as text.
 """
@@ -39,6 +43,14 @@ ROOT = HERE.parents[1]
 sys.path.insert(0, str(ROOT / "newton-solver-diff-phd_amd"))
 
 from diffphd_amd import meshes  # noqa: E402
+
+
+def setup_surface_vertices(tets):
+    """Vertices of faces of exactly one tetrahedron (the reference's default frictional vertices)."""
+    tets = np.asarray(tets)
+    faces = np.concatenate([tets[:, [0, 1, 2]], tets[:, [0, 1, 3]], tets[:, [0, 2, 3]], tets[:, [1, 2, 3]]])
+    unique, counts = np.unique(np.sort(faces, axis=1), axis=0, return_counts=True)
+    return np.unique(unique[counts == 1]).astype(np.int32)
 
 DENSITY = 1000.0
 ONLY = None  # file-name prefix filter (--only)
@@ -320,6 +332,78 @@ def main():
         velocities=np.array(velocities), forces=np.array(forces), step_options=np.array(step_options), step_qd=v,
         step_force=force, step_x=x, step_iterations=iterations, step_converged=converged,
     )
+
+    # Frictional boundaries: rollouts of ReferenceStep with ReferenceContact /
+    # ReferenceMeshContact (tests/test_diff_phd_contact.py:41-136,
+    # tests/test_diff_phd_mesh_boundary.py:43-133). Each step starts from the
+    # previous step's float32 outputs, as the solver writes them.
+    from scenes import REFERENCE_CONTACT, TIGHT_CONTACT, contact_grid, lifted, two_blocks
+
+    def rollout(step, contact, q, v, steps, options, boundary_nodes):
+        qs, vs, xs, its, convs, mults = [], [], [], [], [], []
+        for _ in range(steps):
+            x, iterations, converged = step.step(
+                q.astype(np.float64), v.astype(np.float64), np.zeros(q.shape), gravity, dt, contact=contact,
+                **options,
+            )
+            qs.append(q)
+            vs.append(v)
+            xs.append(x)
+            its.append(iterations)
+            convs.append(converged)
+            mults.append(contact.registered_multiplier() if hasattr(contact, "registered_multiplier")
+                         else contact.multiplier.copy())
+            q, v = x.astype(np.float32), ((x - q.astype(np.float64)) / dt).astype(np.float32)
+        return dict(q=np.array(qs), qd=np.array(vs), x=np.array(xs), iterations=np.array(its),
+                    converged=np.array(convs), multiplier=np.array(mults), nodes=boundary_nodes)
+
+    vertices, tets, poses, materials, free = contact_grid()
+    nodes = setup_surface_vertices(tets)
+    center = vertices.astype(np.float64).mean(axis=0)
+    sphere = ("spherical", (center[0], center[1], -0.25, 0.25))
+    plane = ("planar", (0.0, 0.0, 1.0, 0.0))
+    cases = {
+        "plane": (plane, 0.0, 0.0, 0.001, (0.0, 0.0, -1.0), 3, TIGHT_CONTACT),
+        "stick": (plane, 2.0, 0.0, 0.001, (0.1, 0.0, -1.0), 3, TIGHT_CONTACT),
+        "slide": (plane, 0.1, 0.0, 0.001, (2.0, 0.0, -1.0), 3, TIGHT_CONTACT),
+        "sphere": (sphere, 0.4, 0.0, 0.0005, (0.0, 0.0, -1.0), 3, REFERENCE_CONTACT),
+        "chatter": (sphere, 0.4, 0.0, 0.0005, (0.0, 0.0, -1.0), 2, TIGHT_CONTACT),
+        "restitution": (plane, 0.0, 0.5, 0.001, (0.0, 0.0, -1.0), 2, TIGHT_CONTACT),
+    }
+    for name, (boundary, friction, restitution, height, velocity, steps, options) in cases.items():
+        step = ro.ReferenceStep(
+            tets, poses, materials[:, 0].astype(np.float64), materials[:, 1].astype(np.float64), DENSITY, free, "pd"
+        )
+        contact = ro.ReferenceContact(*boundary, nodes, friction=friction, restitution=restitution)
+        q, v = lifted(vertices, height, velocity)
+        out = rollout(step, contact, q, v, steps, options, nodes)
+        save(
+            HERE / f"contact_{name}.npz", kind=boundary[0], params=np.array(boundary[1]), friction=friction,
+            restitution=restitution, **{k: options[k] for k in options}, **out,
+        )
+    radius = 0.01
+    for name, gap, velocity, friction, restitution, steps, options in (
+        ("mesh", radius + 0.002, (0.0, 0.0, -1.0), 0.0, 0.0, 6, TIGHT_CONTACT),
+        ("mesh_stick", radius + 0.002, (0.5, 0.0, -1.0), 2.0, 0.0, 6, REFERENCE_CONTACT),
+        ("mesh_restitution", radius + 0.002, (0.0, 0.0, -1.0), 0.0, 0.5, 6, TIGHT_CONTACT),
+    ):
+        vertices_b, tets_b, poses_b, materials_b, free_b, split, left, right = two_blocks(gap)
+        registered = np.unique(np.concatenate([left, right]))
+        registered = registered[free_b[registered] & (registered < split)]
+        step = ro.ReferenceStep(
+            tets_b, poses_b, materials_b[:, 0].astype(np.float64), materials_b[:, 1].astype(np.float64), DENSITY,
+            free_b, "pd",
+        )
+        contact = ro.ReferenceMeshContact(radius, split, left, right, registered, friction, restitution)
+        q = vertices_b.copy()
+        v = np.zeros_like(q)
+        v[:split] = velocity
+        out = rollout(step, contact, q, v, steps, options, registered)
+        save(
+            HERE / f"contact_{name}.npz", kind="mesh", radius=radius, split=split, faces_left=left,
+            faces_right=right, friction=friction, restitution=restitution, **{k: options[k] for k in options},
+            **out,
+        )
 
     # The Hex8 cantilever geometry of the reference example (text of its USD asset).
     save(HERE / "cantilever_geometry.npz", **cantilever_geometry(args.reference))

@@ -118,3 +118,48 @@ def two_bodies(gap):
     faces_a = faces[np.all(faces < grid, axis=1)]
     faces_b = faces[np.all(faces >= grid, axis=1)]
     return vertices.astype(np.float32), tets, poses, materials, free, faces_a, faces_b
+
+
+# Frictional boundary scenes (tests/test_diff_phd_contact.py, tests/test_diff_phd_mesh_boundary.py).
+TIGHT_CONTACT = {"max_iterations": 2000, "abs_tol": 1.0e-12, "rel_tol": 1.0e-11}
+REFERENCE_CONTACT = {"max_iterations": 2000, "abs_tol": 1.0e-9, "rel_tol": 1.0e-6}
+
+
+def contact_grid():
+    """The free 3x2x2 grid of 0.1 m cells at z = 1 (mu 2e4, lambda 5e4), no pinned vertices."""
+    vertices, tets, poses = meshes.kuhn_tet_grid(3, 2, 2, 0.1, origin=(0.0, 0.0, 1.0))
+    materials = np.tile([2.0e4, 5.0e4, 0.0], (len(tets), 1)).astype(np.float32)
+    return vertices.astype(np.float32), tets, poses, materials, np.ones(len(vertices), bool)
+
+
+def lifted(vertices, height, velocity):
+    """Start state with the lowest vertex at ``height`` and a uniform velocity (float32)."""
+    q = np.asarray(vertices, dtype=np.float32).copy()
+    q[:, 2] += np.float32(height) - q[:, 2].min()
+    return q, np.tile(np.asarray(velocity, dtype=np.float32), (len(q), 1))
+
+
+def surface_faces(tets):
+    """Faces used by one tetrahedron, in first-occurrence order (tests/test_diff_phd_mesh_boundary.py:21-24)."""
+    tets = np.asarray(tets)
+    faces = np.concatenate([tets[:, [0, 1, 2]], tets[:, [0, 1, 3]], tets[:, [0, 2, 3]], tets[:, [1, 2, 3]]])
+    _, index, counts = np.unique(np.sort(faces, axis=1), axis=0, return_index=True, return_counts=True)
+    return faces[np.sort(index[counts == 1])]
+
+
+def two_blocks(gap):
+    """A free 2x1x1 block ``gap`` above a 3x2x2 block with a pinned bottom, offset half a cell in x.
+
+    Returns vertices, tets, poses, materials, free, split (first lower vertex),
+    faces_left (upper block) and faces_right (lower block).
+    """
+    upper_v, upper_t, _ = meshes.kuhn_tet_grid(2, 1, 1, 0.1, origin=(0.05, 0.0, 0.2 + gap))
+    lower_v, lower_t, _ = meshes.kuhn_tet_grid(3, 2, 2, 0.1)
+    split = len(upper_v)
+    vertices = np.concatenate([upper_v, lower_v]).astype(np.float32)
+    tets = np.concatenate([upper_t, lower_t + split]).astype(np.int32)
+    poses = meshes.tet_poses(vertices.astype(np.float64), tets)
+    materials = np.tile([2.0e4, 5.0e4, 0.0], (len(tets), 1)).astype(np.float32)
+    free = np.ones(len(vertices), bool)
+    free[split:] = lower_v[:, 2] > 0.0
+    return vertices, tets, poses, materials, free, split, surface_faces(upper_t), surface_faces(lower_t + split)
