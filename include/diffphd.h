@@ -134,6 +134,27 @@ int dphd_set_planar(dphd_solver* s, int count, const int* vertices, const double
  * remove it. */
 int dphd_set_mesh_contact(dphd_solver* s, double contact_radius, double kn_left, double kn_right, double kf,
                           double mu, int faces_a_count, const int* faces_a, int faces_b_count, const int* faces_b);
+/* Optional frictional boundary (FrictionalContact, DP/contact.py:686-1015;
+ * solver.py:302-319, 628-630, 772-774): kind 0 planar (nx, ny, nz, offset),
+ * 1 spherical (cx, cy, cz, radius), 2 mesh (parameters[0] = contact radius;
+ * vertices below `split` touch faces [left_count, face_count), the others
+ * faces [0, left_count) of `faces` [face_count,3]); `count` registered
+ * vertices [count] int32; Coulomb friction and restitution coefficients.
+ * Host arrays. kind < 0 removes it. Needs a single-body solver with the
+ * FP64 factor (DPHD_ENOTIMPL otherwise, at dphd_build). */
+int dphd_set_frictional_boundary(dphd_solver* s, int kind, const double* parameters, int count, const int* vertices,
+                                 double friction, double restitution, int split, int left_count, int face_count,
+                                 const int* faces);
+/* Read (get) and/or replace (set) the four boundary parameters between steps
+ * (SolverDiffPhD.boundary_parameters, DP/solver.py:340-343); host arrays. */
+int dphd_boundary_parameters(dphd_solver* s, const double* set, double* get);
+/* Contact state after the last step (SolverDiffPhD.contact_multipliers,
+ * DP/solver.py:345-348): optional copies of the normal multipliers [K], the
+ * Fischer-Burmeister weights omega [K], the frozen normals [K,3] and the mesh
+ * candidate flags [K]; set_multiplier replaces the multipliers (the warm
+ * start). Host or device pointers; synchronizes. */
+int dphd_contact_state(dphd_solver* s, const double* set_multiplier, double* multiplier, double* omega, double* normal,
+                       int* candidate);
 /* Mesh contact force [N,3] float64 of float64 positions/velocities [N,3]
  * (MeshContact.evaluate, DP/mesh_contact.py:245-272); device pointers. */
 int dphd_mesh_contact_force(dphd_solver* s, const double* position, const double* velocity, double* force,

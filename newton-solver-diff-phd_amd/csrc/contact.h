@@ -58,3 +58,59 @@ void launch_mesh_contact(const MeshContactDev& mc, const float* pos32, const flo
                          hipStream_t stream);
 
 }  // namespace dphd
+
+namespace dphd {
+
+// Frictional boundary (the reference's FrictionalContact,
+// src/newton_solver_diff_phd/_src/solvers/diff_phd/contact.py) of a
+// single-body solver: planar or spherical obstacle, or the mesh boundary
+// between two bodies. K registered vertices; A^{-1} restricted to them and
+// its columns (the panel) are formed once per factor.
+//
+// Layout: panel [K][n] (column j = A^{-1} e_nodes[j], one scalar column per
+// vertex, the same for every coordinate, so a thread over vertices reads it
+// coalesced); blockT [K][K] with blockT[j*K + i] = A^{-1}(nodes[i], nodes[j]);
+// per-candidate scalars and frames [K] / [K*3]; the coupled Newton system
+// [A*A] row-major. The Newton step runs in one workgroup.
+struct FrictionDev {
+  int kind = 0;  // 0 planar, 1 spherical, 2 mesh
+  int K = 0, n = 0;
+  double friction = 0.0, restitution = 0.0;
+  double* params = nullptr;  // [4] (mutable between steps)
+  const int* nodes = nullptr;
+  double* panel = nullptr;
+  double* blockT = nullptr;
+  double* delassus = nullptr;
+  double* reg = nullptr;
+  double *normal = nullptr, *t1 = nullptr, *t2 = nullptr;
+  double* fl = nullptr;  // [K*2] tangential multipliers
+  double* mult = nullptr;  // [K] normal multipliers (warm start across steps)
+  double *gap = nullptr, *omega = nullptr, *diag = nullptr, *resid = nullptr, *rhs = nullptr;
+  double *force = nullptr, *response = nullptr;
+  int *order = nullptr, *slot = nullptr, *nactive = nullptr, *perm = nullptr;
+  double *matrix = nullptr, *compact = nullptr, *work = nullptr;
+  int* candidate = nullptr;
+  // mesh boundary: faces = left then right; vertices below `split` touch the right faces
+  double radius = 0.0;
+  int split = 0, left_count = 0, face_count = 0;
+  const int* faces = nullptr;
+  double* surface = nullptr;  // [n*3] the iterate the boundary last saw (restitution)
+};
+
+// Columns A^{-1} e_j for registered vertices [j0, j0 + 3) from out [n*3]
+// (lane l of out = column j0 + l), then the K x K block once all are stored.
+void launch_contact_units(const FrictionDev& fc, int j0, double* rhs, hipStream_t stream);
+void launch_contact_store(const FrictionDev& fc, int j0, const double* out, hipStream_t stream);
+void launch_contact_block(const FrictionDev& fc, hipStream_t stream);
+// Step start: frames (and the mesh candidate filter), Delassus and scaling.
+void launch_contact_begin(const FrictionDev& fc, const double* position, double dt, hipStream_t stream);
+// One Fischer-Burmeister Newton step and friction update: x is the current
+// iterate, xr holds A^{-1} b on entry and A^{-1}(b + J^T lambda) on exit.
+// Skipped when *active is zero.
+void launch_contact_iteration(const FrictionDev& fc, const double* x, double* xr, const double* position,
+                              const int* active, hipStream_t stream);
+// Post-solve restitution on x (positions/velocities of the step start).
+void launch_contact_finish(const FrictionDev& fc, const double* position, const float* velocity, double dt, double* x,
+                           hipStream_t stream);
+
+}  // namespace dphd

@@ -224,6 +224,500 @@ __global__ __launch_bounds__(256) void k_mc_gather(MeshContactDev mc, Vec3Src po
 
 int blocks_for(long long threads) { return static_cast<int>((threads + 255) / 256); }
 
+// ---------------------------------------------------------------------------
+// Frictional boundary (contact.py).
+
+__device__ __forceinline__ V3 ld3(const double* p, int i) { return V3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ __forceinline__ void st3(double* p, int i, V3 v) {
+  p[3 * i] = v.x;
+  p[3 * i + 1] = v.y;
+  p[3 * i + 2] = v.z;
+}
+
+// boundary_gap / boundary_normal (contact.py:72-95).
+__device__ __forceinline__ double boundary_gap(const FrictionDev& fc, V3 x) {
+#pragma clang fp contract(off)
+  const double* q = fc.params;
+  if (fc.kind == 0) {
+    const V3 axis{q[0], q[1], q[2]};
+    const double norm = sqrt(dot(axis, axis));
+    return dot(x, V3{axis.x / norm, axis.y / norm, axis.z / norm}) + q[3] / norm;
+  }
+  const V3 d = x - V3{q[0], q[1], q[2]};
+  return sqrt(dot(d, d)) - q[3];
+}
+
+__device__ __forceinline__ V3 boundary_normal(const FrictionDev& fc, V3 x) {
+#pragma clang fp contract(off)
+  const double* q = fc.params;
+  V3 v = fc.kind == 0 ? V3{q[0], q[1], q[2]} : x - V3{q[0], q[1], q[2]};
+  const double length = sqrt(dot(v, v));
+  return V3{v.x / length, v.y / length, v.z / length};
+}
+
+// The first face of minimal distance on the surface opposite `node`
+// (mesh_closest, contact.py:98-137): closest point, unit face normal, dist2.
+__device__ void mesh_closest(const FrictionDev& fc, V3 p, int node, const double* x, V3& closest, V3& normal,
+                             double& best) {
+#pragma clang fp contract(off)
+  int first = fc.left_count, last = fc.face_count;
+  if (node >= fc.split) {
+    first = 0;
+    last = fc.left_count;
+  }
+  best = INFINITY;
+  closest = V3{0, 0, 0};
+  normal = V3{0, 0, 1};
+  for (int f = first; f < last; ++f) {
+    const int* face = fc.faces + 3 * f;
+    const V3 a = ld3(x, face[0]), b = ld3(x, face[1]), c = ld3(x, face[2]);
+    V3 point, w;
+    double dist2;
+    closest_point_triangle(p, a, b, c, point, w, dist2);
+    if (dist2 < best) {
+      best = dist2;
+      closest = point;
+      const V3 n = cross3(b - a, c - a);
+      const double length = sqrt(dot(n, n));
+      normal = length > kEps ? V3{n.x / length, n.y / length, n.z / length} : V3{0, 0, 1};
+    }
+  }
+}
+
+__device__ __forceinline__ V3 mesh_direction(V3 p, V3 closest, V3 face_normal) {
+#pragma clang fp contract(off)
+  const V3 offset = p - closest;
+  const double length = sqrt(dot(offset, offset));
+  if (length > 1.0e-12) return V3{offset.x / length, offset.y / length, offset.z / length};
+  return face_normal;
+}
+
+__global__ __launch_bounds__(256) void k_fc_units(FrictionDev fc, int j0, double* rhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= fc.n) return;
+  for (int l = 0; l < 3; ++l) {
+    const int j = j0 + l;
+    rhs[3 * i + l] = (j < fc.K && fc.nodes[j] == i) ? 1.0 : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fc_store(FrictionDev fc, int j0, const double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= fc.n) return;
+  for (int l = 0; l < 3 && j0 + l < fc.K; ++l) fc.panel[static_cast<size_t>(j0 + l) * fc.n + i] = out[3 * i + l];
+}
+
+__global__ __launch_bounds__(256) void k_fc_block(FrictionDev fc) {
+  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= static_cast<long long>(fc.K) * fc.K) return;
+  const int j = static_cast<int>(t / fc.K), i = static_cast<int>(t % fc.K);
+  fc.blockT[t] = fc.panel[static_cast<size_t>(j) * fc.n + fc.nodes[i]];
+  if (i == j) fc.mult[i] = 0.0;  // a new factor restarts the multipliers
+}
+
+// begin_contact_step / begin_mesh_step (contact.py:171-244).
+__global__ __launch_bounds__(256) void k_fc_begin(FrictionDev fc, const double* position) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= fc.K) return;
+  const int node = fc.nodes[c];
+  const V3 p = ld3(position, node);
+  V3 n, t1;
+  if (fc.kind == 2) {
+    V3 closest, face_normal;
+    double dist2;
+    mesh_closest(fc, p, node, position, closest, face_normal, dist2);
+    const bool active = sqrt(dist2) - fc.radius <= 5.0 * fc.radius;
+    n = mesh_direction(p, closest, face_normal);
+    int axis = 0;
+    for (int d = 1; d < 3; ++d)
+      if (fabs(vget(n, d)) < fabs(vget(n, axis))) axis = d;
+    V3 unit{0, 0, 0};
+    vset(unit, axis, 1.0);
+    t1 = unit - dot(n, unit) * n;
+    const double l1 = sqrt(dot(t1, t1));
+    t1 = V3{t1.x / l1, t1.y / l1, t1.z / l1};
+    fc.candidate[c] = active ? 1 : 0;
+    if (!active) fc.mult[c] = 0.0;
+  } else {
+    n = boundary_normal(fc, p);
+    V3 best{0, 0, 0};
+    for (int i = 0; i < 3; ++i) {
+      V3 axis{0, 0, 0};
+      vset(axis, i, 1.0);
+      const V3 cand = cross3(n, axis);
+      if (dot(cand, cand) > dot(best, best)) best = cand;
+    }
+    const double lb = sqrt(dot(best, best));
+    t1 = V3{best.x / lb, best.y / lb, best.z / lb};
+    fc.candidate[c] = 1;
+  }
+  const V3 u = cross3(n, t1);
+  const double l2 = sqrt(dot(u, u));
+  st3(fc.normal, c, n);
+  st3(fc.t1, c, t1);
+  st3(fc.t2, c, V3{u.x / l2, u.y / l2, u.z / l2});
+  fc.fl[2 * c] = fc.fl[2 * c + 1] = 0.0;
+}
+
+// contact_delassus (contact.py:247-263): W_ij = (n_lo . n_hi) B(hi, lo).
+__global__ __launch_bounds__(256) void k_fc_delassus(FrictionDev fc, double dt) {
+#pragma clang fp contract(off)
+  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= static_cast<long long>(fc.K) * fc.K) return;
+  const int i = static_cast<int>(t / fc.K), j = static_cast<int>(t % fc.K);
+  const int lo = i < j ? i : j, hi = i < j ? j : i;
+  const double w = dot(ld3(fc.normal, lo), ld3(fc.normal, hi)) * fc.blockT[static_cast<size_t>(lo) * fc.K + hi];
+  fc.delassus[t] = w;
+  if (i == j) fc.reg[i] = fmax(dt * dt * w, 1.0e-10);
+}
+
+__global__ __launch_bounds__(256) void k_fc_surface(FrictionDev fc, const double* x, const int* active) {
+  if (!*active) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 3 * fc.n) fc.surface[t] = x[t];
+}
+
+__device__ __forceinline__ void complementarity(double delta, double lam, double r, double& omega, double& e,
+                                                double& phi) {
+#pragma clang fp contract(off)
+  double s = sqrt(delta * delta + r * r * lam * lam);
+  if (s < 1.0e-14) s = 1.0e-14;
+  omega = 1.0 - delta / s;
+  e = (1.0 - r * lam / s) * r;
+  phi = delta + r * lam - s;
+}
+
+// contact_scalars / mesh_contact_scalars(_tiled) (contact.py:266-370): one
+// wavefront per candidate; for the mesh boundary its lanes split the faces
+// and the first face of minimal distance wins.
+__global__ __launch_bounds__(64) void k_fc_scalars(FrictionDev fc, const double* x, const int* active) {
+#pragma clang fp contract(off)
+  if (!*active) return;
+  const int c = blockIdx.x, lane = threadIdx.x;
+  const int node = fc.nodes[c];
+  const V3 p = ld3(x, node);
+  double delta;
+  if (fc.kind == 2) {
+    if (!fc.candidate[c]) {
+      if (lane == 0) {
+        fc.gap[c] = 0.0;
+        fc.omega[c] = 0.0;
+        fc.diag[c] = 1.0;
+        fc.resid[c] = 0.0;
+      }
+      return;
+    }
+    int first = fc.left_count, last = fc.face_count;
+    if (node >= fc.split) {
+      first = 0;
+      last = fc.left_count;
+    }
+    double best = INFINITY;
+    int best_face = last;
+    for (int f = first + lane; f < last; f += 64) {
+      const int* face = fc.faces + 3 * f;
+      V3 point, w;
+      double dist2;
+      closest_point_triangle(p, ld3(x, face[0]), ld3(x, face[1]), ld3(x, face[2]), point, w, dist2);
+      if (dist2 < best) {
+        best = dist2;
+        best_face = f;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off, 64);
+      const int of = __shfl_xor(best_face, off, 64);
+      if (ob < best || (ob == best && of < best_face)) {
+        best = ob;
+        best_face = of;
+      }
+    }
+    if (lane != 0) return;
+    V3 closest{0, 0, 0};
+    if (best_face < last) {
+      const int* face = fc.faces + 3 * best_face;
+      V3 w;
+      double dist2;
+      closest_point_triangle(p, ld3(x, face[0]), ld3(x, face[1]), ld3(x, face[2]), closest, w, dist2);
+    }
+    delta = dot(p - closest, ld3(fc.normal, c)) - fc.radius;
+  } else {
+    if (lane != 0) return;
+    delta = boundary_gap(fc, p);
+  }
+  fc.gap[c] = delta;
+  complementarity(delta, fc.mult[c], fc.reg[c], fc.omega[c], fc.diag[c], fc.resid[c]);
+}
+
+__device__ void fc_forces(const FrictionDev& fc, bool tangential) {
+#pragma clang fp contract(off)
+  for (int c = threadIdx.x; c < fc.K; c += blockDim.x) {
+    V3 v = (fc.omega[c] * fc.mult[c]) * ld3(fc.normal, c);
+    if (tangential) v = v + (fc.fl[2 * c] * ld3(fc.t1, c) + fc.fl[2 * c + 1] * ld3(fc.t2, c));
+    st3(fc.force, c, v);
+  }
+}
+
+// contact_response: B f over the coupled candidates, in their order.
+__device__ void fc_response(const FrictionDev& fc, int A) {
+#pragma clang fp contract(off)
+  for (int c = threadIdx.x; c < fc.K; c += blockDim.x) {
+    V3 v{0, 0, 0};
+    for (int k = 0; k < A; ++k) {
+      const int j = fc.order[k];
+      v = v + fc.blockT[static_cast<size_t>(j) * fc.K + c] * ld3(fc.force, j);
+    }
+    st3(fc.response, c, v);
+  }
+}
+
+// Cholesky solve of the coupled block (up to 32 rows), one lane.
+__device__ void cholesky_solve(double* m, int n, double* b) {
+#pragma clang fp contract(off)
+  for (int j = 0; j < n; ++j) {
+    double d = m[j * n + j];
+    for (int k = 0; k < j; ++k) d -= m[j * n + k] * m[j * n + k];
+    d = sqrt(d);
+    m[j * n + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double v = m[i * n + j];
+      for (int k = 0; k < j; ++k) v -= m[i * n + k] * m[j * n + k];
+      m[i * n + j] = v / d;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    double v = b[i];
+    for (int k = 0; k < i; ++k) v -= m[i * n + k] * b[k];
+    b[i] = v / m[i * n + i];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double v = b[i];
+    for (int k = i + 1; k < n; ++k) v -= m[k * n + i] * b[k];
+    b[i] = v / m[i * n + i];
+  }
+}
+
+// Eigen's LDLT with diagonal pivoting on the lower triangle (dense.py:40-106),
+// row updates spread over the workgroup.
+__device__ void ldlt_solve_block(double* m, int n, double* b, int* perm, double* work) {
+#pragma clang fp contract(off)
+  __shared__ int s_big;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = i;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    if (threadIdx.x == 0) {
+      int big = k;
+      for (int i = k + 1; i < n; ++i)
+        if (fabs(m[i * n + i]) > fabs(m[big * n + big])) big = i;
+      s_big = big;
+    }
+    __syncthreads();
+    const int big = s_big;
+    if (big != k) {
+      for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        const double t = m[k * n + j];
+        m[k * n + j] = m[big * n + j];
+        m[big * n + j] = t;
+      }
+      for (int i = big + 1 + threadIdx.x; i < n; i += blockDim.x) {
+        const double t = m[i * n + k];
+        m[i * n + k] = m[i * n + big];
+        m[i * n + big] = t;
+      }
+      for (int i = k + 1 + threadIdx.x; i < big; i += blockDim.x) {
+        const double t = m[i * n + k];
+        m[i * n + k] = m[big * n + i];
+        m[big * n + i] = t;
+      }
+      if (threadIdx.x == 0) {
+        const double t = m[k * n + k];
+        m[k * n + k] = m[big * n + big];
+        m[big * n + big] = t;
+        const int pi = perm[k];
+        perm[k] = perm[big];
+        perm[big] = pi;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int j = 0; j < k; ++j) m[k * n + k] = m[k * n + k] - m[k * n + j] * m[k * n + j] * m[j * n + j];
+    __syncthreads();
+    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) {
+      for (int j = 0; j < k; ++j) m[i * n + k] = m[i * n + k] - m[i * n + j] * m[j * n + j] * m[k * n + j];
+      if (m[k * n + k] != 0.0) m[i * n + k] = m[i * n + k] / m[k * n + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < n; ++i) work[i] = b[perm[i]];
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < i; ++j) work[i] = work[i] - m[i * n + j] * work[j];
+    for (int i = 0; i < n; ++i) work[i] = fabs(m[i * n + i]) > 2.2250738585072014e-308 ? work[i] / m[i * n + i] : 0.0;
+    for (int i = n - 1; i >= 0; --i)
+      for (int j = i + 1; j < n; ++j) work[i] = work[i] - m[j * n + i] * work[j];
+    for (int i = 0; i < n; ++i) b[perm[i]] = work[i];
+  }
+  __syncthreads();
+}
+
+// One Newton step on the multipliers and the friction update
+// (FrictionalContact._newton / iterate, contact.py:865-987), one workgroup.
+__global__ __launch_bounds__(256) void k_fc_newton(FrictionDev fc, const double* x, const double* solved,
+                                                   const double* position, const int* active) {
+#pragma clang fp contract(off)
+  if (!*active) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ int s_wave[4];
+  __shared__ int s_base;
+  // contact_active(_scan): coupled candidates (omega != 0) in candidate order.
+  if (tid == 0) s_base = 0;
+  __syncthreads();
+  for (int start = 0; start < fc.K; start += 256) {
+    const int c = start + tid;
+    const bool flag = c < fc.K && fc.omega[c] != 0.0;
+    const uint64_t mask = __ballot(flag);
+    if (lane == 0) s_wave[wave] = __popcll(mask);
+    __syncthreads();
+    int offset = s_base;
+    for (int w = 0; w < wave; ++w) offset += s_wave[w];
+    const int k = offset + __popcll(mask & ((1ull << lane) - 1ull));
+    if (c < fc.K) {
+      fc.slot[c] = flag ? k : -1;
+      if (flag) fc.order[k] = c;
+    }
+    __syncthreads();
+    if (tid == 0) s_base += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    __syncthreads();
+  }
+  const int A = s_base;
+  if (tid == 0) *fc.nactive = A;
+  fc_forces(fc, true);
+  __syncthreads();
+  fc_response(fc, A);
+  __syncthreads();
+  // contact_newton_rhs: -phi + omega n.x - omega n.(A^{-1} g).
+  for (int c = tid; c < fc.K; c += 256) {
+    const int node = fc.nodes[c];
+    const V3 n = ld3(fc.normal, c);
+    const double w = fc.omega[c];
+    fc.rhs[c] = -fc.resid[c] + w * dot(n, ld3(x, node)) - w * dot(n, ld3(solved, node) + ld3(fc.response, c));
+  }
+  __syncthreads();
+  // contact_system: Omega W Omega + E over the coupled block.
+  for (long long t = tid; t < static_cast<long long>(A) * A; t += 256) {
+    const int i = static_cast<int>(t / A), j = static_cast<int>(t % A);
+    const int a = fc.order[i], b = fc.order[j];
+    double v = fc.omega[a] * fc.delassus[static_cast<size_t>(a) * fc.K + b] * fc.omega[b];
+    if (i == j) v += fc.diag[a];
+    fc.matrix[t] = v;
+    if (j == 0) fc.compact[i] = fc.rhs[a];
+  }
+  __syncthreads();
+  if (A > 32) {
+    ldlt_solve_block(fc.matrix, A, fc.compact, fc.perm, fc.work);
+  } else if (A > 0) {
+    if (tid == 0) cholesky_solve(fc.matrix, A, fc.compact);
+    __syncthreads();
+  }
+  // contact_update
+  for (int c = tid; c < fc.K; c += 256) {
+    const int k = fc.slot[c];
+    const double step = k >= 0 ? fc.compact[k] : fc.rhs[c] / fc.diag[c];
+    fc.mult[c] = fmin(fmax(fc.mult[c] + step, 0.0), 1.0e8);
+  }
+  __syncthreads();
+  if (fc.friction > 0.0) {
+    fc_forces(fc, false);
+    __syncthreads();
+    fc_response(fc, A);
+    __syncthreads();
+    // contact_friction (contact.py:546-584)
+    for (int c = tid; c < fc.K; c += 256) {
+      if (fc.mult[c] <= 0.0) {
+        fc.fl[2 * c] = fc.fl[2 * c + 1] = 0.0;
+        continue;
+      }
+      const int node = fc.nodes[c];
+      const V3 step = ld3(solved, node) + ld3(fc.response, c) - ld3(position, node);
+      const V3 n = ld3(fc.normal, c), t1 = ld3(fc.t1, c), t2 = ld3(fc.t2, c);
+      const V3 tangential = step - dot(step, n) * n;
+      const double u1 = dot(tangential, t1), u2 = dot(tangential, t2);
+      const double bd = fc.blockT[static_cast<size_t>(c) * fc.K + c];
+      const double w11 = dot(t1, t1) * bd, w12 = dot(t1, t2) * bd, w22 = dot(t2, t2) * bd;
+      const double det = w11 * w22 - w12 * w12;
+      double s1 = 0.0, s2 = 0.0;
+      if (fabs(det) > 1.0e-20) {
+        s1 = -(w22 * u1 - w12 * u2) / det;
+        s2 = -(-w12 * u1 + w11 * u2) / det;
+      }
+      const double radius = fc.friction * fc.mult[c];
+      if (sqrt(s1 * s1 + s2 * s2) <= radius + 1.0e-14) {
+        fc.fl[2 * c] = s1;
+        fc.fl[2 * c + 1] = s2;
+      } else {
+        const double mag = sqrt(u1 * u1 + u2 * u2);
+        fc.fl[2 * c] = mag > 1.0e-14 ? -radius * u1 / mag : 0.0;
+        fc.fl[2 * c + 1] = mag > 1.0e-14 ? -radius * u2 / mag : 0.0;
+      }
+    }
+    __syncthreads();
+  }
+  fc_forces(fc, true);
+}
+
+// contact_solution: x = A^{-1} b + sum over coupled candidates of A^{-1} e_j f_j.
+__global__ __launch_bounds__(256) void k_fc_solution(FrictionDev fc, double* xr, const int* active) {
+#pragma clang fp contract(off)
+  if (!*active) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= fc.n) return;
+  const int A = *fc.nactive;
+  V3 v = ld3(xr, i);
+  for (int k = 0; k < A; ++k) {
+    const int j = fc.order[k];
+    v = v + fc.panel[static_cast<size_t>(j) * fc.n + i] * ld3(fc.force, j);
+  }
+  st3(xr, i, v);
+}
+
+// apply_restitution / apply_mesh_restitution (contact.py:587-660).
+__global__ __launch_bounds__(256) void k_fc_restitution(FrictionDev fc, const double* position, const float* velocity,
+                                                        double dt, double* x) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= fc.K || fc.mult[c] <= 0.0) return;
+  const int node = fc.nodes[c];
+  const V3 point = ld3(x, node);
+  V3 n;
+  double distance;
+  if (fc.kind == 2) {
+    V3 closest, face_normal;
+    double dist2;
+    mesh_closest(fc, point, node, fc.surface, closest, face_normal, dist2);
+    const V3 reference = ld3(fc.normal, c);
+    distance = dot(point - closest, reference) - fc.radius;
+    n = mesh_direction(point, closest, face_normal);
+    if (dot(n, reference) < 0.0) n = -1.0 * n;
+  } else {
+    distance = boundary_gap(fc, point);
+    n = boundary_normal(fc, point);
+  }
+  const double norm = sqrt(dot(n, n));
+  if (norm <= 1.0e-12) return;
+  n = V3{n.x / norm, n.y / norm, n.z / norm};
+  if (distance > (fc.kind == 2 ? 0.1 * fc.radius : 1.0e-8)) return;
+  const V3 v{static_cast<double>(velocity[3 * node]), static_cast<double>(velocity[3 * node + 1]),
+             static_cast<double>(velocity[3 * node + 2])};
+  const double incoming = dot(v, n);
+  if (incoming < 0.0) {
+    const V3 d = point - ld3(position, node);
+    const double outgoing = dot(V3{d.x / dt, d.y / dt, d.z / dt}, n);
+    const double change = -fc.restitution * incoming - outgoing;
+    if (change > 0.0) st3(x, node, point + (dt * change) * n);
+  }
+}
+
 }  // namespace
 
 void launch_mesh_contact(const MeshContactDev& mc, const float* pos32, const float* vel32, const double* pos64,
@@ -241,6 +735,45 @@ void launch_mesh_contact(const MeshContactDev& mc, const float* pos32, const flo
   }
   hipLaunchKernelGGL(k_mc_gather, dim3(blocks_for(mc.n)), dim3(256), 0, stream, mc, pos, vel, target, vbody, mass,
                      dt);
+}
+
+}  // namespace dphd
+
+namespace dphd {
+
+void launch_contact_units(const FrictionDev& fc, int j0, double* rhs, hipStream_t stream) {
+  hipLaunchKernelGGL(k_fc_units, dim3(blocks_for(fc.n)), dim3(256), 0, stream, fc, j0, rhs);
+}
+
+void launch_contact_store(const FrictionDev& fc, int j0, const double* out, hipStream_t stream) {
+  hipLaunchKernelGGL(k_fc_store, dim3(blocks_for(fc.n)), dim3(256), 0, stream, fc, j0, out);
+}
+
+void launch_contact_block(const FrictionDev& fc, hipStream_t stream) {
+  hipLaunchKernelGGL(k_fc_block, dim3(blocks_for(static_cast<long long>(fc.K) * fc.K)), dim3(256), 0, stream, fc);
+}
+
+void launch_contact_begin(const FrictionDev& fc, const double* position, double dt, hipStream_t stream) {
+  if (fc.K == 0) return;
+  hipLaunchKernelGGL(k_fc_begin, dim3(blocks_for(fc.K)), dim3(256), 0, stream, fc, position);
+  hipLaunchKernelGGL(k_fc_delassus, dim3(blocks_for(static_cast<long long>(fc.K) * fc.K)), dim3(256), 0, stream, fc,
+                     dt);
+}
+
+void launch_contact_iteration(const FrictionDev& fc, const double* x, double* xr, const double* position,
+                              const int* active, hipStream_t stream) {
+  if (fc.K == 0) return;
+  if (fc.kind == 2 && fc.restitution > 0.0)
+    hipLaunchKernelGGL(k_fc_surface, dim3(blocks_for(3LL * fc.n)), dim3(256), 0, stream, fc, x, active);
+  hipLaunchKernelGGL(k_fc_scalars, dim3(fc.K), dim3(64), 0, stream, fc, x, active);
+  hipLaunchKernelGGL(k_fc_newton, dim3(1), dim3(256), 0, stream, fc, x, xr, position, active);
+  hipLaunchKernelGGL(k_fc_solution, dim3(blocks_for(fc.n)), dim3(256), 0, stream, fc, xr, active);
+}
+
+void launch_contact_finish(const FrictionDev& fc, const double* position, const float* velocity, double dt, double* x,
+                           hipStream_t stream) {
+  if (fc.K == 0 || fc.restitution <= 0.0) return;
+  hipLaunchKernelGGL(k_fc_restitution, dim3(blocks_for(fc.K)), dim3(256), 0, stream, fc, position, velocity, dt, x);
 }
 
 }  // namespace dphd

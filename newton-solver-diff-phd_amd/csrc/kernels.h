@@ -198,7 +198,12 @@ struct StepScalars {
 // Launchers (all asynchronous on `stream`).
 void launch_begin_step(const DeviceState& d, const StepScalars& s, const float* q, const float* qd, const float* f,
                        const float* dirichlet, hipStream_t stream);
-void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const double* targets, hipStream_t stream);
+struct FrictionDev;
+// One local/global iteration of every active body; with a frictional
+// boundary `fc`, its Newton step turns the factor products' A^{-1} b into
+// A^{-1}(b + J^T lambda) before the Anderson step.
+void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const double* targets,
+                              const FrictionDev* fc, hipStream_t stream);
 void launch_any_active(const DeviceState& d, const int* active, hipStream_t stream);
 void launch_write_state(const DeviceState& d, const StepScalars& s, float* q, float* qd, double* checkpoint,
                         hipStream_t stream);

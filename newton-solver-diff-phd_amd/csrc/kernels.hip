@@ -22,6 +22,7 @@
 // flags built in LDS at kernel start. Every reduction is a fixed-shape
 // butterfly or a chunk-ordered sum: results are bitwise repeatable run to
 // run, and no float atomics are used.
+#include "contact.h"
 #include "kernels.h"
 #include "pd_math.h"
 
@@ -1577,11 +1578,13 @@ void launch_begin_step(const DeviceState& d, const StepScalars& s, const float* 
   hipLaunchKernelGGL(k_begin_step, dim3(d.nvc), dim3(kBlock), 0, stream, d, s, q, qd, f, dirichlet);
 }
 
-void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const double* targets, hipStream_t stream) {
+void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const double* targets,
+                              const FrictionDev* fc, hipStream_t stream) {
   const size_t lds = prefix_lds(d);
   hipLaunchKernelGGL(k_local, dim3(d.g_elem), dim3(kBlock), lds, stream, d, targets);
   hipLaunchKernelGGL(k_gather, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   spmv_iteration(d, d.defect ? d.x : nullptr, d.active, stream);
+  if (fc) launch_contact_iteration(*fc, d.x, d.xr, d.q0, d.active, stream);
   if (d.W > 0) hipLaunchKernelGGL(k_push_gram, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   hipLaunchKernelGGL(k_apply_conv, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   hipLaunchKernelGGL(k_conv_check, dim3(d.B), dim3(64), 0, stream, d, s);

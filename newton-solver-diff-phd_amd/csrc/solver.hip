@@ -95,6 +95,14 @@ struct dphd_solver {
   double mc_params[5] = {0, 0, 0, 0, 0};
   std::vector<int> mc_faces_a, mc_faces_b;
   dphd::MeshContactDev mc;
+  // frictional boundary (contact.py FrictionalContact)
+  bool has_friction = false;
+  int fc_kind = 0;
+  double fc_params[4] = {0, 0, 0, 0};
+  std::vector<int> fc_nodes, fc_faces;
+  double fc_friction = 0.0, fc_restitution = 0.0, fc_radius = 0.0;
+  int fc_split = 0, fc_left = 0;
+  dphd::FrictionDev fc;
   bool have_topology = false, have_particles = false, have_materials = false;
   // build products
   bool built = false;
@@ -133,6 +141,7 @@ namespace {
 int chunk_size(const dphd_solver* s) { return s->cfg.iteration_chunk > 0 ? s->cfg.iteration_chunk : 8; }
 
 void build_mesh_contact(dphd_solver* s);
+void build_friction(dphd_solver* s);
 
 void build_solver(dphd_solver* s, double dt) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -590,6 +599,7 @@ void build_solver(dphd_solver* s, double dt) {
   d.integrate = s->cfg.integrate_pinned_velocities ? 1 : 0;
   s->targets = muscles ? dupload(o, std::vector<double>(E, 1.0)) : nullptr;
   if (s->has_mesh_contact) build_mesh_contact(s);
+  if (s->has_friction) build_friction(s);
   void* host = nullptr;
   HIP_CHECK(hipHostMalloc(&host, sizeof(int), hipHostMallocMapped));
   s->any_host = static_cast<int*>(host);
@@ -687,6 +697,69 @@ void build_mesh_contact(dphd_solver* s) {
   mc.pairs = dzeros<int>(o, 1);
 }
 
+// Device state of the frictional boundary and its panel A^{-1} e_j of the
+// registered vertices (FrictionalContact.__init__ / build, contact.py:702-800).
+void build_friction(dphd_solver* s) {
+  if (s->B != 1) throw std::invalid_argument("a frictional boundary needs a single-body solver");
+  if (s->d.defect) throw std::domain_error("a frictional boundary needs the FP64 factor");
+  auto& o = s->owned;
+  const int K = static_cast<int>(s->fc_nodes.size()), n = s->N;
+  const size_t k1 = std::max(K, 1);
+  FrictionDev& fc = s->fc;
+  fc = FrictionDev{};
+  fc.kind = s->fc_kind;
+  fc.K = K;
+  fc.n = n;
+  fc.friction = s->fc_friction;
+  fc.restitution = s->fc_restitution;
+  fc.params = dupload(o, std::vector<double>(s->fc_params, s->fc_params + 4));
+  fc.nodes = dupload(o, s->fc_nodes.empty() ? std::vector<int>{0} : s->fc_nodes);
+  fc.panel = dzeros<double>(o, k1 * n);
+  fc.blockT = dzeros<double>(o, k1 * k1);
+  fc.delassus = dzeros<double>(o, k1 * k1);
+  fc.matrix = dzeros<double>(o, k1 * k1);
+  fc.reg = dzeros<double>(o, k1);
+  fc.normal = dzeros<double>(o, 3 * k1);
+  fc.t1 = dzeros<double>(o, 3 * k1);
+  fc.t2 = dzeros<double>(o, 3 * k1);
+  fc.fl = dzeros<double>(o, 2 * k1);
+  fc.mult = dzeros<double>(o, k1);
+  fc.gap = dzeros<double>(o, k1);
+  fc.omega = dzeros<double>(o, k1);
+  fc.diag = dzeros<double>(o, k1);
+  fc.resid = dzeros<double>(o, k1);
+  fc.rhs = dzeros<double>(o, k1);
+  fc.force = dzeros<double>(o, 3 * k1);
+  fc.response = dzeros<double>(o, 3 * k1);
+  fc.order = dzeros<int>(o, k1);
+  fc.slot = dzeros<int>(o, k1);
+  fc.nactive = dzeros<int>(o, 1);
+  fc.perm = dzeros<int>(o, k1);
+  fc.compact = dzeros<double>(o, k1);
+  fc.work = dzeros<double>(o, k1);
+  fc.candidate = dzeros<int>(o, k1);
+  if (fc.kind == 2) {
+    fc.radius = s->fc_radius;
+    fc.split = s->fc_split;
+    fc.left_count = s->fc_left;
+    fc.face_count = static_cast<int>(s->fc_faces.size() / 3);
+    fc.faces = dupload(o, s->fc_faces.empty() ? std::vector<int>{0, 0, 0} : s->fc_faces);
+    fc.surface = dzeros<double>(o, 3 * static_cast<size_t>(n));
+  }
+  if (K == 0) return;
+  double* rhs = dzeros<double>(o, 3 * static_cast<size_t>(n));
+  double* out = dzeros<double>(o, 3 * static_cast<size_t>(n));
+  hipStream_t st = nullptr;
+  for (int j0 = 0; j0 < K; j0 += 3) {
+    launch_contact_units(fc, j0, rhs, st);
+    launch_apply_inverse(s->d, rhs, out, st);
+    launch_contact_store(fc, j0, out, st);
+  }
+  launch_contact_block(fc, st);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipDeviceSynchronize());
+}
+
 StepScalars scalars(const dphd_solver* s) {
   return StepScalars{s->dt, s->cfg.abs_tol, s->cfg.rel_tol, s->cfg.max_iterations};
 }
@@ -700,7 +773,7 @@ hipGraphExec_t capture(dphd_solver* s, bool adjoint) {
     if (adjoint)
       launch_adjoint_iteration(s->d, sc, s->stream);
     else
-      launch_forward_iteration(s->d, sc, s->targets, s->stream);
+      launch_forward_iteration(s->d, sc, s->targets, s->has_friction ? &s->fc : nullptr, s->stream);
   }
   launch_any_active(s->d, adjoint ? s->d.aactive : s->d.active, s->stream);
   HIP_CHECK(hipStreamEndCapture(s->stream, &graph));
@@ -752,6 +825,9 @@ using dphd::set_error;
   } catch (const std::invalid_argument& e) {              \
     set_error(e.what());                                  \
     return DPHD_EINVAL;                                   \
+  } catch (const std::domain_error& e) {                  \
+    set_error(e.what());                                  \
+    return DPHD_ENOTIMPL;                                 \
   } catch (const std::bad_alloc&) {                       \
     set_error("out of memory");                           \
     return DPHD_ENOMEM;                                   \
@@ -970,6 +1046,88 @@ int dphd_set_mesh_contact(dphd_solver* s, double contact_radius, double kn_left,
   })
 }
 
+int dphd_set_frictional_boundary(dphd_solver* s, int kind, const double* parameters, int count, const int* vertices,
+                                 double friction, double restitution, int split, int left_count, int face_count,
+                                 const int* faces) {
+  DPHD_TRY({
+    if (!s || s->N == 0) throw std::invalid_argument("dphd_set_frictional_boundary: invalid arguments");
+    if (kind < 0) {
+      s->has_friction = false;
+      s->release();
+      return DPHD_OK;
+    }
+    if (kind > 2) throw std::invalid_argument("frictional boundary must be 'planar', 'spherical' or 'mesh'");
+    if (!parameters) throw std::invalid_argument("frictional boundary parameters must be four finite values");
+    double q[4];
+    for (int t = 0; t < 4; ++t) {
+      q[t] = parameters[t];
+      if (!std::isfinite(q[t])) throw std::invalid_argument("frictional boundary parameters must be four finite values");
+    }
+    std::vector<int> f;
+    if (kind == 2) {
+      if (q[0] < 0.0) throw std::invalid_argument("contact_radius must be finite and nonnegative");
+      if (face_count < 0 || left_count < 0 || left_count > face_count || (face_count > 0 && !faces))
+        throw std::invalid_argument("mesh boundary parameters are (contact_radius, split_index, faces_left, faces_right)");
+      f.assign(faces, faces + 3 * static_cast<size_t>(face_count));
+      for (const int v : f)
+        if (v < 0 || v >= s->N) throw std::invalid_argument("mesh boundary faces must reference existing particles");
+    } else if (kind == 0 && std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) <= 2.220446049250313e-16) {
+      throw std::invalid_argument("planar boundary needs a nonzero normal");
+    }
+    if (count < 0 || (count > 0 && !vertices)) throw std::invalid_argument("frictional vertices must be unique particle indices");
+    std::vector<int> nodes(vertices, vertices + count);
+    std::vector<char> seen(s->N, 0);
+    for (const int v : nodes) {
+      if (v < 0 || v >= s->N) throw std::invalid_argument("frictional vertices must reference existing particles");
+      if (seen[v]) throw std::invalid_argument("frictional vertices must be unique particle indices");
+      seen[v] = 1;
+    }
+    if (!std::isfinite(friction) || friction < 0.0) throw std::invalid_argument("friction must be finite and nonnegative");
+    if (!std::isfinite(restitution) || restitution < 0.0)
+      throw std::invalid_argument("restitution must be finite and nonnegative");
+    s->has_friction = true;
+    s->fc_kind = kind;
+    for (int t = 0; t < 4; ++t) s->fc_params[t] = kind == 2 ? (t == 0 ? q[0] : 0.0) : q[t];
+    s->fc_radius = kind == 2 ? q[0] : 0.0;
+    s->fc_split = split;
+    s->fc_left = left_count;
+    s->fc_faces = std::move(f);
+    s->fc_nodes = std::move(nodes);
+    s->fc_friction = friction;
+    s->fc_restitution = restitution;
+    s->release();
+  })
+}
+
+int dphd_boundary_parameters(dphd_solver* s, const double* set, double* get) {
+  DPHD_TRY({
+    if (!s || !s->built || !s->has_friction) throw std::invalid_argument("dphd_boundary_parameters: no frictional boundary");
+    HIP_CHECK(hipSetDevice(s->device));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    if (set) {
+      for (int t = 0; t < 4; ++t)
+        if (!std::isfinite(set[t])) throw std::invalid_argument("frictional boundary parameters must be four finite values");
+      HIP_CHECK(hipMemcpy(s->fc.params, set, sizeof(double) * 4, hipMemcpyHostToDevice));
+    }
+    if (get) HIP_CHECK(hipMemcpy(get, s->fc.params, sizeof(double) * 4, hipMemcpyDeviceToHost));
+  })
+}
+
+int dphd_contact_state(dphd_solver* s, const double* set_multiplier, double* multiplier, double* omega, double* normal,
+                       int* candidate) {
+  DPHD_TRY({
+    if (!s || !s->built || !s->has_friction) throw std::invalid_argument("dphd_contact_state: no frictional boundary");
+    HIP_CHECK(hipSetDevice(s->device));
+    HIP_CHECK(hipDeviceSynchronize());
+    const size_t K = s->fc.K;
+    if (set_multiplier && K) HIP_CHECK(hipMemcpy(s->fc.mult, set_multiplier, sizeof(double) * K, hipMemcpyDefault));
+    if (multiplier && K) HIP_CHECK(hipMemcpy(multiplier, s->fc.mult, sizeof(double) * K, hipMemcpyDefault));
+    if (omega && K) HIP_CHECK(hipMemcpy(omega, s->fc.omega, sizeof(double) * K, hipMemcpyDefault));
+    if (normal && K) HIP_CHECK(hipMemcpy(normal, s->fc.normal, sizeof(double) * 3 * K, hipMemcpyDefault));
+    if (candidate && K) HIP_CHECK(hipMemcpy(candidate, s->fc.candidate, sizeof(int) * K, hipMemcpyDefault));
+  })
+}
+
 int dphd_mesh_contact_force(dphd_solver* s, const double* position, const double* velocity, double* force,
                             void* stream) {
   DPHD_TRY({
@@ -1059,8 +1217,10 @@ int dphd_step(dphd_solver* s, const float* q_in, const float* qd_in, const float
     if (s->has_mesh_contact)
       dphd::launch_mesh_contact(s->mc, q_in, qd_in, nullptr, nullptr, s->d.qt, s->d.vbody, s->d.mass, s->dt,
                                 s->stream);
+    if (s->has_friction) dphd::launch_contact_begin(s->fc, s->d.q0, s->dt, s->stream);
     const int chunks = dphd::run_loop(s, false);
     s->last_fwd_iters = chunks * dphd::chunk_size(s);
+    if (s->has_friction) dphd::launch_contact_finish(s->fc, s->d.q0, qd_in, s->dt, s->d.x, s->stream);
     dphd::launch_write_state(s->d, sc, q_out, qd_out, checkpoint, s->stream);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(s->ev_out, s->stream));

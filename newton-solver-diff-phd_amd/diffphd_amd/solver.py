@@ -39,6 +39,13 @@ STATUS_ADJOINT_NOT_CONVERGED = 8
 STATUS_ADJOINT_SAFEGUARD = 16
 
 
+def _surface_vertices(tets):
+    """Vertices of faces of exactly one tetrahedron (contact.py:679-684), the default frictional vertices."""
+    faces = np.concatenate([tets[:, [0, 1, 2]], tets[:, [0, 1, 3]], tets[:, [0, 2, 3]], tets[:, [1, 2, 3]]])
+    unique, counts = np.unique(np.sort(faces, axis=1), axis=0, return_counts=True)
+    return np.unique(unique[counts == 1]).astype(np.int32)
+
+
 class ConvergenceError(RuntimeError):
     """A DiffPhD fixed point did not converge; a smaller interval may restore convergence."""
 
@@ -91,12 +98,10 @@ class SolverDiffPhD:
             raise ValueError("Explicit muscle_targets and planar_energies currently require hex_mesh")
         if hex_mesh is not None and any(x is not None for x in (frictional_boundary, mesh_contact)):
             raise NotImplementedError("Hex8 currently supports planar_energies, not frictional or mesh contact")
-        if frictional_boundary is not None:
-            raise NotImplementedError("frictional contact is not part of this solver path yet")
+        if frictional_boundary is None and (frictional_vertices is not None or friction or restitution):
+            raise ValueError("frictional_vertices, friction and restitution need a frictional_boundary")
         if mesh_contact is not None and len(mesh_contact) != 7:
             raise ValueError("mesh_contact is (contact_radius, kn_left, kn_right, kf, mu, faces_a, faces_b)")
-        if frictional_vertices is not None or friction or restitution:
-            raise ValueError("frictional_vertices, friction and restitution need a frictional_boundary")
         density = np.broadcast_to(np.asarray(density, dtype=np.float64), (1,) if np.ndim(density) == 0 else np.shape(density))
         if not np.all(np.isfinite(density)) or np.any(density <= 0):
             raise ValueError("density must be finite and positive")
@@ -245,6 +250,45 @@ class SolverDiffPhD:
                 "mesh_contact",
             )
             self._mesh_contact = (fa, fb)
+        # Frictional boundary (DP/solver.py:302-319, contact.py:686-780), validated by the library.
+        self._friction = None
+        if frictional_boundary is not None:
+            kind, params = frictional_boundary
+            kinds = {"planar": 0, "spherical": 1, "mesh": 2}
+            if kind not in kinds:
+                raise ValueError("frictional boundary must be 'planar', 'spherical' or 'mesh'")
+            faces = np.zeros((0, 3), np.int32)
+            split = left = 0
+            if kind == "mesh":
+                if len(params) != 4:
+                    raise ValueError("mesh boundary parameters are (contact_radius, split_index, faces_left, faces_right)")
+                radius, split = float(params[0]), int(params[1])
+                lf, rf = (np.asarray(f, dtype=np.int32).reshape(-1, 3) for f in params[2:4])
+                faces = np.ascontiguousarray(np.concatenate([lf, rf]))
+                left = len(lf)
+                values = np.array([radius, 0.0, 0.0, 0.0])
+                if frictional_vertices is None:
+                    frictional_vertices = np.unique(faces)
+            else:
+                values = np.asarray(params, dtype=np.float64)
+                if values.shape != (4,):
+                    raise ValueError("frictional boundary parameters must be four finite values")
+                if frictional_vertices is None:
+                    tets = model.tet_indices.detach().cpu().numpy().reshape(-1, 4)
+                    frictional_vertices = _surface_vertices(tets)
+            nodes = np.asarray(frictional_vertices)
+            if nodes.ndim != 1 or len(np.unique(nodes)) != len(nodes):
+                raise ValueError("frictional vertices must be unique particle indices")
+            nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+            values = np.ascontiguousarray(values, dtype=np.float64)
+            check(
+                lib().dphd_set_frictional_boundary(
+                    self._h, kinds[kind], ptr(values), len(nodes), ptr(nodes), float(friction), float(restitution),
+                    split, left, len(faces), ptr(faces),
+                ),
+                "frictional_boundary",
+            )
+            self._friction = {"kind": kind, "nodes": nodes, "friction": float(friction)}
         # The converged iterate of the last step in FP64 (the reference's solver._x).
         self._x = torch.zeros((self._n, 3), dtype=torch.float64, device=self.device)
         self._status_word = 0
@@ -408,6 +452,10 @@ class SolverDiffPhD:
                 raise NotImplementedError("Hex8 material gradients need Tet4 materials on the model")
             if self._planar is not None:
                 raise NotImplementedError("planar_energies are forward only: the reference records no Hex8 adjoint")
+            if self._friction is not None and (
+                materials.requires_grad or (activations is not None and activations.requires_grad)
+            ):
+                raise NotImplementedError("parameter gradients with a frictional boundary need the Woodbury rows")
             from .autograd import StepFunction
 
             q_out, qd_out = StepFunction.apply(
@@ -505,6 +553,49 @@ class SolverDiffPhD:
             raise ConvergenceError("A DiffPhD backward fixed point did not converge")
         if bits & STATUS_ADJOINT_SAFEGUARD:
             raise ConvergenceError("A reference DiffPhD backward safeguard modified the gradient")
+
+    @property
+    def contact_multipliers(self):
+        """Normal multipliers of the registered vertices after the last step (a copy), or None."""
+        if self._friction is None or self._factor_key is None:
+            return None
+        out = torch.zeros(len(self._friction["nodes"]), dtype=torch.float64, device=self.device)
+        check(lib().dphd_contact_state(self._h, None, ptr(out), None, None, None))
+        return out
+
+    @contact_multipliers.setter
+    def contact_multipliers(self, values):
+        """Replace the multipliers' warm start."""
+        values = torch.as_tensor(values, dtype=torch.float64, device=self.device).contiguous()
+        if self._friction is None or tuple(values.shape) != (len(self._friction["nodes"]),):
+            raise ValueError("contact_multipliers needs one value per registered vertex")
+        check(lib().dphd_contact_state(self._h, ptr(values), None, None, None, None))
+
+    def contact_state(self):
+        """(multipliers, omega, normals, candidate) of the last step for the adjoint's contact rows."""
+        k = len(self._friction["nodes"])
+        mult = torch.zeros(k, dtype=torch.float64, device=self.device)
+        omega = torch.zeros_like(mult)
+        normal = torch.zeros((k, 3), dtype=torch.float64, device=self.device)
+        candidate = torch.zeros(k, dtype=torch.int32, device=self.device)
+        check(lib().dphd_contact_state(self._h, None, ptr(mult), ptr(omega), ptr(normal), ptr(candidate)))
+        return mult, omega, normal, candidate
+
+    @property
+    def boundary_parameters(self):
+        """The four frictional boundary parameters (a copy), or None; assign to move the obstacle."""
+        if self._friction is None:
+            return None
+        if self._factor_key is None:
+            raise RuntimeError("boundary_parameters needs a built solver; step or prepare first")
+        out = np.zeros(4)
+        check(lib().dphd_boundary_parameters(self._h, None, ptr(out)))
+        return torch.as_tensor(out)
+
+    @boundary_parameters.setter
+    def boundary_parameters(self, values):
+        values = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(4))
+        check(lib().dphd_boundary_parameters(self._h, ptr(values), None), "boundary_parameters")
 
     @property
     def mesh_contact_pairs(self):

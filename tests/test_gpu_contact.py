@@ -1,0 +1,94 @@
+"""Frictional boundary contact on the GPU against the golden rollouts and the oracle.
+
+Golden rollouts of the reference's dense ``ReferenceContact`` /
+``ReferenceMeshContact`` (tests/golden/make_golden.py); every step is fed the
+golden's inputs and the multipliers carry over inside the solver, as in
+tests/test_diff_phd_contact.py and tests/test_diff_phd_mesh_boundary.py.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from scenes import contact_grid, golden, tet_model, two_blocks
+from test_contact import GRID_CASES, MESH_CASES
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(cuda, name):
+    from diffphd_amd import SolverDiffPhD
+
+    g = golden(f"contact_{name}")
+    options = dict(max_iterations=int(g["max_iterations"]), abs_tol=float(g["abs_tol"]), rel_tol=float(g["rel_tol"]))
+    if name.startswith("mesh"):
+        vertices, tets, poses, materials, free, split, left, right = two_blocks(0.012)
+        boundary = ("mesh", (float(g["radius"]), int(g["split"]), g["faces_left"], g["faces_right"]))
+    else:
+        vertices, tets, poses, materials, free = contact_grid()
+        boundary = (str(g["kind"]), tuple(g["params"]))
+    model = tet_model(cuda, tets, poses, materials, free, vertices, (0.0, 0.0, -9.81))
+    solver = SolverDiffPhD(
+        model, density=1000.0, frictional_boundary=boundary, frictional_vertices=g["nodes"],
+        friction=float(g["friction"]), restitution=float(g["restitution"]), **options,
+    )
+    return g, model, solver
+
+
+@pytest.mark.parametrize("name", GRID_CASES + MESH_CASES)
+def test_contact_rollout_matches_reference(cuda, name):
+    print(f"[case] frictional contact {name}", flush=True)
+    g, model, solver = _solver(cuda, name)
+    for t in range(len(g["x"])):
+        s_in, s_out = model.state(), model.state()
+        s_in.particle_q = torch.as_tensor(g["q"][t], device=cuda)
+        s_in.particle_qd = torch.as_tensor(g["qd"][t], device=cuda)
+        solver.step(s_in, s_out, None, None, 0.004)
+        bits = solver.read_status()
+        assert int(solver.iterations[0]) == int(g["iterations"][t]), (t, int(solver.iterations[0]))
+        assert (bits == 0) == bool(g["converged"][t])
+        if bool(g["converged"][t]):
+            x = solver.solution().cpu().numpy()
+            # 1e-9 m as the reference tests; where the gate stops at rel 1e-6 the
+            # block-ordered factor products may round the stopping iterate 10x further.
+            atol = 1e-9 if float(g["rel_tol"]) <= 1e-10 else 1e-8
+            np.testing.assert_allclose(x, g["x"][t], rtol=0, atol=atol, err_msg=f"step {t}")
+            # Multipliers are resolved only to the convergence gate on x and p. At rel 1e-11
+            # the reference tests' bounds hold; at rel 1e-6 (sticking mesh friction) a
+            # rounding-level change of the stopping iterate moves them by up to ~1e-4
+            # relative (the oracle itself sits 1e-5 from the golden there).
+            tol = dict(rtol=1e-5, atol=1e-6) if name.startswith("mesh") else dict(rtol=1e-6, atol=1e-9)
+            if float(g["rel_tol"]) > 1e-10:
+                tol = dict(rtol=1e-3, atol=1e-6)
+            mult = solver.contact_multipliers.cpu().numpy()
+            np.testing.assert_allclose(mult, g["multiplier"][t], **tol, err_msg=f"step {t}")
+
+
+def test_moving_sphere_parameters_apply_between_steps(cuda):
+    print("[case] frictional contact moving sphere", flush=True)
+    from diffphd_amd import SolverDiffPhD
+
+    vertices, tets, poses, materials, free = contact_grid()
+    model = tet_model(cuda, tets, poses, materials, free, vertices, (0.0, 0.0, -9.81))
+    center = vertices.astype(np.float64).mean(axis=0)
+    boundary = ("spherical", (center[0], center[1], -0.25, 0.25))
+    a = SolverDiffPhD(model, density=1000.0, frictional_boundary=boundary, friction=0.4)
+    s_in = model.state()
+    q = vertices.copy()
+    q[:, 2] += np.float32(0.0005) - q[:, 2].min()
+    s_in.particle_q = torch.as_tensor(q, device=cuda)
+    s_in.particle_qd = torch.as_tensor(np.tile(np.float32([0, 0, -1]), (len(q), 1)), device=cuda)
+    out_a = model.state()
+    a.step(s_in, out_a, None, None, 0.004)
+    for offset in (0.0, 0.01, 0.02):
+        params = (center[0] + offset, center[1], -0.25 + offset, 0.25)
+        b = SolverDiffPhD(model, density=1000.0, frictional_boundary=("spherical", params), friction=0.4)
+        b.prepare(0.004)
+        a.boundary_parameters = params
+        b.contact_multipliers = a.contact_multipliers
+        out_b = model.state()
+        a.step(s_in, out_a, None, None, 0.004)
+        b.step(s_in, out_b, None, None, 0.004)
+        np.testing.assert_array_equal(out_a.particle_q.cpu().numpy(), out_b.particle_q.cpu().numpy())
+        assert np.allclose(a.boundary_parameters.numpy(), params)
+    a.check_status()
