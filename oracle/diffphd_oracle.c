@@ -859,6 +859,208 @@ static int safe_clip(const oracle_problem* p, double* v, int chunks, double* par
   return st;
 }
 
+/* Fixed point (A - K) sol = rhs from a zero start with at most `limit`
+ * iterations (adjoint.py:927-964, 1054-1086): sol <- A^{-1}(rhs + K sol),
+ * safe clips, Dirichlet rows zeroed, Anderson, gate
+ * ||raw - prev|| < rel ||raw|| + abs. Returns the status bits. */
+static int fixed_point(const oracle_problem* p, const double* jac, const double* rhs, double* sol, int limit,
+                       int report, int* iterations) {
+  const int n = p->n, K = p->corners, Q = p->samples, E = p->elements;
+  const size_t vb = sizeof(double) * 3 * (size_t)n;
+  double *prev = malloc(vb), *work = malloc(vb), *raw = malloc(vb), *tmp = malloc(vb);
+  double* contrib = malloc(sizeof(double) * 3 * (size_t)E * Q * K);
+  const int chunks = n > 0 ? (n + CHUNK - 1) / CHUNK : 1;
+  double* part = malloc(sizeof(double) * 2 * chunks);
+  double* cpart = malloc(sizeof(double) * chunks);
+  int st = 0;
+  memset(sol, 0, vb);
+  anderson aa;
+  aa_init(&aa, n, p->window);
+  aa_reset(&aa);
+  int iteration = 0, active = 1;
+  for (int it = 0; it < limit && active; ++it) {
+    memcpy(prev, sol, vb);
+    memcpy(work, rhs, vb);
+#pragma omp parallel for schedule(static)
+    for (int sample = 0; sample < E * Q; ++sample) {
+      const int e = sample / Q, s = sample % Q;
+      double df[9], ds[9] = {0};
+      deformation(p, sol, e, s, df);
+      const double* J = jac + (size_t)sample * 81;
+      for (int c = 0; c < 9; ++c) {
+        const double w = df[3 * (c % 3) + c / 3];
+        for (int t = 0; t < 9; ++t) ds[t] = ds[t] + w * J[c * 9 + t];
+      }
+      const double* g = sample_grads(p, e, s);
+      for (int k = 0; k < K; ++k) m_vec(ds, g + 3 * k, contrib + 3 * ((size_t)sample * K + k));
+    }
+    gather(p, contrib, 1, work);
+    spmv(n, p->s_off, p->s_col, p->s_val, work, tmp);
+    spmv(n, p->t_off, p->t_col, p->t_val, tmp, raw);
+    st |= safe_clip(p, raw, chunks, cpart);
+    for (int c = 0; c < chunks; ++c) {
+      double tot = 0.0;
+      const int hi = (c + 1) * CHUNK < n ? (c + 1) * CHUNK : n;
+      for (int i = c * CHUNK; i < hi; ++i) {
+        if (!p->free[i]) raw[3 * i] = raw[3 * i + 1] = raw[3 * i + 2] = 0.0;
+        double s[3] = {raw[3 * i] - prev[3 * i], raw[3 * i + 1] - prev[3 * i + 1], raw[3 * i + 2] - prev[3 * i + 2]};
+        tot += v_dot(s, s);
+      }
+      part[2 * c] = tot;
+      part[2 * c + 1] = 0.0;
+    }
+    aa_mix(&aa, raw, prev);
+    st |= safe_clip(p, raw, chunks, cpart);
+    for (int c = 0; c < chunks; ++c) {
+      double tot = 0.0;
+      const int hi = (c + 1) * CHUNK < n ? (c + 1) * CHUNK : n;
+      for (int i = c * CHUNK; i < hi; ++i) {
+        for (int d = 0; d < 3; ++d) sol[3 * i + d] = raw[3 * i + d];
+        tot += v_dot(raw + 3 * i, raw + 3 * i);
+      }
+      part[2 * c + 1] = tot;
+    }
+    double t0 = 0.0, t1 = 0.0;
+    for (int c = 0; c < chunks; ++c) {
+      t0 += part[2 * c];
+      t1 += part[2 * c + 1];
+    }
+    const int k = iteration;
+    iteration = k + 1;
+    if (k > 0 && sqrt(t0) < p->rel_tol * sqrt(t1) + p->abs_tol) {
+      active = 0;
+    } else if (k + 1 >= limit) {
+      active = 0;
+      /* Woodbury columns are truncated by design; only the backbone reports. */
+      if (report) st |= 8;
+    }
+  }
+  *iterations = iteration;
+  aa_free(&aa);
+  free(prev);
+  free(work);
+  free(raw);
+  free(tmp);
+  free(contrib);
+  free(part);
+  free(cpart);
+  return st;
+}
+
+/* Woodbury projection of the backbone adjoint off the active contact rows
+ * (Adjoint._woodbury, adjoint.py:324-471, 663-806, 1272-1354): rows omega n
+ * of contacts with lambda > 0 and omega >= 0.5 (and, with friction, two
+ * tangent rows where lambda > 1e-8); rows with |J mu| below
+ * max(1e-12, 1e-3 max |J mu|) dropped; columns W_T = (A - K)^{-1} J^T by
+ * fixed points of at most min(max_iterations, 120) iterations; then
+ * z = mu - W_T G^{-1} (J mu) with G = J W_T (lower triangle, Eigen LDLT). */
+static int woodbury(const oracle_problem* p, const double* jac, const double* mu, int diverged, double* z) {
+  const oracle_contact* c = p->contact;
+  const int K = c->count, n = p->n;
+  const int tangents = c->friction > 0.0;
+  const int R = tangents ? 3 * K : K;
+  const size_t vb = sizeof(double) * 3 * (size_t)n;
+  int* row_node = calloc(R, sizeof(int));
+  int* row_valid = calloc(R, sizeof(int));
+  int* order = calloc(R, sizeof(int));
+  double* row_vec = calloc(3 * (size_t)R, sizeof(double));
+  double* value = calloc(R, sizeof(double));
+  for (int k = 0; k < K; ++k) {
+    const int node = c->nodes[k];
+    const double* nk = c->normal + 3 * (size_t)k;
+    const int active = !diverged && c->multiplier[k] > 0.0 && c->omega[k] >= 0.5;
+    row_node[k] = node;
+    for (int d = 0; d < 3; ++d) row_vec[3 * k + d] = c->omega[k] * nk[d];
+    row_valid[k] = active;
+    if (tangents) {
+      int axis = 0;
+      for (int d = 1; d < 3; ++d)
+        if (fabs(nk[d]) < fabs(nk[axis])) axis = d;
+      double unit[3] = {0.0, 0.0, 0.0}, t1[3], t2[3];
+      unit[axis] = 1.0;
+      const double nu = v_dot(nk, unit);
+      for (int d = 0; d < 3; ++d) t1[d] = unit[d] - nu * nk[d];
+      double length = sqrt(v_dot(t1, t1));
+      if (length > 1.0e-12) {
+        for (int d = 0; d < 3; ++d) t1[d] = t1[d] / length;
+      } else {
+        t1[0] = 1.0;
+        t1[1] = t1[2] = 0.0;
+      }
+      t2[0] = nk[1] * t1[2] - nk[2] * t1[1];
+      t2[1] = nk[2] * t1[0] - nk[0] * t1[2];
+      t2[2] = nk[0] * t1[1] - nk[1] * t1[0];
+      length = sqrt(v_dot(t2, t2));
+      if (length > 1.0e-12) {
+        for (int d = 0; d < 3; ++d) t2[d] = t2[d] / length;
+      } else {
+        t2[1] = 1.0;
+        t2[0] = t2[2] = 0.0;
+      }
+      const int valid = active && c->multiplier[k] > 1.0e-8;
+      for (int r = 0; r < 2; ++r) {
+        const int row = K + 2 * k + r;
+        row_node[row] = node;
+        memcpy(row_vec + 3 * row, r == 0 ? t1 : t2, sizeof(t1));
+        row_valid[row] = valid;
+      }
+    }
+  }
+  double largest = 0.0;
+  for (int r = 0; r < R; ++r) {
+    value[r] = row_valid[r] ? v_dot(row_vec + 3 * r, mu + 3 * (size_t)row_node[r]) : 0.0;
+    if (row_valid[r] && fabs(value[r]) > largest) largest = fabs(value[r]);
+  }
+  const double gate = 1.0e-3 * largest > 1.0e-12 ? 1.0e-3 * largest : 1.0e-12;
+  int count = 0;
+  for (int r = 0; r < R; ++r)
+    if (row_valid[r] && fabs(value[r]) >= gate) order[count++] = r;
+  int st = 0;
+  memcpy(z, mu, vb);
+  if (count > 0) {
+    const int limit = p->max_iterations < 120 ? p->max_iterations : 120;
+    double* columns = calloc((size_t)count * 3 * n, sizeof(double));
+    double* rhs = calloc(3 * (size_t)n, sizeof(double));
+    for (int k = 0; k < count; ++k) {
+      const int r = order[k];
+      memset(rhs, 0, vb);
+      memcpy(rhs + 3 * (size_t)row_node[r], row_vec + 3 * r, sizeof(double) * 3);
+      int iterations;
+      st |= fixed_point(p, jac, rhs, columns + (size_t)k * 3 * n, limit, 0, &iterations);
+    }
+    double* gram = calloc((size_t)count * count, sizeof(double));
+    double* coef = calloc(count, sizeof(double));
+    for (int i = 0; i < count; ++i) {
+      for (int j = 0; j < count; ++j) {
+        const int hi = i > j ? i : j, lo = i > j ? j : i;
+        const int rh = order[hi];
+        gram[(size_t)i * count + j] =
+            v_dot(row_vec + 3 * rh, columns + (size_t)lo * 3 * n + 3 * (size_t)row_node[rh]);
+      }
+      coef[i] = value[order[i]];
+    }
+    oracle_ldlt_solve(gram, count, coef);
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < count; ++k)
+        for (int d = 0; d < 3; ++d)
+          z[3 * (size_t)i + d] = z[3 * (size_t)i + d] - coef[k] * columns[(size_t)k * 3 * n + 3 * (size_t)i + d];
+    free(columns);
+    free(rhs);
+    free(gram);
+    free(coef);
+  }
+  const int chunks = n > 0 ? (n + CHUNK - 1) / CHUNK : 1;
+  double* cpart = malloc(sizeof(double) * chunks);
+  st |= safe_clip(p, z, chunks, cpart);
+  free(cpart);
+  free(row_node);
+  free(row_valid);
+  free(order);
+  free(row_vec);
+  free(value);
+  return st;
+}
+
 int oracle_backward(const oracle_problem* p, double dt, const double* x, const double* targets, const float* gq_out,
                     const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
                     float* gdirichlet, double* gactivations, double* mu_out, int* iterations, int* status) {
@@ -921,67 +1123,9 @@ int oracle_backward(const oracle_problem* p, double dt, const double* x, const d
       memcpy(jac + ((size_t)sample * 9 + c) * 9, dp, sizeof(dp));
     }
   }
-  /* Fixed point (A - K) mu = rhs (adjoint.py:927-964, 1054-1086). */
-  anderson aa;
-  aa_init(&aa, n, p->window);
-  aa_reset(&aa);
-  int iteration = 0, active = 1;
-  for (int it = 0; it < p->max_iterations && active; ++it) {
-    memcpy(prev, sol, vb);
-    memcpy(work, rhs, vb);
-#pragma omp parallel for schedule(static)
-    for (int sample = 0; sample < E * Q; ++sample) {
-      const int e = sample / Q, s = sample % Q;
-      double df[9], ds[9] = {0};
-      deformation(p, sol, e, s, df);
-      const double* J = jac + (size_t)sample * 81;
-      for (int c = 0; c < 9; ++c) {
-        const double w = df[3 * (c % 3) + c / 3];
-        for (int t = 0; t < 9; ++t) ds[t] = ds[t] + w * J[c * 9 + t];
-      }
-      const double* g = sample_grads(p, e, s);
-      for (int k = 0; k < K; ++k) m_vec(ds, g + 3 * k, contrib + 3 * ((size_t)sample * K + k));
-    }
-    gather(p, contrib, 1, work);
-    spmv(n, p->s_off, p->s_col, p->s_val, work, tmp);
-    spmv(n, p->t_off, p->t_col, p->t_val, tmp, raw);
-    st |= safe_clip(p, raw, chunks, cpart);
-    for (int c = 0; c < chunks; ++c) {
-      double tot = 0.0;
-      const int hi = (c + 1) * CHUNK < n ? (c + 1) * CHUNK : n;
-      for (int i = c * CHUNK; i < hi; ++i) {
-        if (!p->free[i]) raw[3 * i] = raw[3 * i + 1] = raw[3 * i + 2] = 0.0;
-        double s[3] = {raw[3 * i] - prev[3 * i], raw[3 * i + 1] - prev[3 * i + 1], raw[3 * i + 2] - prev[3 * i + 2]};
-        tot += v_dot(s, s);
-      }
-      part[2 * c] = tot;
-      part[2 * c + 1] = 0.0;
-    }
-    aa_mix(&aa, raw, prev);
-    st |= safe_clip(p, raw, chunks, cpart);
-    for (int c = 0; c < chunks; ++c) {
-      double tot = 0.0;
-      const int hi = (c + 1) * CHUNK < n ? (c + 1) * CHUNK : n;
-      for (int i = c * CHUNK; i < hi; ++i) {
-        for (int d = 0; d < 3; ++d) sol[3 * i + d] = raw[3 * i + d];
-        tot += v_dot(raw + 3 * i, raw + 3 * i);
-      }
-      part[2 * c + 1] = tot;
-    }
-    double t0 = 0.0, t1 = 0.0;
-    for (int c = 0; c < chunks; ++c) {
-      t0 += part[2 * c];
-      t1 += part[2 * c + 1];
-    }
-    const int k = iteration;
-    iteration = k + 1;
-    if (k > 0 && sqrt(t0) < p->rel_tol * sqrt(t1) + p->abs_tol) {
-      active = 0;
-    } else if (k + 1 >= p->max_iterations) {
-      active = 0;
-      st |= 8;
-    }
-  }
+  /* Fixed point (A - K) mu = rhs. */
+  int iteration = 0;
+  st |= fixed_point(p, jac, rhs, sol, p->max_iterations, 1, &iteration);
   /* Routing guard (adjoint.py:226-261). */
   double r0 = 0.0, r1 = 0.0;
   for (int c = 0; c < chunks; ++c) {
@@ -1006,6 +1150,15 @@ int oracle_backward(const oracle_problem* p, double dt, const double* x, const d
       scale = cap / inj;
       st |= 16;
     }
+  }
+  /* Parameter gradients use z, the backbone projected off the active
+   * contact rows; state gradients use mu (adjoint.py:1145-1148). */
+  double* z = sol;
+  double* zbuf = NULL;
+  if ((gmaterials || gactivations) && p->contact && p->contact->count > 0) {
+    zbuf = malloc(vb);
+    st |= woodbury(p, jac, sol, diverged, zbuf);
+    z = zbuf;
   }
   /* State gradients from mu (adjoint.py:474-490). */
   for (int i = 0; i < n; ++i)
@@ -1093,7 +1246,7 @@ int oracle_backward(const oracle_problem* p, double dt, const double* x, const d
         const double* gr = sample_grads(p, e, s);
         double fv = 0.0, sv = 0.0;
         for (int k = 0; k < K; ++k) {
-          const double* zk = sol + 3 * (size_t)p->indices[(size_t)e * K + k];
+          const double* zk = z + 3 * (size_t)p->indices[(size_t)e * K + k];
           double a[3], b[3];
           m_vec(first, gr + 3 * k, a);
           m_vec(second, gr + 3 * k, b);
@@ -1122,7 +1275,7 @@ int oracle_backward(const oracle_problem* p, double dt, const double* x, const d
       const double* gr = sample_grads(p, e, 0);
       double value = 0.0;
       for (int k = 0; k < K; ++k) {
-        const double* zk = sol + 3 * (size_t)p->indices[(size_t)e * K + k];
+        const double* zk = z + 3 * (size_t)p->indices[(size_t)e * K + k];
         const double c = wm / length * v_dot(m, gr + 3 * k);
         const double t[3] = {c * fibre[0], c * fibre[1], c * fibre[2]};
         value += v_dot(zk, t);
@@ -1149,7 +1302,7 @@ int oracle_backward(const oracle_problem* p, double dt, const double* x, const d
   if (mu_out) memcpy(mu_out, sol, vb);
   *iterations = iteration;
   *status = st;
-  aa_free(&aa);
+  free(zbuf);
   free(rhs);
   free(base);
   free(sol);

@@ -27,6 +27,8 @@ reference's own tests:
   stick/slide friction, sphere, chatter, restitution) and ``ReferenceMeshContact``
   (two blocks: frictionless, sticking, restitution), as in
   ``tests/test_diff_phd_contact.py`` and ``tests/test_diff_phd_mesh_boundary.py``;
+* ``contact_woodbury*`` ``reference_backward`` with the contact rows after a
+  fibre rollout against a plane (``tests/test_diff_phd_adjoint.py:339-416``);
 * ``cantilever_geometry`` the Hex8 cells, reference points, root vertices and
   x-cell indices of ``examples/assets/cantilever.usda``, parsed as text.
 """
@@ -403,6 +405,58 @@ def main():
             HERE / f"contact_{name}.npz", kind="mesh", radius=radius, split=split, faces_left=left,
             faces_right=right, friction=friction, restitution=restitution, **{k: options[k] for k in options},
             **out,
+        )
+
+    # Woodbury projection of the parameter gradients with contact
+    # (tests/test_diff_phd_adjoint.py:339-416): fibres, a plane, two steps, then
+    # the recorded step's reference_backward with the contact rows. The rows'
+    # omega and normals come from this repository's oracle run on the same
+    # step inputs (the reference test takes them from its solver), every other
+    # input and the backward itself from the reference's oracle.
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as own_oracle
+
+    for name, friction in (("woodbury", 0.0), ("woodbury_friction", 0.5)):
+        vertices, tets, poses, materials, free = contact_grid()
+        nodes = setup_surface_vertices(tets)
+        rng = np.random.default_rng(31)
+        directions = rng.standard_normal((len(tets), 3))
+        contraction = rng.uniform(0.0, 0.3, len(tets)).astype(np.float32)
+        targets = 1.0 - contraction.astype(np.float64)
+        step = ro.ReferenceStep(
+            tets, poses, materials[:, 0].astype(np.float64), materials[:, 1].astype(np.float64), DENSITY, free, "pd",
+            muscle_directions=directions, muscle_stiffness=4.0e4,
+        )
+        contact = ro.ReferenceContact("planar", (0.0, 0.0, 1.0, 0.0), nodes, friction=friction)
+        own = own_oracle.OracleBody(
+            "tet4", tets, poses=poses, materials=materials, density=DENSITY, inverse_mass=free.astype(np.float32),
+            gravity=gravity, dt=dt, muscle_directions=directions, muscle_stiffness=4.0e4,
+            frictional_boundary=("planar", (0.0, 0.0, 1.0, 0.0)), frictional_vertices=nodes, friction=friction,
+            **TIGHT_CONTACT,
+        )
+        q, v = lifted(vertices, 0.001, (0.3, 0.0, -1.0))
+        qs, vs = [], []
+        for _ in range(3):
+            x, iterations, converged = step.step(
+                q.astype(np.float64), v.astype(np.float64), np.zeros(q.shape), gravity, dt, contact=contact,
+                targets=targets, **TIGHT_CONTACT,
+            )
+            own.forward(dt, q, v, targets=targets)
+            qs.append(q)
+            vs.append(v)
+            q, v = x.astype(np.float32), ((x - q.astype(np.float64)) / dt).astype(np.float32)
+        q_weight = rng.standard_normal((len(vertices), 3)).astype(np.float32)
+        qd_weight = (0.01 * rng.standard_normal((len(vertices), 3))).astype(np.float32)
+        out = ro.reference_backward(
+            step, x, dt, q_weight.astype(np.float64), qd_weight.astype(np.float64),
+            contact=(nodes, contact.multiplier, own.contact["omega"].copy(), own.contact["normal"].copy(), friction),
+            targets=targets,
+        )
+        save(
+            HERE / f"contact_{name}.npz", directions=directions, contraction=contraction, targets=targets,
+            friction=friction, nodes=nodes, q=np.array(qs), qd=np.array(vs), x=x, multiplier=contact.multiplier,
+            omega=own.contact["omega"], normal=own.contact["normal"], gq=q_weight, gqd=qd_weight,
+            **{f"out_{k}": np.asarray(val) for k, val in out.items()}, **TIGHT_CONTACT,
         )
 
     # The Hex8 cantilever geometry of the reference example (text of its USD asset).

@@ -46,3 +46,31 @@ def test_oracle_contact_rollout_matches_reference(name):
             np.testing.assert_allclose(body.contact["multiplier"], g["multiplier"][t], **tol, err_msg=f"step {t}")
     if name != "chatter":
         assert np.count_nonzero(g["multiplier"]) > 0
+
+
+def woodbury_body(name):
+    g = golden(f"contact_{name}")
+    vertices, tets, poses, materials, free = contact_grid()
+    body = O.OracleBody(
+        "tet4", tets, poses=poses, materials=materials, density=1000.0, inverse_mass=free.astype(np.float32),
+        gravity=(0.0, 0.0, -9.81), dt=0.004, muscle_directions=g["directions"], muscle_stiffness=4.0e4,
+        frictional_boundary=("planar", (0.0, 0.0, 1.0, 0.0)), frictional_vertices=g["nodes"],
+        friction=float(g["friction"]), max_iterations=int(g["max_iterations"]), abs_tol=float(g["abs_tol"]),
+        rel_tol=float(g["rel_tol"]),
+    )
+    return g, body
+
+
+@pytest.mark.parametrize("name", ["woodbury", "woodbury_friction"])
+def test_oracle_woodbury_projection_matches_dense_reference(name):
+    g, body = woodbury_body(name)
+    for t in range(3):
+        out = body.forward(0.004, g["q"][t], g["qd"][t], targets=g["targets"])
+        assert out["status"] == 0
+    np.testing.assert_allclose(out["x"], g["x"], rtol=0, atol=1e-9)
+    bw = body.backward(0.004, out["x"], g["gq"], g["gqd"], targets=g["targets"], activations=True)
+    assert bw["status"] & 8 == 0
+    for key in ("q", "qd", "f", "materials", "activations"):
+        ref = g[f"out_{key}"]
+        scale = max(np.abs(ref).max(), 1e-30)
+        np.testing.assert_allclose(bw[key], ref, rtol=0, atol=1e-5 * scale, err_msg=key)
