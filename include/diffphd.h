@@ -155,6 +155,12 @@ int dphd_boundary_parameters(dphd_solver* s, const double* set, double* get);
  * start). Host or device pointers; synchronizes. */
 int dphd_contact_state(dphd_solver* s, const double* set_multiplier, double* multiplier, double* omega, double* normal,
                        int* candidate);
+/* The contact state a following dphd_backward projects its parameter
+ * gradients with (StepRecord.contact, DP/adjoint.py:809-841): multipliers
+ * [K], omega [K], normals [K,3] of the recorded step, e.g. copies taken with
+ * dphd_contact_state after it. Every dphd_step records its own; this
+ * replaces it for the backward of an earlier step. Host or device pointers. */
+int dphd_contact_record(dphd_solver* s, const double* multiplier, const double* omega, const double* normal);
 /* Mesh contact force [N,3] float64 of float64 positions/velocities [N,3]
  * (MeshContact.evaluate, DP/mesh_contact.py:245-272); device pointers. */
 int dphd_mesh_contact_force(dphd_solver* s, const double* position, const double* velocity, double* force,
@@ -186,7 +192,9 @@ int dphd_step(dphd_solver* s, const float* q_in, const float* qd_in, const float
  * step used explicit Dirichlet targets) and gactivations [E] float64
  * (d/du of the Tet4 fibre contraction u = 1 - target, DP/adjoint.py:595-620;
  * NULL skips them, non-NULL needs Tet4 fibres). Material and activation
- * gradients pass the reference's aggregate guard (DP/adjoint.py:622-659). */
+ * gradients pass the reference's aggregate guard (DP/adjoint.py:622-659);
+ * with a frictional boundary they use the adjoint projected off the
+ * recorded step's active contact rows (Woodbury, DP/adjoint.py:1272-1354). */
 int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle_targets, const float* gq_out,
                   const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
                   float* gdirichlet, double* gactivations, void* stream);

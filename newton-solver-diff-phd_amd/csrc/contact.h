@@ -114,3 +114,44 @@ void launch_contact_finish(const FrictionDev& fc, const double* position, const 
                            hipStream_t stream);
 
 }  // namespace dphd
+
+namespace dphd {
+
+// Woodbury projection of the adjoint off the active contact rows for the
+// parameter gradients (the reference's Adjoint._woodbury,
+// src/newton_solver_diff_phd/_src/solvers/diff_phd/adjoint.py:324-471,
+// 1272-1354). Rows: omega n of each registered vertex (and, with friction,
+// two tangent rows), valid where the recorded multiplier is positive and
+// omega >= 0.5; selected where |J mu| >= max(1e-12, 1e-3 max |J mu|). Each
+// selected row's column (A - K)^{-1} J^T is an adjoint fixed point run by the
+// solver; then z = mu - W_T G^{-1} (J mu).
+struct WoodburyDev {
+  int K = 0, R = 0, n = 0;
+  int tangents = 0;
+  const int* nodes = nullptr;
+  const double *mult = nullptr, *omega = nullptr, *normal = nullptr;  // the recorded step's contact state
+  int* row_node = nullptr;     // [R]
+  double* row_vec = nullptr;   // [R*3]
+  int* row_valid = nullptr;    // [R]
+  double* value = nullptr;     // [R] J mu
+  int* order = nullptr;        // [R] selected rows, ascending
+  int* count = nullptr;        // [1]
+  double* columns = nullptr;   // [count][n*3]
+  double* gram = nullptr;      // [R*R]
+  double* coef = nullptr;      // [R]
+  int* perm = nullptr;         // [R]
+  double* work = nullptr;      // [R]
+};
+
+// Rows, values and the selection (one workgroup); `route` holds the routing
+// guard's (scale, diverged) of body 0.
+void launch_woodbury_rows(const WoodburyDev& w, const double* mu, const double* route, hipStream_t stream);
+// Seed the adjoint solve of the k-th selected row's column: rhs = J_r^T,
+// solution zero, iteration state of body 0 reset.
+void launch_woodbury_seed(const WoodburyDev& w, int k, double* rhs, double* sol, int* aiter, int* aactive, int* aa,
+                          hipStream_t stream);
+// G alpha = J mu and z = mu - sum_k alpha_k column_k, then the safe clip of z.
+void launch_woodbury_project(const WoodburyDev& w, const double* mu, const uint8_t* free, double* z, int* status,
+                             hipStream_t stream);
+
+}  // namespace dphd

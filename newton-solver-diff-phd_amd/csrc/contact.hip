@@ -718,6 +718,157 @@ __global__ __launch_bounds__(256) void k_fc_restitution(FrictionDev fc, const do
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Woodbury rows and projection (adjoint.py:324-471, 1272-1354).
+
+// _backward_tangents (adjoint.py:324-345).
+__device__ void backward_tangents(V3 n, V3& t1, V3& t2) {
+#pragma clang fp contract(off)
+  int axis = 0;
+  for (int d = 1; d < 3; ++d)
+    if (fabs(vget(n, d)) < fabs(vget(n, axis))) axis = d;
+  V3 unit{0, 0, 0};
+  vset(unit, axis, 1.0);
+  t1 = unit - dot(n, unit) * n;
+  double length = sqrt(dot(t1, t1));
+  t1 = length > 1.0e-12 ? V3{t1.x / length, t1.y / length, t1.z / length} : V3{1, 0, 0};
+  t2 = cross3(n, t1);
+  length = sqrt(dot(t2, t2));
+  t2 = length > 1.0e-12 ? V3{t2.x / length, t2.y / length, t2.z / length} : V3{0, 1, 0};
+}
+
+// _basis, _project_rows, _select_rows, _compact_rows in one workgroup.
+__global__ __launch_bounds__(256) void k_wb_rows(WoodburyDev w, const double* mu, const double* route) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool diverged = route[1] != 0.0;
+  for (int c = tid; c < w.K; c += 256) {
+    const int node = w.nodes[c];
+    const V3 n = ld3(w.normal, c);
+    const bool active = !diverged && w.mult[c] > 0.0 && w.omega[c] >= 0.5;
+    w.row_node[c] = node;
+    st3(w.row_vec, c, w.omega[c] * n);
+    w.row_valid[c] = active ? 1 : 0;
+    if (w.tangents) {
+      V3 t1, t2;
+      backward_tangents(n, t1, t2);
+      const int valid = (active && w.mult[c] > 1.0e-8) ? 1 : 0;
+      for (int r = 0; r < 2; ++r) {
+        const int row = w.K + 2 * c + r;
+        w.row_node[row] = node;
+        st3(w.row_vec, row, r == 0 ? t1 : t2);
+        w.row_valid[row] = valid;
+      }
+    }
+  }
+  __syncthreads();
+  __shared__ double s_max[4];
+  double largest = 0.0;
+  for (int r = tid; r < w.R; r += 256) {
+    const double v = w.row_valid[r] ? dot(ld3(w.row_vec, r), ld3(mu, w.row_node[r])) : 0.0;
+    w.value[r] = v;
+    if (w.row_valid[r]) largest = fmax(largest, fabs(v));
+  }
+  for (int off = 32; off > 0; off >>= 1) largest = fmax(largest, __shfl_xor(largest, off, 64));
+  if (lane == 0) s_max[wave] = largest;
+  __syncthreads();
+  const double gate = fmax(1.0e-12, 1.0e-3 * fmax(fmax(s_max[0], s_max[1]), fmax(s_max[2], s_max[3])));
+  __shared__ int s_wave[4];
+  __shared__ int s_base;
+  if (tid == 0) s_base = 0;
+  __syncthreads();
+  for (int start = 0; start < w.R; start += 256) {
+    const int r = start + tid;
+    const bool flag = r < w.R && w.row_valid[r] && fabs(w.value[r]) >= gate;
+    const uint64_t mask = __ballot(flag);
+    if (lane == 0) s_wave[wave] = __popcll(mask);
+    __syncthreads();
+    int offset = s_base;
+    for (int q = 0; q < wave; ++q) offset += s_wave[q];
+    if (flag) w.order[offset + __popcll(mask & ((1ull << lane) - 1ull))] = r;
+    __syncthreads();
+    if (tid == 0) s_base += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    __syncthreads();
+  }
+  if (tid == 0) *w.count = s_base;
+}
+
+__global__ __launch_bounds__(256) void k_wb_seed(WoodburyDev w, int k, double* rhs, double* sol, int* aiter,
+                                                 int* aactive, int* aa) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    *aiter = 0;
+    *aactive = 1;
+    aa[0] = aa[1] = 0;
+  }
+  if (i >= w.n) return;
+  const int r = w.order[k];
+  const V3 v = i == w.row_node[r] ? ld3(w.row_vec, r) : V3{0, 0, 0};
+  st3(rhs, i, v);
+  st3(sol, i, V3{0, 0, 0});
+}
+
+// _gram (adjoint.py:404-425) and the Eigen LDLT of G over the selected rows.
+__global__ __launch_bounds__(256) void k_wb_solve(WoodburyDev w) {
+#pragma clang fp contract(off)
+  const int count = *w.count;
+  if (count == 0) return;
+  const size_t stride = 3 * static_cast<size_t>(w.n);
+  for (long long t = threadIdx.x; t < static_cast<long long>(count) * count; t += 256) {
+    const int i = static_cast<int>(t / count), j = static_cast<int>(t % count);
+    const int hi = i > j ? i : j, lo = i > j ? j : i;
+    const int rh = w.order[hi];
+    w.gram[t] = dot(ld3(w.row_vec, rh), ld3(w.columns + lo * stride, w.row_node[rh]));
+    if (j == 0) w.coef[i] = w.value[w.order[i]];
+  }
+  __syncthreads();
+  ldlt_solve_block(w.gram, count, w.coef, w.perm, w.work);
+}
+
+// _project_adjoint (adjoint.py:457-471): z = mu - sum_k alpha_k column_k in row order.
+__global__ __launch_bounds__(256) void k_wb_project(WoodburyDev w, const double* mu, double* z) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.n) return;
+  const int count = *w.count;
+  const size_t stride = 3 * static_cast<size_t>(w.n);
+  V3 v = ld3(mu, i);
+  for (int k = 0; k < count; ++k) v = v - w.coef[k] * ld3(w.columns + k * stride, i);
+  st3(z, i, v);
+}
+
+// _safe_clip of one vector (adjoint.py:179-221): non-finite entries of free
+// rows scrubbed to zero, Dirichlet rows zeroed, the norm capped at 1e6;
+// either safeguard sets status 16.
+__global__ __launch_bounds__(256) void k_wb_clip(int n, const uint8_t* free, double* z, int* status) {
+#pragma clang fp contract(off)
+  __shared__ double s_sum[4];
+  double total = 0.0;
+  bool scrubbed = false;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    V3 v = free[i] ? ld3(z, i) : V3{0, 0, 0};
+    for (int d = 0; d < 3; ++d)
+      if (!isfinite(vget(v, d))) {
+        vset(v, d, 0.0);
+        scrubbed = true;
+      }
+    st3(z, i, v);
+    total += dot(v, v);
+  }
+  if (scrubbed) atomicOr(status, 16);
+  for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off, 64);
+  if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = total;
+  __syncthreads();
+  const double length = sqrt(((s_sum[0] + s_sum[1]) + s_sum[2]) + s_sum[3]);
+  const double limit = 1.0e6;
+  if (isfinite(length) && length > limit) {
+    const double scale = limit / length;
+    for (int t = threadIdx.x; t < 3 * n; t += 256) z[t] = scale * z[t];
+    if (threadIdx.x == 0) atomicOr(status, 16);
+  }
+}
+
 }  // namespace
 
 void launch_mesh_contact(const MeshContactDev& mc, const float* pos32, const float* vel32, const double* pos64,
@@ -774,6 +925,26 @@ void launch_contact_finish(const FrictionDev& fc, const double* position, const 
                            hipStream_t stream) {
   if (fc.K == 0 || fc.restitution <= 0.0) return;
   hipLaunchKernelGGL(k_fc_restitution, dim3(blocks_for(fc.K)), dim3(256), 0, stream, fc, position, velocity, dt, x);
+}
+
+}  // namespace dphd
+
+namespace dphd {
+
+void launch_woodbury_rows(const WoodburyDev& w, const double* mu, const double* route, hipStream_t stream) {
+  hipLaunchKernelGGL(k_wb_rows, dim3(1), dim3(256), 0, stream, w, mu, route);
+}
+
+void launch_woodbury_seed(const WoodburyDev& w, int k, double* rhs, double* sol, int* aiter, int* aactive, int* aa,
+                          hipStream_t stream) {
+  hipLaunchKernelGGL(k_wb_seed, dim3(blocks_for(w.n)), dim3(256), 0, stream, w, k, rhs, sol, aiter, aactive, aa);
+}
+
+void launch_woodbury_project(const WoodburyDev& w, const double* mu, const uint8_t* free, double* z, int* status,
+                             hipStream_t stream) {
+  hipLaunchKernelGGL(k_wb_solve, dim3(1), dim3(256), 0, stream, w);
+  hipLaunchKernelGGL(k_wb_project, dim3(blocks_for(w.n)), dim3(256), 0, stream, w, mu, z);
+  hipLaunchKernelGGL(k_wb_clip, dim3(1), dim3(256), 0, stream, w.n, free, z, status);
 }
 
 }  // namespace dphd

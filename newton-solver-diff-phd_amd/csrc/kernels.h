@@ -193,6 +193,7 @@ struct StepScalars {
   double dt;
   double abs_tol, rel_tol;
   int max_iterations;
+  int report = 1;  // an adjoint solve that hits max_iterations sets status 8
 };
 
 // Launchers (all asynchronous on `stream`).
@@ -211,11 +212,15 @@ void launch_write_state(const DeviceState& d, const StepScalars& s, float* q, fl
 void launch_adjoint_setup(const DeviceState& d, const StepScalars& s, const double* x, const double* targets,
                           const float* gq_out, const float* gqd_out, hipStream_t stream);
 void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStream_t stream);
+// Routing guard of a finished adjoint solve (scale and divergence flag per body).
+void launch_adjoint_route(const DeviceState& d, hipStream_t stream);
 // State, Dirichlet, material (gmat) and activation (gact, Tet4 fibres)
-// gradients of a finished adjoint solve; gmat/gact may be null.
+// gradients of a finished, routed adjoint solve; gmat/gact may be null. The
+// parameter gradients read z (the Woodbury-projected adjoint with contact)
+// or, when z is null, the adjoint itself.
 void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const double* x, const float* gqd_out,
                             float* gq_in, float* gqd_in, float* gf_in, double* gmat, double* gact, float* gdir,
-                            bool any_pinned, hipStream_t stream);
+                            const double* z, bool any_pinned, hipStream_t stream);
 void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, hipStream_t stream);
 void launch_hex_determinants(const DeviceState& d, const float* q, double* out, hipStream_t stream);
 

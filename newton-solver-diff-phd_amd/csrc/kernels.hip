@@ -1238,7 +1238,8 @@ __global__ void k_adj_check(DeviceState d, StepScalars s) {
     d.aactive[b] = 0;
   } else if (k + 1 >= s.max_iterations) {
     d.aactive[b] = 0;
-    atomicOr(d.status, 8);
+    // Woodbury columns are truncated by design; only the backbone reports.
+    if (s.report) atomicOr(d.status, 8);
   }
 }
 
@@ -1335,7 +1336,7 @@ __device__ __forceinline__ void material_terms(const M3& f, double& first_out, d
   second_out = sv;
 }
 
-__global__ __launch_bounds__(kBlock) void k_adj_material_tet4(DeviceState d, const double* x) {
+__global__ __launch_bounds__(kBlock) void k_adj_material_tet4(DeviceState d, const double* x, const double* zs) {
   const Chunk c = d.echunks[blockIdx.x];
   const int e = c.begin + threadIdx.x;
   if (e >= c.end) return;
@@ -1346,7 +1347,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_material_tet4(DeviceState d, con
   for (int k = 0; k < 4; ++k) {
     const int n = d.idx[4 * e + k];
     add_outer(f, ld3(x, n), g[k]);
-    z[k] = ld3(d.sol, n);
+    z[k] = ld3(zs, n);
   }
   double fv, sv;
   if (d.material == 0) {
@@ -1363,7 +1364,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_material_tet4(DeviceState d, con
   d.gmat_tmp[2 * e + 1] = d.material == 0 ? vol * sv : 0.0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_adj_material_hex8(DeviceState d, const double* x) {
+__global__ __launch_bounds__(kBlock) void k_adj_material_hex8(DeviceState d, const double* x, const double* zs) {
   const Chunk c = d.echunks[blockIdx.x];
   const int b = c.body;
   const int s = threadIdx.x & 7;
@@ -1377,7 +1378,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_material_hex8(DeviceState d, con
       const int n = d.idx[8 * e + j];
       g[j] = hex_grad(d, b, s, j);
       add_outer(f, ld3(x, n), g[j]);
-      z[j] = ld3(d.sol, n);
+      z[j] = ld3(zs, n);
     }
     if (d.material == 0) {
       material_terms(f, fv, sv, g, z, 8);
@@ -1402,7 +1403,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_material_hex8(DeviceState d, con
 // contraction u with target length a = 1 - u is
 // -sum_k z_k . (w_m / |F m|) (m . g_k) F m, zero for a disabled or collapsed
 // fibre.
-__global__ __launch_bounds__(kBlock) void k_adj_activation_tet4(DeviceState d, const double* x) {
+__global__ __launch_bounds__(kBlock) void k_adj_activation_tet4(DeviceState d, const double* x, const double* zs) {
   const Chunk c = d.echunks[blockIdx.x];
   const int e = c.begin + threadIdx.x;
   if (e >= c.end) return;
@@ -1418,7 +1419,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_activation_tet4(DeviceState d, c
     const double len = sqrt(dot(fibre, fibre));
     if (len > 2.220446049250313e-16) {
       for (int k = 0; k < 4; ++k) {
-        const V3 z = ld3(d.sol, d.idx[4 * e + k]);
+        const V3 z = ld3(zs, d.idx[4 * e + k]);
         value += dot(z, (wm / len * dot(m, g[k])) * fibre);
       }
       value = -value;
@@ -1620,10 +1621,14 @@ void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStr
   hipLaunchKernelGGL(k_adj_check, dim3(d.B), dim3(64), 0, stream, d, s);
 }
 
+void launch_adjoint_route(const DeviceState& d, hipStream_t stream) {
+  hipLaunchKernelGGL(k_adj_route, dim3(d.B), dim3(kBlock), 0, stream, d);
+}
+
 void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const double* x, const float* gqd_out,
                             float* gq_in, float* gqd_in, float* gf_in, double* gmat, double* gact, float* gdir,
-                            bool any_pinned, hipStream_t stream) {
-  hipLaunchKernelGGL(k_adj_route, dim3(d.B), dim3(kBlock), 0, stream, d);
+                            const double* z, bool any_pinned, hipStream_t stream) {
+  if (!z) z = d.sol;
   hipLaunchKernelGGL(k_adj_state_grads, dim3(d.nvc), dim3(kBlock), 0, stream, d, s.dt, gqd_out, gq_in, gqd_in,
                      gf_in);
   if (any_pinned) {
@@ -1632,11 +1637,11 @@ void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const do
   }
   if (gmat) {
     if (d.Q == 1)
-      hipLaunchKernelGGL(k_adj_material_tet4, dim3(d.nec), dim3(kBlock), 0, stream, d, x);
+      hipLaunchKernelGGL(k_adj_material_tet4, dim3(d.nec), dim3(kBlock), 0, stream, d, x, z);
     else
-      hipLaunchKernelGGL(k_adj_material_hex8, dim3(d.nec), dim3(kBlock), 0, stream, d, x);
+      hipLaunchKernelGGL(k_adj_material_hex8, dim3(d.nec), dim3(kBlock), 0, stream, d, x, z);
   }
-  if (gact) hipLaunchKernelGGL(k_adj_activation_tet4, dim3(d.nec), dim3(kBlock), 0, stream, d, x);
+  if (gact) hipLaunchKernelGGL(k_adj_activation_tet4, dim3(d.nec), dim3(kBlock), 0, stream, d, x, z);
   if (gmat || gact) {
     hipLaunchKernelGGL(k_adj_parameter_guard, dim3(d.B), dim3(kBlock), 0, stream, d, gmat != nullptr ? 1 : 0,
                        gact != nullptr ? 1 : 0);

@@ -103,6 +103,12 @@ struct dphd_solver {
   double fc_friction = 0.0, fc_restitution = 0.0, fc_radius = 0.0;
   int fc_split = 0, fc_left = 0;
   dphd::FrictionDev fc;
+  // Woodbury rows of the backward: the recorded step's contact state, and
+  // buffers of the projection
+  dphd::WoodburyDev wb;
+  double *rec_mult = nullptr, *rec_omega = nullptr, *rec_normal = nullptr;
+  double *wb_mu = nullptr, *wb_rhs = nullptr, *wb_z = nullptr;
+  int wb_capacity = 0;
   bool have_topology = false, have_particles = false, have_materials = false;
   // build products
   bool built = false;
@@ -116,17 +122,21 @@ struct dphd_solver {
   double* targets = nullptr;  // device copy of muscle targets [E]
   hipStream_t stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  hipGraphExec_t fwd_exec = nullptr, adj_exec = nullptr;
-  int fwd_key_iter = -1, adj_key_iter = -1;
-  double fwd_key_abs = -1, fwd_key_rel = -1, adj_key_abs = -1, adj_key_rel = -1;
-  int last_fwd_iters = 0, last_adj_iters = 0;
+  // Iteration graphs of the forward, the adjoint and the Woodbury columns,
+  // keyed on the tolerances they bake in.
+  hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+  int key_iter[3] = {-1, -1, -1};
+  double key_abs[3] = {-1, -1, -1}, key_rel[3] = {-1, -1, -1};
+  int last_iters[3] = {0, 0, 0};
 
   ~dphd_solver() { release(); }
 
   void release() {
-    if (fwd_exec) (void)hipGraphExecDestroy(fwd_exec);
-    if (adj_exec) (void)hipGraphExecDestroy(adj_exec);
-    fwd_exec = adj_exec = nullptr;
+    for (auto& e : exec) {
+      if (e) (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+    wb_capacity = 0;
     for (void* p : owned) (void)hipFree(p);
     owned.clear();
     if (any_host) (void)hipHostFree(any_host);
@@ -613,8 +623,10 @@ void build_solver(dphd_solver* s, double dt) {
   HIP_CHECK(hipDeviceSynchronize());
   s->dt = dt;
   s->built = true;
-  s->fwd_key_iter = s->adj_key_iter = -1;
-  s->last_fwd_iters = s->last_adj_iters = 0;
+  for (int m = 0; m < 3; ++m) {
+    s->key_iter[m] = -1;
+    s->last_iters[m] = 0;
+  }
   // Jacobian, element operator blocks and the assembled operator.
   const int64_t jac_bytes = static_cast<int64_t>(E) * Q * 81 * 8 + static_cast<int64_t>(E) * K * K * 9 * 8 +
                             static_cast<int64_t>(d.kblocks) * 9 * 8;
@@ -746,6 +758,35 @@ void build_friction(dphd_solver* s) {
     fc.faces = dupload(o, s->fc_faces.empty() ? std::vector<int>{0, 0, 0} : s->fc_faces);
     fc.surface = dzeros<double>(o, 3 * static_cast<size_t>(n));
   }
+  // Woodbury rows of the backward and the recorded contact state they read.
+  s->rec_mult = dzeros<double>(o, k1);
+  s->rec_omega = dzeros<double>(o, k1);
+  s->rec_normal = dzeros<double>(o, 3 * k1);
+  s->wb_mu = dzeros<double>(o, 3 * static_cast<size_t>(n));
+  s->wb_rhs = dzeros<double>(o, 3 * static_cast<size_t>(n));
+  s->wb_z = dzeros<double>(o, 3 * static_cast<size_t>(n));
+  WoodburyDev& w = s->wb;
+  w = WoodburyDev{};
+  w.K = K;
+  w.n = n;
+  w.tangents = s->fc_friction > 0.0 ? 1 : 0;
+  w.R = w.tangents ? 3 * K : K;
+  const size_t r1 = std::max(w.R, 1);
+  w.nodes = fc.nodes;
+  w.mult = s->rec_mult;
+  w.omega = s->rec_omega;
+  w.normal = s->rec_normal;
+  w.row_node = dzeros<int>(o, r1);
+  w.row_vec = dzeros<double>(o, 3 * r1);
+  w.row_valid = dzeros<int>(o, r1);
+  w.value = dzeros<double>(o, r1);
+  w.order = dzeros<int>(o, r1);
+  w.count = dzeros<int>(o, 1);
+  w.gram = dzeros<double>(o, r1 * r1);
+  w.coef = dzeros<double>(o, r1);
+  w.perm = dzeros<int>(o, r1);
+  w.work = dzeros<double>(o, r1);
+  s->wb_capacity = 0;
   if (K == 0) return;
   double* rhs = dzeros<double>(o, 3 * static_cast<size_t>(n));
   double* out = dzeros<double>(o, 3 * static_cast<size_t>(n));
@@ -764,18 +805,29 @@ StepScalars scalars(const dphd_solver* s) {
   return StepScalars{s->dt, s->cfg.abs_tol, s->cfg.rel_tol, s->cfg.max_iterations};
 }
 
-hipGraphExec_t capture(dphd_solver* s, bool adjoint) {
-  const StepScalars sc = scalars(s);
+// Loop modes: the forward step, the backbone adjoint, and the Woodbury
+// columns (adjoint iterations capped at min(max_iterations, 120) that do not
+// report non-convergence, adjoint.py:1054-1060, 1028-1052).
+enum LoopMode { kForward = 0, kAdjoint = 1, kWoodbury = 2 };
+
+int loop_limit(const dphd_solver* s, int mode) {
+  return mode == kWoodbury ? std::min(s->cfg.max_iterations, 120) : s->cfg.max_iterations;
+}
+
+hipGraphExec_t capture(dphd_solver* s, int mode) {
+  StepScalars sc = scalars(s);
+  sc.max_iterations = loop_limit(s, mode);
+  sc.report = mode == kWoodbury ? 0 : 1;
   const int C = chunk_size(s);
   hipGraph_t graph = nullptr;
   HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   for (int c = 0; c < C; ++c) {
-    if (adjoint)
+    if (mode != kForward)
       launch_adjoint_iteration(s->d, sc, s->stream);
     else
       launch_forward_iteration(s->d, sc, s->targets, s->has_friction ? &s->fc : nullptr, s->stream);
   }
-  launch_any_active(s->d, adjoint ? s->d.aactive : s->d.active, s->stream);
+  launch_any_active(s->d, mode != kForward ? s->d.aactive : s->d.active, s->stream);
   HIP_CHECK(hipStreamEndCapture(s->stream, &graph));
   hipGraphExec_t exec = nullptr;
   HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -785,21 +837,19 @@ hipGraphExec_t capture(dphd_solver* s, bool adjoint) {
 
 // Run the predicated loop to every body's convergence: launch enough chunks
 // to cover the previous solve's iterations back to back, then poll.
-int run_loop(dphd_solver* s, bool adjoint) {
-  hipGraphExec_t& exec = adjoint ? s->adj_exec : s->fwd_exec;
-  int& key_iter = adjoint ? s->adj_key_iter : s->fwd_key_iter;
-  double& key_abs = adjoint ? s->adj_key_abs : s->fwd_key_abs;
-  double& key_rel = adjoint ? s->adj_key_rel : s->fwd_key_rel;
-  if (!exec || key_iter != s->cfg.max_iterations || key_abs != s->cfg.abs_tol || key_rel != s->cfg.rel_tol) {
+int run_loop(dphd_solver* s, int mode) {
+  hipGraphExec_t& exec = s->exec[mode];
+  if (!exec || s->key_iter[mode] != s->cfg.max_iterations || s->key_abs[mode] != s->cfg.abs_tol ||
+      s->key_rel[mode] != s->cfg.rel_tol) {
     if (exec) HIP_CHECK(hipGraphExecDestroy(exec));
-    exec = capture(s, adjoint);
-    key_iter = s->cfg.max_iterations;
-    key_abs = s->cfg.abs_tol;
-    key_rel = s->cfg.rel_tol;
+    exec = capture(s, mode);
+    s->key_iter[mode] = s->cfg.max_iterations;
+    s->key_abs[mode] = s->cfg.abs_tol;
+    s->key_rel[mode] = s->cfg.rel_tol;
   }
   const int C = chunk_size(s);
-  const int max_chunks = (s->cfg.max_iterations + C - 1) / C;
-  const int last = adjoint ? s->last_adj_iters : s->last_fwd_iters;
+  const int max_chunks = (loop_limit(s, mode) + C - 1) / C;
+  const int last = s->last_iters[mode];
   // One chunk fewer than last time, so a step that converges faster wastes
   // at most one predicated-off chunk.
   int first = std::max(1, std::min(max_chunks, last / C - 1));
@@ -810,7 +860,37 @@ int run_loop(dphd_solver* s, bool adjoint) {
     HIP_CHECK(hipStreamSynchronize(s->stream));
     if (!*reinterpret_cast<volatile int*>(s->any_host)) break;
   }
+  s->last_iters[mode] = launched * C;
   return launched;
+}
+
+// Woodbury projection of the adjoint for the parameter gradients; returns z
+// (Adjoint._woodbury, adjoint.py:1272-1354). d.sol holds mu and d.arhs the
+// seed on entry and again on return.
+const double* woodbury(dphd_solver* s) {
+  DeviceState& d = s->d;
+  WoodburyDev& w = s->wb;
+  hipStream_t st = s->stream;
+  const size_t vb = sizeof(double) * 3 * static_cast<size_t>(s->N);
+  HIP_CHECK(hipMemcpyAsync(s->wb_mu, d.sol, vb, hipMemcpyDeviceToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(s->wb_rhs, d.arhs, vb, hipMemcpyDeviceToDevice, st));
+  launch_woodbury_rows(w, s->wb_mu, d.route, st);
+  int count = 0;
+  HIP_CHECK(hipMemcpyAsync(&count, w.count, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  if (count > s->wb_capacity) {
+    w.columns = dalloc<double>(s->owned, static_cast<size_t>(count) * 3 * s->N);
+    s->wb_capacity = count;
+  }
+  for (int k = 0; k < count; ++k) {
+    launch_woodbury_seed(w, k, d.arhs, d.sol, d.aiter, d.aactive, d.aa, st);
+    run_loop(s, kWoodbury);
+    HIP_CHECK(hipMemcpyAsync(w.columns + static_cast<size_t>(k) * 3 * s->N, d.sol, vb, hipMemcpyDeviceToDevice, st));
+  }
+  launch_woodbury_project(w, s->wb_mu, d.free, s->wb_z, d.status, st);
+  HIP_CHECK(hipMemcpyAsync(d.sol, s->wb_mu, vb, hipMemcpyDeviceToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(d.arhs, s->wb_rhs, vb, hipMemcpyDeviceToDevice, st));
+  return s->wb_z;
 }
 
 }  // namespace
@@ -1128,6 +1208,20 @@ int dphd_contact_state(dphd_solver* s, const double* set_multiplier, double* mul
   })
 }
 
+int dphd_contact_record(dphd_solver* s, const double* multiplier, const double* omega, const double* normal) {
+  DPHD_TRY({
+    if (!s || !s->built || !s->has_friction || !multiplier || !omega || !normal)
+      throw std::invalid_argument("dphd_contact_record: needs a built solver with a frictional boundary");
+    HIP_CHECK(hipSetDevice(s->device));
+    const size_t K = s->fc.K;
+    if (K) {
+      HIP_CHECK(hipMemcpy(s->rec_mult, multiplier, sizeof(double) * K, hipMemcpyDefault));
+      HIP_CHECK(hipMemcpy(s->rec_omega, omega, sizeof(double) * K, hipMemcpyDefault));
+      HIP_CHECK(hipMemcpy(s->rec_normal, normal, sizeof(double) * 3 * K, hipMemcpyDefault));
+    }
+  })
+}
+
 int dphd_mesh_contact_force(dphd_solver* s, const double* position, const double* velocity, double* force,
                             void* stream) {
   DPHD_TRY({
@@ -1218,9 +1312,18 @@ int dphd_step(dphd_solver* s, const float* q_in, const float* qd_in, const float
       dphd::launch_mesh_contact(s->mc, q_in, qd_in, nullptr, nullptr, s->d.qt, s->d.vbody, s->d.mass, s->dt,
                                 s->stream);
     if (s->has_friction) dphd::launch_contact_begin(s->fc, s->d.q0, s->dt, s->stream);
-    const int chunks = dphd::run_loop(s, false);
-    s->last_fwd_iters = chunks * dphd::chunk_size(s);
-    if (s->has_friction) dphd::launch_contact_finish(s->fc, s->d.q0, qd_in, s->dt, s->d.x, s->stream);
+    dphd::run_loop(s, dphd::kForward);
+    if (s->has_friction) {
+      dphd::launch_contact_finish(s->fc, s->d.q0, qd_in, s->dt, s->d.x, s->stream);
+      // The step's contact state, read by its backward unless replaced (StepRecord.contact).
+      const size_t K = s->fc.K;
+      if (K) {
+        HIP_CHECK(hipMemcpyAsync(s->rec_mult, s->fc.mult, sizeof(double) * K, hipMemcpyDeviceToDevice, s->stream));
+        HIP_CHECK(hipMemcpyAsync(s->rec_omega, s->fc.omega, sizeof(double) * K, hipMemcpyDeviceToDevice, s->stream));
+        HIP_CHECK(
+            hipMemcpyAsync(s->rec_normal, s->fc.normal, sizeof(double) * 3 * K, hipMemcpyDeviceToDevice, s->stream));
+      }
+    }
     dphd::launch_write_state(s->d, sc, q_out, qd_out, checkpoint, s->stream);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(s->ev_out, s->stream));
@@ -1253,9 +1356,9 @@ int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle
       s->d.jac = dphd::dalloc<double>(s->owned, static_cast<size_t>(s->E) * s->Q * 81);
       s->d.ke = dphd::dalloc<double>(s->owned, static_cast<size_t>(s->E) * s->K * s->K * 9);
       s->d.kval = dphd::dalloc<double>(s->owned, static_cast<size_t>(s->d.kblocks) * 9);
-      if (s->adj_exec) {
-        HIP_CHECK(hipGraphExecDestroy(s->adj_exec));
-        s->adj_exec = nullptr;
+      for (int m = dphd::kAdjoint; m <= dphd::kWoodbury; ++m) {
+        if (s->exec[m]) HIP_CHECK(hipGraphExecDestroy(s->exec[m]));
+        s->exec[m] = nullptr;
       }
     }
     hipStream_t user = static_cast<hipStream_t>(stream);
@@ -1265,10 +1368,13 @@ int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle
       HIP_CHECK(hipMemcpyAsync(s->targets, muscle_targets, sizeof(double) * s->E, hipMemcpyDeviceToDevice, s->stream));
     const dphd::StepScalars sc = dphd::scalars(s);
     dphd::launch_adjoint_setup(s->d, sc, checkpoint, s->targets, gq_out, gqd_out, s->stream);
-    const int chunks = dphd::run_loop(s, true);
-    s->last_adj_iters = chunks * dphd::chunk_size(s);
+    dphd::run_loop(s, dphd::kAdjoint);
+    dphd::launch_adjoint_route(s->d, s->stream);
+    // Only parameter gradients use z; state gradients use mu (adjoint.py:1145-1148).
+    const double* z = nullptr;
+    if ((gmaterials || gactivations) && s->has_friction && s->fc.K > 0) z = dphd::woodbury(s);
     dphd::launch_adjoint_outputs(s->d, sc, checkpoint, gqd_out, gq_in, gqd_in, gf_in, gmaterials, gactivations,
-                                 gdirichlet, s->any_pinned, s->stream);
+                                 gdirichlet, z, s->any_pinned, s->stream);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(s->ev_out, s->stream));
     HIP_CHECK(hipStreamWaitEvent(user, s->ev_out, 0));

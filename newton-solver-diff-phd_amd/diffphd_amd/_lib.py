@@ -82,6 +82,7 @@ SIGNATURES = {
     "dphd_set_frictional_boundary": ([_P, _I, _P, _I, _P, _D, _D, _I, _I, _I, _P], _I),
     "dphd_boundary_parameters": ([_P, _P, _P], _I),
     "dphd_contact_state": ([_P, _P, _P, _P, _P, _P], _I),
+    "dphd_contact_record": ([_P, _P, _P, _P], _I),
     "dphd_set_tolerances": ([_P, _I, _D, _D], _I),
     "dphd_build": ([_P, _D], _I),
     "dphd_get_info": ([_P, ctypes.POINTER(Info)], _I),

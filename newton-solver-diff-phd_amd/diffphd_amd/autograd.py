@@ -41,6 +41,8 @@ class StepFunction(torch.autograd.Function):
         ctx.has_targets = targets is not None
         ctx.activations_shape = None if activations is None else tuple(activations.shape)
         ctx.activations_dtype = None if activations is None else activations.dtype
+        # The step's contact state for the Woodbury rows of its backward (StepRecord.contact).
+        ctx.contact = solver.contact_state() if solver._friction is not None else None
         return q_out, qd_out
 
     @staticmethod
@@ -70,6 +72,7 @@ class StepFunction(torch.autograd.Function):
             gmat,
             gdir,
             gact,
+            ctx.contact,
         )
         gmaterials = None
         if want_materials:

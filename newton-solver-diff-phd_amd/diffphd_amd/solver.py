@@ -452,10 +452,6 @@ class SolverDiffPhD:
                 raise NotImplementedError("Hex8 material gradients need Tet4 materials on the model")
             if self._planar is not None:
                 raise NotImplementedError("planar_energies are forward only: the reference records no Hex8 adjoint")
-            if self._friction is not None and (
-                materials.requires_grad or (activations is not None and activations.requires_grad)
-            ):
-                raise NotImplementedError("parameter gradients with a frictional boundary need the Woodbury rows")
             from .autograd import StepFunction
 
             q_out, qd_out = StepFunction.apply(
@@ -505,7 +501,10 @@ class SolverDiffPhD:
         if self.iteration_log is not None:
             self.iteration_log.append(("forward", self.iterations.numpy()))
 
-    def _native_backward(self, checkpoint, targets, gq_out, gqd_out, gq, gqd, gf, gmat, gdir, gact=None):
+    def _native_backward(self, checkpoint, targets, gq_out, gqd_out, gq, gqd, gf, gmat, gdir, gact=None, contact=None):
+        if contact is not None:
+            mult, omega, normal = contact[:3]
+            check(lib().dphd_contact_record(self._h, ptr(mult), ptr(omega), ptr(normal)), "contact_record")
         check(
             lib().dphd_backward(
                 self._h, ptr(checkpoint), ptr(targets), ptr(gq_out), ptr(gqd_out), ptr(gq), ptr(gqd), ptr(gf),

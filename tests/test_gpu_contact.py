@@ -92,3 +92,47 @@ def test_moving_sphere_parameters_apply_between_steps(cuda):
         np.testing.assert_array_equal(out_a.particle_q.cpu().numpy(), out_b.particle_q.cpu().numpy())
         assert np.allclose(a.boundary_parameters.numpy(), params)
     a.check_status()
+
+
+@pytest.mark.parametrize("name", ["woodbury", "woodbury_friction"])
+def test_woodbury_projection_matches_dense_reference(cuda, name):
+    """Parameter gradients through the contact rows (tests/test_diff_phd_adjoint.py:339-416)."""
+    print(f"[case] woodbury {name}", flush=True)
+    from diffphd_amd import SolverDiffPhD
+
+    g = golden(f"contact_{name}")
+    vertices, tets, poses, materials, free = contact_grid()
+    model = tet_model(cuda, tets, poses, materials, free, vertices, (0.0, 0.0, -9.81), requires_grad=True)
+    solver = SolverDiffPhD(
+        model, density=1000.0, frictional_boundary=("planar", (0.0, 0.0, 1.0, 0.0)), frictional_vertices=g["nodes"],
+        friction=float(g["friction"]), muscle_directions=g["directions"], muscle_stiffness=4.0e4,
+        max_iterations=int(g["max_iterations"]), abs_tol=float(g["abs_tol"]), rel_tol=float(g["rel_tol"]),
+    )
+    control = model.control()
+    control.tet_activations = torch.as_tensor(g["contraction"], device=cuda).requires_grad_(True)
+    for t in range(3):
+        s_in, s_out = model.state(), model.state()
+        s_in.particle_q = torch.as_tensor(g["q"][t], device=cuda)
+        s_in.particle_qd = torch.as_tensor(g["qd"][t], device=cuda)
+        if t == 2:
+            for name_ in ("particle_q", "particle_qd", "particle_f"):
+                getattr(s_in, name_).requires_grad_(True)
+            solver.step(s_in, s_out, control, None, 0.004)
+        else:
+            with torch.no_grad():
+                solver.step(s_in, s_out, control, None, 0.004)
+        assert solver.read_status() == 0
+    torch.autograd.backward(
+        [s_out.particle_q, s_out.particle_qd],
+        [torch.as_tensor(g["gq"], device=cuda), torch.as_tensor(g["gqd"], device=cuda)],
+    )
+    assert solver.read_status() & 8 == 0
+    assert np.count_nonzero(solver.contact_multipliers.cpu().numpy()) > 0
+    pairs = (
+        ("q", s_in.particle_q.grad), ("qd", s_in.particle_qd.grad), ("f", s_in.particle_f.grad),
+        ("materials", model.tet_materials.grad[:, :2]), ("activations", control.tet_activations.grad),
+    )
+    for key, got in pairs:
+        ref = g[f"out_{key}"]
+        scale = max(np.abs(ref).max(), 1e-30)
+        np.testing.assert_allclose(got.cpu().numpy(), ref, rtol=0, atol=1e-5 * scale, err_msg=key)
