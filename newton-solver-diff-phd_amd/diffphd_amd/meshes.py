@@ -109,3 +109,35 @@ def lame(young, poisson):
     mu = young / (2.0 * (1.0 + poisson))
     lam = young * poisson / ((1.0 + poisson) * (1.0 - 2.0 * poisson))
     return mu, lam
+
+
+def icosphere(radius, subdivisions, center=(0.0, 0.0, 0.0)):
+    """Triangulated sphere: an icosahedron with every face split in four ``subdivisions`` times.
+
+    Returns vertices [V,3] float64 on the sphere and outward faces [F,3] int32.
+    """
+    t = (1.0 + 5.0**0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    points = [np.asarray(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    for _ in range(subdivisions):
+        midpoint = {}
+
+        def middle(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in midpoint:
+                m = points[a] + points[b]
+                points.append(m / np.linalg.norm(m))
+                midpoint[key] = len(points) - 1
+            return midpoint[key]
+
+        split = []
+        for a, b, c in faces:
+            ab, bc, ca = middle(a, b), middle(b, c), middle(c, a)
+            split += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = split
+    vertices = np.asarray(center, dtype=np.float64) + radius * np.asarray(points)
+    return vertices, np.asarray(faces, dtype=np.int32)

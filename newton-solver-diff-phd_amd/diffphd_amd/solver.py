@@ -448,8 +448,6 @@ class SolverDiffPhD:
             for t in (state_in.particle_q, state_in.particle_qd, f, materials, dirichlet_positions, activations)
         )
         if tracked:
-            if self._hex and materials.requires_grad:
-                raise NotImplementedError("Hex8 material gradients need Tet4 materials on the model")
             if self._planar is not None:
                 raise NotImplementedError("planar_energies are forward only: the reference records no Hex8 adjoint")
             from .autograd import StepFunction

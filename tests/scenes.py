@@ -163,3 +163,84 @@ def two_blocks(gap):
     free = np.ones(len(vertices), bool)
     free[split:] = lower_v[:, 2] > 0.0
     return vertices, tets, poses, materials, free, split, surface_faces(upper_t), surface_faces(lower_t + split)
+
+
+# ---------------------------------------------------------------------------
+# The benchmark configurations (their inputs as the metric defines them), at
+# full or reduced size. Hex8 x-cells i with L // 3 < i <= 2 (L // 3) are the
+# stiff middle (examples/_diff_phd_cantilever.py:92).
+
+
+def hex_beam(nx, ny, nz, cell, young_soft, young_stiff, poisson=0.45):
+    vertices, hexes = meshes.hex_grid(nx, ny, nz, cell)
+    xcell = np.arange(len(hexes)) // (ny * nz)
+    stiff = (xcell > nx // 3) & (xcell <= (nx // 3) * 2)
+    mu, lam = meshes.lame(np.where(stiff, young_stiff, young_soft), poisson)
+    return vertices, hexes, np.column_stack([mu, lam]), vertices[:, 0] != 0.0
+
+
+def config1(nx=20, ny=5, nz=5):
+    """Hex8 cantilever 20x5x5 of 0.01 m, axial load 80 t/25 N on the tip, dt 0.01, rel 1e-4, no gravity."""
+    vertices, hexes, materials, free = hex_beam(nx, ny, nz, 0.01, 1.0e6, 1.0e7)
+    tip = np.flatnonzero(vertices[:, 0] == vertices[:, 0].max())
+    return dict(vertices=vertices, hexes=hexes, materials=materials, free=free, tip=tip, dt=0.01, steps=25,
+                rel_tol=1.0e-4, abs_tol=1.0e-9, q0=vertices.astype(np.float32))
+
+
+def config1_force(scene, step):
+    f = np.zeros_like(scene["q0"])
+    f[scene["tip"], 0] = 80.0 * step / 25
+    return f
+
+
+def config3(nx=64, ny=16, nz=16):
+    """Hex8 twist release of 0.005 m cells: 30 degrees at the tip, dt 1e-3, rel 1e-6, no gravity."""
+    vertices, hexes, materials, free = hex_beam(nx, ny, nz, 0.005, 1.0e6, 1.0e7)
+    q0 = meshes.twisted_positions(vertices, np.pi / 6.0).astype(np.float32)
+    tip = np.flatnonzero(vertices[:, 0] == vertices[:, 0].max())
+    return dict(vertices=vertices, hexes=hexes, materials=materials, free=free, tip=tip, dt=1.0e-3, steps=100,
+                rel_tol=1.0e-6, abs_tol=1.0e-9, q0=q0)
+
+
+def tip_rotation(q, rest, tip):
+    """Signed rotation [rad] of the tip cross-section about x (tests/test_cantilever.py:109-116)."""
+    center = rest[tip, 1:].mean(axis=0)
+    r0 = rest[tip, 1:] - center
+    r = q[tip, 1:] - q[tip, 1:].mean(axis=0)
+    return np.arctan2(np.sum(r0[:, 0] * r[:, 1] - r0[:, 1] * r[:, 0]), np.sum(r0 * r))
+
+
+def config2(k=20, seed=1):
+    """Tet4 cube of k^3 cubes of 5 mm, bottom pinned, gravity, per-tet log-uniform E (seed 1), nu 0.4."""
+    vertices, tets, poses, materials, free = cube_body(k, seed=seed, poisson=0.4)
+    top = np.flatnonzero(vertices[:, 2] == vertices[:, 2].max())
+    targets = vertices[top].astype(np.float64) + np.random.default_rng(2).normal(0.0, 1.0e-3, (len(top), 3))
+    return dict(vertices=vertices, tets=tets, poses=poses, materials=materials, free=free, top=top,
+                targets=targets, dt=1.0e-3, steps=50, rel_tol=1.0e-6, abs_tol=1.0e-9)
+
+
+def config5(cells=(16, 16, 8), steps=200):
+    """Tet4 block 40x40x20 mm (2.5 mm cells), E 5e5, nu 0.45, density 1100, bottom pinned, pressed by a
+    rigid sphere of radius 5 mm moving 6 mm down on a linear path with a 3 mm lateral sweep over
+    ``steps`` steps of 1e-3 s, friction 0.5; the sphere is the reference's spherical frictional
+    boundary (contact Newton) with its centre moved between steps, the top-face vertices registered;
+    the solver's default tolerances (rel 1e-6, abs 1e-9). Loss: squared distance of the top-face vertices to seeded targets (N(rest, 1 mm), seed 2)."""
+    cx, cy, cz = cells
+    cell = 0.04 / cx
+    vertices, tets, poses = meshes.kuhn_tet_grid(cx, cy, cz, cell)
+    top = vertices[:, 2].max()
+    mu, lam = meshes.lame(np.full(len(tets), 5.0e5), 0.45)
+    materials = np.column_stack([mu, lam, np.zeros_like(mu)]).astype(np.float32)
+    free = vertices[:, 2] > vertices[:, 2].min()
+    surface = np.flatnonzero(vertices[:, 2] == top).astype(np.int32)
+    targets = vertices[surface] + np.random.default_rng(2).normal(0.0, 1.0e-3, (len(surface), 3))
+    center0 = np.array([0.02, 0.02, top + 0.005])
+
+    def sphere(step):
+        s = min(step, steps) / steps
+        c = center0 + np.array([0.003 * s, 0.0, -0.006 * s])
+        return (float(c[0]), float(c[1]), float(c[2]), 0.005)
+
+    return dict(vertices=vertices.astype(np.float32), tets=tets.astype(np.int32), poses=poses, materials=materials,
+                free=free, surface=surface, targets=targets, sphere=sphere, dt=1.0e-3, steps=steps, density=1100.0,
+                friction=0.5, rel_tol=1.0e-6, abs_tol=1.0e-9)
