@@ -25,6 +25,7 @@
 namespace dphd {
 
 constexpr int kBlock = 256;
+constexpr int kWave = 64;  // wavefront width of gfx950
 constexpr int kMaxWindow = 8;
 constexpr int kGramEntries = kMaxWindow * (kMaxWindow + 1) / 2 + kMaxWindow;  // upper triangle + rhs
 
@@ -41,14 +42,15 @@ constexpr int kColsPerGroup = 256;
 // S product work item (one wavefront): rows out_row .. out_row + count - 1
 // (count <= kRowsPerGroup) of one block, whose values start at `offset`
 // with 4*quads values per row from column col0. Self-contained, so a
-// wavefront needs one uniform load per item.
+// wavefront needs one load per item.
 struct RowItem {
   long long offset;
   int col0;
   int quads;
   int out_row;
   int count;
-  int pad[2];
+  int body;
+  int pad;
 };
 
 // One row of one block as an S^T column group sees it: the row's values
@@ -63,7 +65,7 @@ struct ColEntry {
 };
 
 // S^T product work item (one workgroup): the 256 columns [c0, c0 + 256) of
-// one body (columns from `end` on belong to the next body), over the rows
+// body `body` (columns from `end` on belong to the next body), over the rows
 // entries[e0 .. e1) of the blocks that cover them (block by block, rows
 // ascending).
 struct ColGroup {
@@ -71,6 +73,8 @@ struct ColGroup {
   int e0;
   int e1;
   int end;
+  int body;
+  int pad[3];
 };
 
 struct Chunk {
@@ -109,12 +113,15 @@ struct DeviceState {
   int* gpost = nullptr;       // [N] original vertex -> slot
   int* gorig = nullptr;       // [slots] slot -> original vertex, -1 for a gap
   void* fval = nullptr;       // block values, double or float
+  // Work lists: wavefront w of the S product takes the items
+  // s_list[s_woff[w] .. s_woff[w + 1]), workgroup g of the S^T product the
+  // groups t_list[t_woff[g] .. t_woff[g + 1]) (cost-balanced on the host).
   RowItem* s_items = nullptr;
   int s_nitems = 0;
-  int* s_tb = nullptr;        // [B+1] S items of each body
+  int *s_woff = nullptr, *s_list = nullptr;
   ColGroup* t_items = nullptr;
   int t_nitems = 0;
-  int* t_tb = nullptr;        // [B+1] S^T items of each body
+  int *t_woff = nullptr, *t_list = nullptr;
   ColEntry* t_entries = nullptr;  // block rows of the column groups
   int g_s = 0, g_t = 0;       // grids of the two products
   bool f32 = false;           // FP32 factor values

@@ -29,7 +29,6 @@
 namespace dphd {
 namespace {
 
-constexpr int kWave = 64;
 constexpr int kRhsIndex = kMaxWindow * (kMaxWindow + 1) / 2;  // first right-hand-side entry of the Gram partials
 
 __device__ __forceinline__ V3 ld3(const double* p, int i) { return V3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
@@ -134,35 +133,6 @@ __device__ __forceinline__ void for_active(const int* active, const int* base, i
   for (int v = r.begin + r.first; v < r.end; v += r.stride) {
     const int b = find_body(pref, B, v);
     fn(base[b] + (v - pref[b]), b);
-  }
-}
-
-// Wavefront-granular work list: every wavefront takes its own items
-// (stride over all wavefronts of the grid), and fn must not use block
-// barriers. fn(item, next) also receives the wavefront's next item (or -1),
-// so that it can prefetch that item's descriptor.
-template <typename Fn>
-__device__ __forceinline__ void for_active_waves(const int* active, const int* base, int B, Fn fn) {
-  extern __shared__ int pref[];
-  const int total = active_prefix(active, base, B, pref);
-  const int workers = kBlock / kWave;
-  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-  const int stride = static_cast<int>(gridDim.x) * workers;
-  int v = static_cast<int>(blockIdx.x) * workers + wave;
-  if (v >= total) return;
-  int b = find_body(pref, B, v);
-  int item = base[b] + (v - pref[b]);
-  while (true) {
-    const int vn = v + stride;
-    int next = -1;
-    if (vn < total) {
-      while (pref[b + 1] <= vn) ++b;
-      next = base[b] + (vn - pref[b]);
-    }
-    fn(item, next);
-    if (next < 0) break;
-    v = vn;
-    item = next;
   }
 }
 
@@ -476,97 +446,131 @@ __device__ __forceinline__ void store_vec4(T* out, size_t row, V3 r) {
 
 static_assert(kColsPerGroup == 4 * kWave, "a lane owns four columns of a column group");
 
+// Lane `lane` of the calling wavefront's value v (a wavefront-uniform lane).
+__device__ __forceinline__ int lane_int(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ long long lane_i64(long long v, int lane) {
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v & 0xffffffffll), lane));
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane);
+  return (static_cast<long long>(hi) << 32) | lo;
+}
+
+// S product: wavefront w streams the items s_list[woff[w] .. woff[w + 1])
+// whose body is active. Lane l holds the descriptor of the l-th item of a
+// window of 64, so the descriptors of a window cost one load round trip.
 template <typename Real, typename Acc, typename In>
-__global__ __launch_bounds__(kBlock) void k_factor_rows(const RowItem* __restrict__ items, const Real* __restrict__ val,
+__global__ __launch_bounds__(kBlock) void k_factor_rows(const RowItem* __restrict__ items, const int* __restrict__ woff,
+                                                         const int* __restrict__ list, const Real* __restrict__ val,
                                                          const In* __restrict__ in, In* __restrict__ out,
-                                                         const int* active, const int* tb, int B) {
+                                                         const int* __restrict__ active) {
   const int lane = static_cast<int>(threadIdx.x) % kWave;
-  RowItem g{};
-  bool have = false;
-  for_active_waves(active, tb, B, [&](int item, int next) {
-    if (!have) g = items[item];
-    // Prefetch the next item's descriptor while this one streams.
-    RowItem gn{};
-    if (next >= 0) gn = items[next];
-    Acc acc[kRowsPerGroup][3];
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * (kBlock / kWave) +
+                                                  static_cast<int>(threadIdx.x) / kWave);
+  const int l0 = woff[wave], l1 = woff[wave + 1];
+  for (int w0 = l0; w0 < l1; w0 += kWave) {
+    RowItem mine{};
+    bool on = false;
+    if (w0 + lane < l1) {
+      mine = items[list[w0 + lane]];
+      on = active[mine.body] != 0;
+    }
+    unsigned long long pending = __ballot(on);
+    while (pending) {
+      const int j = __builtin_amdgcn_readfirstlane(static_cast<int>(__builtin_ctzll(pending)));
+      pending &= pending - 1;
+      const long long offset = lane_i64(mine.offset, j);
+      const int col0 = lane_int(mine.col0, j), quads = lane_int(mine.quads, j);
+      const int out_row = lane_int(mine.out_row, j), count = lane_int(mine.count, j);
+      Acc acc[kRowsPerGroup][3];
 #pragma unroll
-    for (int j = 0; j < kRowsPerGroup; ++j) acc[j][0] = acc[j][1] = acc[j][2] = Acc(0);
-    const size_t width = 4 * static_cast<size_t>(g.quads);
-    const Real* rows = val + g.offset;
-    for (int qi = lane; qi < g.quads; qi += kWave) {
-      // All loads of the pass first (rows past the item's count re-read its
-      // last row), then the products of the item's rows.
-      Acc x[4][3];
+      for (int r = 0; r < kRowsPerGroup; ++r) acc[r][0] = acc[r][1] = acc[r][2] = Acc(0);
+      const size_t width = 4 * static_cast<size_t>(quads);
+      const Real* rows = val + offset;
+      for (int qi = lane; qi < quads; qi += kWave) {
+        // All loads of the pass first (rows past the item's count re-read its
+        // last row), then the products of the item's rows.
+        Acc x[4][3];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) load_vec4(in, static_cast<size_t>(g.col0 + 4 * qi + u), x[u]);
-      Real v[kRowsPerGroup][4];
+        for (int u = 0; u < 4; ++u) load_vec4(in, static_cast<size_t>(col0 + 4 * qi + u), x[u]);
+        Real v[kRowsPerGroup][4];
 #pragma unroll
-      for (int j = 0; j < kRowsPerGroup; ++j) load_quad<false>(rows + width * min(j, g.count - 1) + 4 * qi, v[j]);
+        for (int r = 0; r < kRowsPerGroup; ++r) load_quad<false>(rows + width * min(r, count - 1) + 4 * qi, v[r]);
 #pragma unroll
-      for (int j = 0; j < kRowsPerGroup; ++j) {
-        if (j < g.count) {
+        for (int r = 0; r < kRowsPerGroup; ++r) {
+          if (r < count) {
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            acc[j][0] += static_cast<Acc>(v[j][u]) * x[u][0];
-            acc[j][1] += static_cast<Acc>(v[j][u]) * x[u][1];
-            acc[j][2] += static_cast<Acc>(v[j][u]) * x[u][2];
+            for (int u = 0; u < 4; ++u) {
+              acc[r][0] += static_cast<Acc>(v[r][u]) * x[u][0];
+              acc[r][1] += static_cast<Acc>(v[r][u]) * x[u][1];
+              acc[r][2] += static_cast<Acc>(v[r][u]) * x[u][2];
+            }
           }
         }
       }
-    }
-    // Butterfly over the lanes the block spans (all 64 once it is wide).
-    const int span = g.quads >= kWave ? kWave : g.quads;
+      // Butterfly over the lanes the block spans (all 64 once it is wide).
+      const int span = quads >= kWave ? kWave : quads;
 #pragma unroll
-    for (int j = 0; j < kRowsPerGroup; ++j) {
-      if (j < g.count) {
-        for (int o = 1; o < span; o <<= 1) {
-          acc[j][0] += __shfl_xor(acc[j][0], o, kWave);
-          acc[j][1] += __shfl_xor(acc[j][1], o, kWave);
-          acc[j][2] += __shfl_xor(acc[j][2], o, kWave);
+      for (int r = 0; r < kRowsPerGroup; ++r) {
+        if (r < count) {
+          for (int o = 1; o < span; o <<= 1) {
+            acc[r][0] += __shfl_xor(acc[r][0], o, kWave);
+            acc[r][1] += __shfl_xor(acc[r][1], o, kWave);
+            acc[r][2] += __shfl_xor(acc[r][2], o, kWave);
+          }
+          if (lane == 0)
+            store_vec4(out, static_cast<size_t>(out_row + r),
+                       V3{static_cast<double>(acc[r][0]), static_cast<double>(acc[r][1]), static_cast<double>(acc[r][2])});
         }
-        if (lane == 0)
-          store_vec4(out, static_cast<size_t>(g.out_row + j),
-                     V3{static_cast<double>(acc[j][0]), static_cast<double>(acc[j][1]), static_cast<double>(acc[j][2])});
       }
     }
-    g = gn;
-    have = true;
-  });
+  }
 }
 
+// S^T product: workgroup g takes the column groups t_list[woff[g] ..
+// woff[g + 1]) whose body is active; its wavefronts split a group's rows.
 template <typename Real, typename Acc, typename In>
-__global__ __launch_bounds__(kBlock) void k_factor_cols(const ColGroup* __restrict__ items,
+__global__ __launch_bounds__(kBlock) void k_factor_cols(const ColGroup* __restrict__ items, const int* __restrict__ woff,
+                                                         const int* __restrict__ list,
                                                          const ColEntry* __restrict__ entries,
                                                          const Real* __restrict__ val, const In* __restrict__ y,
                                                          double* __restrict__ out, const int* __restrict__ gorig,
-                                                         const double* base, const int* active, const int* tb, int B) {
-  __shared__ Acc part[kBlock / kWave - 1][12][kWave];
+                                                         const double* base, const int* __restrict__ active) {
+  constexpr int kWaves = kBlock / kWave, kBatch = 8;
+  static_assert(kBatch <= kWave, "a batch's entries are held one per lane");
+  __shared__ Acc part[kWaves - 1][12][kWave];
   const int lane = static_cast<int>(threadIdx.x) % kWave;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-  for_active(active, tb, B, [&](int item, int) {
-    const ColGroup g = items[item];
+  const int l0 = woff[blockIdx.x], l1 = woff[blockIdx.x + 1];
+  for (int li = l0; li < l1; ++li) {
+    const ColGroup g = items[list[li]];
+    if (!active[g.body]) continue;  // uniform over the workgroup
     const int c = g.c0 + 4 * lane;
     Acc acc[4][3];
 #pragma unroll
     for (int u = 0; u < 4; ++u) acc[u][0] = acc[u][1] = acc[u][2] = Acc(0);
     // Wavefront w takes the w-th quarter of the group's rows, in batches of
-    // kBatch rows (from any number of blocks) whose loads are all issued
-    // before their products; rows past the quarter re-read its last row and
-    // are dropped.
-    constexpr int kWaves = kBlock / kWave, kBatch = 8;
+    // kBatch rows (from any number of blocks). Lane k < kBatch holds the
+    // k-th entry of a batch, loaded while the batch before it streams, so a
+    // batch costs one round trip; rows past the quarter re-read its last row
+    // and are dropped.
     const int count = g.e1 - g.e0, share = (count + kWaves - 1) / kWaves;
     const int w0 = g.e0 + min(wave * share, count), w1 = g.e0 + min((wave + 1) * share, count);
+    const int slot = lane % kBatch;
+    ColEntry cur{};
+    if (w0 < w1) cur = entries[min(w0 + slot, w1 - 1)];
     for (int e = w0; e < w1; e += kBatch) {
+      ColEntry nxt = cur;
+      if (e + kBatch < w1) nxt = entries[min(e + kBatch + slot, w1 - 1)];
       Acc yr[kBatch][3];
       Real v[kBatch][4];
       bool on[kBatch];
 #pragma unroll
       for (int k = 0; k < kBatch; ++k) {
-        const ColEntry en = entries[min(e + k, w1 - 1)];
-        const int qi = (c - en.col0) >> 2;
-        on[k] = e + k < w1 && c >= en.col0 && qi < en.quads;
-        load_vec4(y, static_cast<size_t>(en.yrow), yr[k]);
-        load_quad<false>(val + en.rowoff + 4 * static_cast<size_t>(on[k] ? qi : 0), v[k]);
+        const long long rowoff = lane_i64(cur.rowoff, k);
+        const int col0 = lane_int(cur.col0, k), quads = lane_int(cur.quads, k), yrow = lane_int(cur.yrow, k);
+        const int qi = (c - col0) >> 2;
+        on[k] = e + k < w1 && c >= col0 && qi < quads;
+        load_vec4(y, static_cast<size_t>(yrow), yr[k]);
+        load_quad<false>(val + rowoff + 4 * static_cast<long long>(on[k] ? qi : 0), v[k]);
       }
 #pragma unroll
       for (int k = 0; k < kBatch; ++k) {
@@ -578,6 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_factor_cols(const ColGroup* __restri
           acc[u][2] += w * yr[k][2];
         }
       }
+      cur = nxt;
     }
     if (wave > 0) {
 #pragma unroll
@@ -586,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_factor_cols(const ColGroup* __restri
     __syncthreads();
     if (wave == 0) {
 #pragma unroll
-      for (int w = 0; w < kBlock / kWave - 1; ++w)
+      for (int w = 0; w < kWaves - 1; ++w)
 #pragma unroll
         for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] += part[w][t][lane];
 #pragma unroll
@@ -603,7 +608,7 @@ __global__ __launch_bounds__(kBlock) void k_factor_cols(const ColGroup* __restri
       }
     }
     __syncthreads();
-  });
+  }
 }
 
 // rhs [N*3] FP64 (original order) -> the 4-wide postorder right-hand side.
@@ -1549,11 +1554,11 @@ void spmv_pair(const DeviceState& d, double* out, const double* base, const int*
   const Real* val = static_cast<const Real*>(d.fval);
   In* y4 = static_cast<In*>(d.y4);
   if (d.s_nitems)
-    hipLaunchKernelGGL((k_factor_rows<Real, Acc, In>), dim3(d.g_s), dim3(kBlock), prefix_lds(d), stream, d.s_items, val,
-                       static_cast<const In*>(d.rhs4), y4, active, d.s_tb, d.B);
+    hipLaunchKernelGGL((k_factor_rows<Real, Acc, In>), dim3(d.g_s), dim3(kBlock), 0, stream, d.s_items, d.s_woff,
+                       d.s_list, val, static_cast<const In*>(d.rhs4), y4, active);
   if (d.t_nitems)
-    hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_t), dim3(kBlock), prefix_lds(d), stream, d.t_items,
-                       d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base, active, d.t_tb, d.B);
+    hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_t), dim3(kBlock), 0, stream, d.t_items, d.t_woff,
+                       d.t_list, d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base, active);
 }
 
 // Dispatch on the stored precision.

@@ -14,7 +14,10 @@
 // small subtree is stored whole as one block. The values are the factor's
 // own numbers, only placed differently.
 #include <algorithm>
+#include <functional>
+#include <queue>
 #include <stdexcept>
+#include <utility>
 
 #include "setup.h"
 
@@ -165,6 +168,67 @@ FactorLayout factor_layout(const Csr& a, const Factor& f) {
     k = k1;
   }
   return out;
+}
+
+// Longest-processing-time-first assignment (Graham's LPT rule): items by
+// decreasing cost, each to the currently least loaded of `bins`; ties go to
+// the lower bin and to the lower item.
+namespace {
+void lpt(const std::vector<int>& items, const std::vector<double>& cost, const std::vector<int>& bins,
+         std::vector<int>& owner) {
+  std::vector<int> order(items);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  using Load = std::pair<double, int>;
+  std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+  for (const int w : bins) heap.push(Load{0.0, w});
+  for (const int i : order) {
+    Load t = heap.top();
+    heap.pop();
+    owner[i] = t.second;
+    t.first += cost[i];
+    heap.push(t);
+  }
+}
+}  // namespace
+
+void balanced_lists(const std::vector<double>& cost, const std::vector<int>& body, int bodies, int workers,
+                    int per_block, std::vector<int>& off, std::vector<int>& list) {
+  const int n = static_cast<int>(cost.size());
+  if (static_cast<int>(body.size()) != n || workers < 1 || per_block < 1 || workers % per_block != 0)
+    throw std::invalid_argument("balanced_lists: inconsistent sizes");
+  const int blocks = workers / per_block;
+  int xcds = blocks % 8 == 0 ? 8 : 1;
+  // Bodies onto XCDs; when that leaves the XCDs more than 5% apart (few or
+  // very unequal bodies), every item may go to any worker instead.
+  std::vector<double> bcost(bodies, 0.0);
+  double total = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (body[i] < 0 || body[i] >= bodies) throw std::invalid_argument("balanced_lists: body out of range");
+    bcost[body[i]] += cost[i];
+    total += cost[i];
+  }
+  std::vector<int> all_bodies(bodies), xcd_bins(8), bxcd(bodies, 0);
+  for (int b = 0; b < bodies; ++b) all_bodies[b] = b;
+  for (int x = 0; x < 8; ++x) xcd_bins[x] = x;
+  if (xcds == 8) {
+    lpt(all_bodies, bcost, xcd_bins, bxcd);
+    std::vector<double> load(8, 0.0);
+    for (int b = 0; b < bodies; ++b) load[bxcd[b]] += bcost[b];
+    if (*std::max_element(load.begin(), load.end()) > 1.05 * total / 8) xcds = 1;
+  }
+  if (xcds == 1) std::fill(bxcd.begin(), bxcd.end(), 0);
+  // Each XCD's items onto its workers.
+  std::vector<std::vector<int>> xitems(xcds), xworkers(xcds);
+  for (int i = 0; i < n; ++i) xitems[bxcd[body[i]]].push_back(i);
+  for (int w = 0; w < workers; ++w) xworkers[(w / per_block) % xcds].push_back(w);
+  std::vector<int> owner(n, 0);
+  for (int x = 0; x < xcds; ++x) lpt(xitems[x], cost, xworkers[x], owner);
+  off.assign(workers + 1, 0);
+  for (int i = 0; i < n; ++i) ++off[owner[i] + 1];
+  for (int w = 0; w < workers; ++w) off[w + 1] += off[w];
+  list.assign(n, 0);
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int i = 0; i < n; ++i) list[fill[owner[i]]++] = i;  // ascending within a worker
 }
 
 }  // namespace dphd

@@ -66,6 +66,18 @@ struct FactorLayout {
 // Throws std::logic_error if a factor entry falls outside the structure.
 FactorLayout factor_layout(const Csr& a, const Factor& f);
 
+// Static work lists of the factor products. Item i (cost[i], of body[i])
+// goes to one of `workers` workers, `per_block` workers to a workgroup.
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup g runs on
+// XCD g % 8), so when the workgroup count is a multiple of 8 the bodies are
+// first spread over the XCDs (costliest first, each onto the least loaded),
+// and then each XCD's items over that XCD's workers the same way: a body's
+// items then share one L2. If the bodies do not split over the XCDs within
+// 5% of even, all items are spread over all workers instead. Worker w's
+// items, ascending, are list[off[w] .. off[w + 1]).
+void balanced_lists(const std::vector<double>& cost, const std::vector<int>& body, int bodies, int workers,
+                    int per_block, std::vector<int>& off, std::vector<int>& list);
+
 // Validates like DP/sparse.py:73-95 (finite, positive diagonal, exactly
 // symmetric) and drops exact zeros. Throws std::runtime_error.
 Csr validate_rows(const Csr& a);

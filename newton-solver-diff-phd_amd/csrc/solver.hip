@@ -349,7 +349,6 @@ void build_solver(dphd_solver* s, double dt) {
   std::vector<RowItem> s_items;
   std::vector<ColGroup> t_items;
   std::vector<ColEntry> t_entries;
-  std::vector<int> s_tb(B + 1, 0), t_tb(B + 1, 0);
   for (int b = 0; b < B; ++b) {
     const int v0 = s->voff[b], n = s->voff[b + 1] - v0, c0 = slot0[b];
     FactorLayout& L = layouts[b];
@@ -358,7 +357,6 @@ void build_solver(dphd_solver* s, double dt) {
       gorig[c0 + L.post[o]] = v0 + o;
     }
     const long long voff_b = static_cast<long long>(fval.size());
-    s_tb[b] = static_cast<int>(s_items.size());
     for (const FactorBlock& fb : L.blocks) {
       const long long width = 4 * static_cast<long long>(fb.quads);
       for (int r = 0; r < fb.rows; r += kRowsPerGroup) {
@@ -368,6 +366,7 @@ void build_solver(dphd_solver* s, double dt) {
         it.quads = fb.quads;
         it.out_row = c0 + fb.row0 + r;
         it.count = std::min(kRowsPerGroup, fb.rows - r);
+        it.body = b;
         s_items.push_back(it);
       }
     }
@@ -381,7 +380,6 @@ void build_solver(dphd_solver* s, double dt) {
       const int g0 = fb.col0 / kColsPerGroup, g1 = (fb.row0 + fb.rows - 1) / kColsPerGroup;
       for (int g = g0; g <= g1; ++g) cover[g].push_back(static_cast<int>(k));
     }
-    t_tb[b] = static_cast<int>(t_items.size());
     for (int g = 0; g < groups; ++g) {
       const int gc0 = g * kColsPerGroup;  // body-local first column of the group
       const int e0 = static_cast<int>(t_entries.size());
@@ -399,12 +397,10 @@ void build_solver(dphd_solver* s, double dt) {
         }
       }
       if (t_entries.size() > static_cast<size_t>(INT32_MAX)) throw std::invalid_argument("too many factor rows");
-      t_items.push_back(ColGroup{c0 + gc0, e0, static_cast<int>(t_entries.size()), c0 + L.slots});
+      t_items.push_back(ColGroup{c0 + gc0, e0, static_cast<int>(t_entries.size()), c0 + L.slots, b, {0, 0, 0}});
     }
     L = FactorLayout{};
   }
-  s_tb[B] = static_cast<int>(s_items.size());
-  t_tb[B] = static_cast<int>(t_items.size());
   std::vector<int> a_off, a_col;
   std::vector<double> a_val;
   const bool defect = s->cfg.factor_precision == 32;
@@ -533,13 +529,35 @@ void build_solver(dphd_solver* s, double dt) {
   std::vector<double>().swap(fval);
   d.s_items = dupload(o, s_items);
   d.s_nitems = static_cast<int>(s_items.size());
-  d.s_tb = dupload(o, s_tb);
   d.t_items = dupload(o, t_items);
   d.t_nitems = static_cast<int>(t_items.size());
-  d.t_tb = dupload(o, t_tb);
   d.t_entries = dupload(o, t_entries);
   d.g_s = grid_for((d.s_nitems + 3) / 4);
   d.g_t = grid_for(d.t_nitems);
+  {
+    // Cost of an item in memory round trips of its wavefront: an S item
+    // streams its rows in passes of 64 quads, a wavefront of an S^T group
+    // takes a quarter of the group's rows in batches of 8; plus the item's
+    // reduction and store.
+    std::vector<double> cost;
+    std::vector<int> body, off, list;
+    for (const RowItem& it : s_items) {
+      cost.push_back(0.25 + (it.quads + kWave - 1) / kWave);
+      body.push_back(it.body);
+    }
+    balanced_lists(cost, body, B, d.g_s * (kBlock / kWave), kBlock / kWave, off, list);
+    d.s_woff = dupload(o, off);
+    d.s_list = dupload(o, list);
+    cost.clear();
+    body.clear();
+    for (const ColGroup& g : t_items) {
+      cost.push_back(1.0 + (g.e1 - g.e0 + 4 * 8 - 1) / (4 * 8));
+      body.push_back(g.body);
+    }
+    balanced_lists(cost, body, B, d.g_t, 1, off, list);
+    d.t_woff = dupload(o, off);
+    d.t_list = dupload(o, list);
+  }
   if (defect) {
     d.a_off = dupload(o, a_off);
     d.a_col = dupload(o, a_col);
