@@ -38,6 +38,7 @@ constexpr int kGramEntries = kMaxWindow * (kMaxWindow + 1) / 2 + kMaxWindow;  //
 // S^T-product work item (one workgroup: 64 lanes x 4 columns).
 constexpr int kRowsPerGroup = 8;
 constexpr int kColsPerGroup = 256;
+constexpr int kColWaves = 4;  // wavefronts of an S^T workgroup
 
 // S product work item (one wavefront): rows out_row .. out_row + count - 1
 // (count <= kRowsPerGroup) of one block, whose values start at `offset`
@@ -229,6 +230,9 @@ void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const do
                             float* gq_in, float* gqd_in, float* gf_in, double* gmat, double* gact, float* gdir,
                             const double* z, bool any_pinned, hipStream_t stream);
 void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, hipStream_t stream);
+// Workgroups of the S and S^T product kernels the current device holds at
+// once (the kernel's occupancy per compute unit times the compute units).
+hipError_t product_capacity(bool f32, int* rows_blocks, int* cols_blocks);
 void launch_hex_determinants(const DeviceState& d, const float* q, double* out, hipStream_t stream);
 
 void launch_eval_projections(int count, const double* f, const double* dirs, double mu, double lame, double penalty,

@@ -454,6 +454,131 @@ __device__ __forceinline__ long long lane_i64(long long v, int lane) {
   return (static_cast<long long>(hi) << 32) | lo;
 }
 
+// Lane exchanges without LDS: the gfx950 row swaps (v_permlane32_swap
+// exchanges the upper half of its first operand with the lower half of its
+// second, v_permlane16_swap the odd 16-lane rows of the first with the even
+// rows of the second) and DPP moves within 16-lane rows (row_mirror: lane
+// l <- l ^ 15, row_half_mirror: l <- l ^ 7, quad_perm).
+constexpr int kDppRowMirror = 0x140, kDppHalfMirror = 0x141;
+constexpr int kDppXor2 = 0x4e;  // quad_perm [2, 3, 0, 1]
+constexpr int kDppXor1 = 0xb1;  // quad_perm [1, 0, 3, 2]
+
+template <int Ctrl>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), Ctrl, 0xf, 0xf, false));
+}
+template <int Ctrl>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b & 0xffffffffll), Ctrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), Ctrl, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+// a + b after swapping a's upper half-wavefront (Half = 32) or its odd
+// 16-lane rows (Half = 16) with the matching lanes of b: lanes whose Half bit
+// is clear get a(l) + a(l ^ Half), the others b(l ^ Half) + b(l).
+template <int Half>
+__device__ __forceinline__ float swap_sum(float a, float b) {
+  const unsigned ua = __builtin_bit_cast(unsigned, a), ub = __builtin_bit_cast(unsigned, b);
+  const auto r = Half == 32 ? __builtin_amdgcn_permlane32_swap(ua, ub, false, false)
+                            : __builtin_amdgcn_permlane16_swap(ua, ub, false, false);
+  return __builtin_bit_cast(float, static_cast<unsigned>(r[0])) + __builtin_bit_cast(float, static_cast<unsigned>(r[1]));
+}
+template <int Half>
+__device__ __forceinline__ double swap_sum(double a, double b) {
+  const unsigned long long ua = __builtin_bit_cast(unsigned long long, a), ub = __builtin_bit_cast(unsigned long long, b);
+  const unsigned alo = static_cast<unsigned>(ua), ahi = static_cast<unsigned>(ua >> 32);
+  const unsigned blo = static_cast<unsigned>(ub), bhi = static_cast<unsigned>(ub >> 32);
+  const auto lo = Half == 32 ? __builtin_amdgcn_permlane32_swap(alo, blo, false, false)
+                             : __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+  const auto hi = Half == 32 ? __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false)
+                             : __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+  const double na = __builtin_bit_cast(
+      double, (static_cast<unsigned long long>(static_cast<unsigned>(hi[0])) << 32) | static_cast<unsigned>(lo[0]));
+  const double nb = __builtin_bit_cast(
+      double, (static_cast<unsigned long long>(static_cast<unsigned>(hi[1])) << 32) | static_cast<unsigned>(lo[1]));
+  return na + nb;
+}
+
+// One halving step over a lane pairing l <-> l ^ mask (a DPP move): lanes
+// with `upper` set keep b, the others a, each plus the partner's copy.
+template <int Ctrl, typename T>
+__device__ __forceinline__ T mirror_sum(T a, T b, bool upper) {
+  const T keep = upper ? b : a, send = upper ? a : b;
+  return keep + dpp_move<Ctrl>(send);
+}
+
+// Sum over the 64 lanes of each of 32 values by recursive halving (every
+// step pairs lanes across one more bit: 32, 16, then l ^ 15, l ^ 7, l ^ 2
+// within rows, and l ^ 1): lanes 2k and 2k + 1 return the total of v[k].
+// 32 exchanges instead of 32 x 6 shuffles, none through LDS.
+template <typename T>
+__device__ __forceinline__ T wave_sum32(T (&v)[32], int lane) {
+  T a[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) a[i] = swap_sum<32>(v[i], v[16 + i]);  // value 16*b5 + i
+  T b[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) b[i] = swap_sum<16>(a[i], a[8 + i]);  // + 8*b4
+  const bool b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1, b1 = (lane >> 1) & 1;
+  T c[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) c[i] = mirror_sum<kDppRowMirror>(b[i], b[4 + i], b3);  // + 4*b3
+  T d[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) d[i] = mirror_sum<kDppHalfMirror>(c[i], c[2 + i], b2);  // + 2*b2
+  const T e = mirror_sum<kDppXor2>(d[0], d[1], b1);  // + b1
+  return e + dpp_move<kDppXor1>(e);
+}
+
+// One S item on one wavefront: ROWS rows; a lane takes one quad of every
+// row per pass, and all loads of a pass are issued before its products
+// (rows past the item's count re-read its last row and are dropped).
+template <int ROWS, typename Real, typename Acc, typename In>
+__device__ __forceinline__ void row_item(const Real* __restrict__ val, const In* __restrict__ in, In* __restrict__ out,
+                                         long long offset, int col0, int quads, int out_row, int count, int lane) {
+  Acc acc[ROWS][3];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) acc[r][0] = acc[r][1] = acc[r][2] = Acc(0);
+  const size_t width = 4 * static_cast<size_t>(quads);
+  const Real* rows = val + offset;
+  for (int qi = lane; qi < quads; qi += kWave) {
+    Acc x[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) load_vec4(in, static_cast<size_t>(col0 + 4 * qi + u), x[u]);
+    Real v[ROWS][4];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) load_quad<false>(rows + width * min(r, count - 1) + 4 * qi, v[r]);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      if (r < count) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          acc[r][0] += static_cast<Acc>(v[r][u]) * x[u][0];
+          acc[r][1] += static_cast<Acc>(v[r][u]) * x[u][1];
+          acc[r][2] += static_cast<Acc>(v[r][u]) * x[u][2];
+        }
+      }
+    }
+  }
+  // Row r's component c is value 4r + c (c = 3 is the zero fourth lane of
+  // the 4-wide output); after the wavefront sum, lanes 2k and 2k + 1 hold
+  // value k.
+  static_assert(ROWS * 4 == 32, "the wavefront sum reduces 32 values");
+  Acc vals[32];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    vals[4 * r + 0] = acc[r][0];
+    vals[4 * r + 1] = acc[r][1];
+    vals[4 * r + 2] = acc[r][2];
+    vals[4 * r + 3] = Acc(0);
+  }
+  const Acc total = wave_sum32(vals, lane);
+  const int k = lane >> 1;
+  if ((lane & 1) == 0 && (k >> 2) < count) out[4 * static_cast<size_t>(out_row) + k] = static_cast<In>(total);
+}
+
 // S product: wavefront w streams the items s_list[woff[w] .. woff[w + 1])
 // whose body is active. Lane l holds the descriptor of the l-th item of a
 // window of 64, so the descriptors of a window cost one load round trip.
@@ -480,134 +605,107 @@ __global__ __launch_bounds__(kBlock) void k_factor_rows(const RowItem* __restric
       const long long offset = lane_i64(mine.offset, j);
       const int col0 = lane_int(mine.col0, j), quads = lane_int(mine.quads, j);
       const int out_row = lane_int(mine.out_row, j), count = lane_int(mine.count, j);
-      Acc acc[kRowsPerGroup][3];
-#pragma unroll
-      for (int r = 0; r < kRowsPerGroup; ++r) acc[r][0] = acc[r][1] = acc[r][2] = Acc(0);
-      const size_t width = 4 * static_cast<size_t>(quads);
-      const Real* rows = val + offset;
-      for (int qi = lane; qi < quads; qi += kWave) {
-        // All loads of the pass first (rows past the item's count re-read its
-        // last row), then the products of the item's rows.
-        Acc x[4][3];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) load_vec4(in, static_cast<size_t>(col0 + 4 * qi + u), x[u]);
-        Real v[kRowsPerGroup][4];
-#pragma unroll
-        for (int r = 0; r < kRowsPerGroup; ++r) load_quad<false>(rows + width * min(r, count - 1) + 4 * qi, v[r]);
-#pragma unroll
-        for (int r = 0; r < kRowsPerGroup; ++r) {
-          if (r < count) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              acc[r][0] += static_cast<Acc>(v[r][u]) * x[u][0];
-              acc[r][1] += static_cast<Acc>(v[r][u]) * x[u][1];
-              acc[r][2] += static_cast<Acc>(v[r][u]) * x[u][2];
-            }
-          }
-        }
-      }
-      // Butterfly over the lanes the block spans (all 64 once it is wide).
-      const int span = quads >= kWave ? kWave : quads;
-#pragma unroll
-      for (int r = 0; r < kRowsPerGroup; ++r) {
-        if (r < count) {
-          for (int o = 1; o < span; o <<= 1) {
-            acc[r][0] += __shfl_xor(acc[r][0], o, kWave);
-            acc[r][1] += __shfl_xor(acc[r][1], o, kWave);
-            acc[r][2] += __shfl_xor(acc[r][2], o, kWave);
-          }
-          if (lane == 0)
-            store_vec4(out, static_cast<size_t>(out_row + r),
-                       V3{static_cast<double>(acc[r][0]), static_cast<double>(acc[r][1]), static_cast<double>(acc[r][2])});
-        }
-      }
+      row_item<kRowsPerGroup, Real, Acc>(val, in, out, offset, col0, quads, out_row, count, lane);
     }
   }
 }
 
 // S^T product: workgroup g takes the column groups t_list[woff[g] ..
-// woff[g + 1]) whose body is active; its wavefronts split a group's rows.
+// woff[g + 1]) whose body is active (lane l of every wavefront holds the
+// l-th group of a window of 64); its kColWaves wavefronts split a group's
+// rows.
 template <typename Real, typename Acc, typename In>
-__global__ __launch_bounds__(kBlock) void k_factor_cols(const ColGroup* __restrict__ items, const int* __restrict__ woff,
-                                                         const int* __restrict__ list,
-                                                         const ColEntry* __restrict__ entries,
-                                                         const Real* __restrict__ val, const In* __restrict__ y,
-                                                         double* __restrict__ out, const int* __restrict__ gorig,
-                                                         const double* base, const int* __restrict__ active) {
-  constexpr int kWaves = kBlock / kWave, kBatch = 8;
+__global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
+    const ColGroup* __restrict__ items, const int* __restrict__ woff, const int* __restrict__ list,
+    const ColEntry* __restrict__ entries, const Real* __restrict__ val, const In* __restrict__ y,
+    double* __restrict__ out, const int* __restrict__ gorig, const double* base, const int* __restrict__ active) {
+  constexpr int kWaves = kColWaves, kBatch = 8;
   static_assert(kBatch <= kWave, "a batch's entries are held one per lane");
   __shared__ Acc part[kWaves - 1][12][kWave];
   const int lane = static_cast<int>(threadIdx.x) % kWave;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
   const int l0 = woff[blockIdx.x], l1 = woff[blockIdx.x + 1];
-  for (int li = l0; li < l1; ++li) {
-    const ColGroup g = items[list[li]];
-    if (!active[g.body]) continue;  // uniform over the workgroup
-    const int c = g.c0 + 4 * lane;
-    Acc acc[4][3];
+  for (int g0 = l0; g0 < l1; g0 += kWave) {
+    ColGroup mine{};
+    bool on = false;
+    if (g0 + lane < l1) {
+      mine = items[list[g0 + lane]];
+      on = active[mine.body] != 0;
+    }
+    // The same in every wavefront of the workgroup, so the barriers below
+    // are reached by all of them.
+    unsigned long long pending = __ballot(on);
+    while (pending) {
+      const int j = __builtin_amdgcn_readfirstlane(static_cast<int>(__builtin_ctzll(pending)));
+      pending &= pending - 1;
+      const int gc0 = lane_int(mine.c0, j), ge0 = lane_int(mine.e0, j), ge1 = lane_int(mine.e1, j);
+      const int gend = lane_int(mine.end, j);
+      const int c = gc0 + 4 * lane;
+      Acc acc[4][3];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc[u][0] = acc[u][1] = acc[u][2] = Acc(0);
-    // Wavefront w takes the w-th quarter of the group's rows, in batches of
-    // kBatch rows (from any number of blocks). Lane k < kBatch holds the
-    // k-th entry of a batch, loaded while the batch before it streams, so a
-    // batch costs one round trip; rows past the quarter re-read its last row
-    // and are dropped.
-    const int count = g.e1 - g.e0, share = (count + kWaves - 1) / kWaves;
-    const int w0 = g.e0 + min(wave * share, count), w1 = g.e0 + min((wave + 1) * share, count);
-    const int slot = lane % kBatch;
-    ColEntry cur{};
-    if (w0 < w1) cur = entries[min(w0 + slot, w1 - 1)];
-    for (int e = w0; e < w1; e += kBatch) {
-      ColEntry nxt = cur;
-      if (e + kBatch < w1) nxt = entries[min(e + kBatch + slot, w1 - 1)];
-      Acc yr[kBatch][3];
-      Real v[kBatch][4];
-      bool on[kBatch];
+      for (int u = 0; u < 4; ++u) acc[u][0] = acc[u][1] = acc[u][2] = Acc(0);
+      // Wavefront w takes the w-th share of the group's rows, in batches of
+      // kBatch rows (from any number of blocks). Lane k < kBatch holds the
+      // k-th entry of a batch, loaded while the batch before it streams, so
+      // a batch costs one round trip; rows past the share re-read its last
+      // row and are dropped.
+      const int count = ge1 - ge0, share = (count + kWaves - 1) / kWaves;
+      const int w0 = ge0 + min(wave * share, count), w1 = ge0 + min((wave + 1) * share, count);
+      const int slot = lane % kBatch;
+      ColEntry cur{};
+      if (w0 < w1) cur = entries[min(w0 + slot, w1 - 1)];
+      for (int e = w0; e < w1; e += kBatch) {
+        ColEntry nxt = cur;
+        if (e + kBatch < w1) nxt = entries[min(e + kBatch + slot, w1 - 1)];
+        Acc yr[kBatch][3];
+        Real v[kBatch][4];
+        bool use[kBatch];
 #pragma unroll
-      for (int k = 0; k < kBatch; ++k) {
-        const long long rowoff = lane_i64(cur.rowoff, k);
-        const int col0 = lane_int(cur.col0, k), quads = lane_int(cur.quads, k), yrow = lane_int(cur.yrow, k);
-        const int qi = (c - col0) >> 2;
-        on[k] = e + k < w1 && c >= col0 && qi < quads;
-        load_vec4(y, static_cast<size_t>(yrow), yr[k]);
-        load_quad<false>(val + rowoff + 4 * static_cast<long long>(on[k] ? qi : 0), v[k]);
+        for (int k = 0; k < kBatch; ++k) {
+          const long long rowoff = lane_i64(cur.rowoff, k);
+          const int col0 = lane_int(cur.col0, k), quads = lane_int(cur.quads, k), yrow = lane_int(cur.yrow, k);
+          const int qi = (c - col0) >> 2;
+          use[k] = e + k < w1 && c >= col0 && qi < quads;
+          load_vec4(y, static_cast<size_t>(yrow), yr[k]);
+          load_quad<false>(val + rowoff + 4 * static_cast<long long>(use[k] ? qi : 0), v[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const Acc w = use[k] ? static_cast<Acc>(v[k][u]) : Acc(0);
+            acc[u][0] += w * yr[k][0];
+            acc[u][1] += w * yr[k][1];
+            acc[u][2] += w * yr[k][2];
+          }
+        }
+        cur = nxt;
       }
+      if (wave > 0) {
 #pragma unroll
-      for (int k = 0; k < kBatch; ++k) {
+        for (int t = 0; t < 12; ++t) part[wave - 1][t][lane] = acc[t / 3][t % 3];
+      }
+      __syncthreads();
+      if (wave == 0) {
+#pragma unroll
+        for (int w = 0; w < kWaves - 1; ++w)
+#pragma unroll
+          for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] += part[w][t][lane];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const Acc w = on[k] ? static_cast<Acc>(v[k][u]) : Acc(0);
-          acc[u][0] += w * yr[k][0];
-          acc[u][1] += w * yr[k][1];
-          acc[u][2] += w * yr[k][2];
-        }
-      }
-      cur = nxt;
-    }
-    if (wave > 0) {
-#pragma unroll
-      for (int t = 0; t < 12; ++t) part[wave - 1][t][lane] = acc[t / 3][t % 3];
-    }
-    __syncthreads();
-    if (wave == 0) {
-#pragma unroll
-      for (int w = 0; w < kWaves - 1; ++w)
-#pragma unroll
-        for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] += part[w][t][lane];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int colx = c + u;
-        if (colx < g.end) {
-          const int o = gorig[colx];
-          if (o >= 0) {
-            V3 r{static_cast<double>(acc[u][0]), static_cast<double>(acc[u][1]), static_cast<double>(acc[u][2])};
-            if (base) r = r + ld3(base, o);
-            st3(out, o, r);
+          const int colx = c + u;
+          if (colx < gend) {
+            const int o = gorig[colx];
+            if (o >= 0) {
+              V3 r{static_cast<double>(acc[u][0]), static_cast<double>(acc[u][1]), static_cast<double>(acc[u][2])};
+              if (base) r = r + ld3(base, o);
+              st3(out, o, r);
+            }
           }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
@@ -1557,7 +1655,7 @@ void spmv_pair(const DeviceState& d, double* out, const double* base, const int*
     hipLaunchKernelGGL((k_factor_rows<Real, Acc, In>), dim3(d.g_s), dim3(kBlock), 0, stream, d.s_items, d.s_woff,
                        d.s_list, val, static_cast<const In*>(d.rhs4), y4, active);
   if (d.t_nitems)
-    hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_t), dim3(kBlock), 0, stream, d.t_items, d.t_woff,
+    hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_t), dim3(kColWaves * kWave), 0, stream, d.t_items, d.t_woff,
                        d.t_list, d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base, active);
 }
 
@@ -1578,6 +1676,22 @@ void spmv_iteration(const DeviceState& d, const double* base, const int* active,
 
 // ---------------------------------------------------------------------------
 // Launchers.
+
+hipError_t product_capacity(bool f32, int* rows_blocks, int* cols_blocks) {
+  int dev = 0, cus = 0, nr = 0, nc = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = f32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nr, k_factor_rows<float, float, float>, kBlock, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nr, k_factor_rows<double, double, double>, kBlock, 0);
+  if (e == hipSuccess)
+    e = f32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, k_factor_cols<float, float, float>, kColWaves * kWave, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, k_factor_cols<double, double, double>,
+                                                           kColWaves * kWave, 0);
+  *rows_blocks = std::max(1, nr) * std::max(1, cus);
+  *cols_blocks = std::max(1, nc) * std::max(1, cus);
+  return e;
+}
 
 void launch_begin_step(const DeviceState& d, const StepScalars& s, const float* q, const float* qd, const float* f,
                        const float* dirichlet, hipStream_t stream) {

@@ -532,12 +532,18 @@ void build_solver(dphd_solver* s, double dt) {
   d.t_items = dupload(o, t_items);
   d.t_nitems = static_cast<int>(t_items.size());
   d.t_entries = dupload(o, t_entries);
-  d.g_s = grid_for((d.s_nitems + 3) / 4);
-  d.g_t = grid_for(d.t_nitems);
+  {
+    // Grids of the products: no more workgroups than the device holds at
+    // once, so that every wavefront's list runs concurrently.
+    int rows_cap = 0, cols_cap = 0;
+    HIP_CHECK(product_capacity(defect, &rows_cap, &cols_cap));
+    d.g_s = std::min(grid_for((d.s_nitems + 3) / 4), rows_cap >= 8 ? rows_cap & ~7 : rows_cap);
+    d.g_t = std::min(grid_for(d.t_nitems), cols_cap >= 8 ? cols_cap & ~7 : cols_cap);
+  }
   {
     // Cost of an item in memory round trips of its wavefront: an S item
     // streams its rows in passes of 64 quads, a wavefront of an S^T group
-    // takes a quarter of the group's rows in batches of 8; plus the item's
+    // takes its share of the group's rows in batches of 8; plus the item's
     // reduction and store.
     std::vector<double> cost;
     std::vector<int> body, off, list;
@@ -551,7 +557,7 @@ void build_solver(dphd_solver* s, double dt) {
     cost.clear();
     body.clear();
     for (const ColGroup& g : t_items) {
-      cost.push_back(1.0 + (g.e1 - g.e0 + 4 * 8 - 1) / (4 * 8));
+      cost.push_back(1.0 + (g.e1 - g.e0 + kColWaves * 8 - 1) / (kColWaves * 8));
       body.push_back(g.body);
     }
     balanced_lists(cost, body, B, d.g_t, 1, off, list);
