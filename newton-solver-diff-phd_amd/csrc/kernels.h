@@ -173,6 +173,9 @@ struct DeviceState {
   int* kcol = nullptr;       // [kblocks] column vertex, -1 = padding (rows end with padding)
   double* kdiag = nullptr;   // [kblocks] A_ij (defect mode only)
   double* ke = nullptr;      // [E*K*K*9] element blocks sum_s G_s^T J_s G_s
+  // Element blocks summed into operator block b: kmap_src[kmap_off[b] ..
+  // kmap_off[b + 1]) (indices into ke's blocks, in the order they are added).
+  int *kmap_off = nullptr, *kmap_src = nullptr;
   double* kval = nullptr;    // [kblocks*9]
   int kblocks = 0;
   int kmaxwidth = 0;         // the widest chunk's slots

@@ -38,22 +38,140 @@ __device__ __forceinline__ void st3(double* p, int i, V3 v) {
   p[3 * i + 2] = v.z;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
+// Lane exchanges without LDS: the gfx950 row swaps (v_permlane32_swap
+// exchanges the upper half of its first operand with the lower half of its
+// second, v_permlane16_swap the odd 16-lane rows of the first with the even
+// rows of the second) and DPP moves within 16-lane rows (row_mirror: lane
+// l <- l ^ 15, row_half_mirror: l <- l ^ 7, quad_perm).
+constexpr int kDppRowMirror = 0x140, kDppHalfMirror = 0x141;
+constexpr int kDppXor2 = 0x4e;  // quad_perm [2, 3, 0, 1]
+constexpr int kDppXor1 = 0xb1;  // quad_perm [1, 0, 3, 2]
+
+template <int Ctrl>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), Ctrl, 0xf, 0xf, false));
+}
+template <int Ctrl>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b & 0xffffffffll), Ctrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), Ctrl, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+// a + b after swapping a's upper half-wavefront (Half = 32) or its odd
+// 16-lane rows (Half = 16) with the matching lanes of b: lanes whose Half bit
+// is clear get a(l) + a(l ^ Half), the others b(l ^ Half) + b(l).
+template <int Half>
+__device__ __forceinline__ float swap_sum(float a, float b) {
+  const unsigned ua = __builtin_bit_cast(unsigned, a), ub = __builtin_bit_cast(unsigned, b);
+  const auto r = Half == 32 ? __builtin_amdgcn_permlane32_swap(ua, ub, false, false)
+                            : __builtin_amdgcn_permlane16_swap(ua, ub, false, false);
+  return __builtin_bit_cast(float, static_cast<unsigned>(r[0])) + __builtin_bit_cast(float, static_cast<unsigned>(r[1]));
+}
+template <int Half>
+__device__ __forceinline__ double swap_sum(double a, double b) {
+  const unsigned long long ua = __builtin_bit_cast(unsigned long long, a), ub = __builtin_bit_cast(unsigned long long, b);
+  const unsigned alo = static_cast<unsigned>(ua), ahi = static_cast<unsigned>(ua >> 32);
+  const unsigned blo = static_cast<unsigned>(ub), bhi = static_cast<unsigned>(ub >> 32);
+  const auto lo = Half == 32 ? __builtin_amdgcn_permlane32_swap(alo, blo, false, false)
+                             : __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+  const auto hi = Half == 32 ? __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false)
+                             : __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+  const double na = __builtin_bit_cast(
+      double, (static_cast<unsigned long long>(static_cast<unsigned>(hi[0])) << 32) | static_cast<unsigned>(lo[0]));
+  const double nb = __builtin_bit_cast(
+      double, (static_cast<unsigned long long>(static_cast<unsigned>(hi[1])) << 32) | static_cast<unsigned>(lo[1]));
+  return na + nb;
+}
+
+// One halving step over a lane pairing l <-> l ^ mask (a DPP move): lanes
+// with `upper` set keep b, the others a, each plus the partner's copy.
+template <int Ctrl, typename T>
+__device__ __forceinline__ T mirror_sum(T a, T b, bool upper) {
+  const T keep = upper ? b : a, send = upper ? a : b;
+  return keep + dpp_move<Ctrl>(send);
+}
+
+// Sum over the 64 lanes of each of 64 values: lane l returns the total of
+// v[l]; the same halving with one more step (l ^ 1) at the end.
+template <typename T>
+__device__ __forceinline__ T wave_sum64(T (&v)[64], int lane) {
+  T a[32];
 #pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
+  for (int i = 0; i < 32; ++i) a[i] = swap_sum<32>(v[i], v[32 + i]);  // value 32*b5 + i
+  T b[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) b[i] = swap_sum<16>(a[i], a[16 + i]);  // + 16*b4
+  const bool b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1, b1 = (lane >> 1) & 1, b0 = lane & 1;
+  T c[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) c[i] = mirror_sum<kDppRowMirror>(b[i], b[8 + i], b3);  // + 8*b3
+  T d[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) d[i] = mirror_sum<kDppHalfMirror>(c[i], c[4 + i], b2);  // + 4*b2
+  T e[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) e[i] = mirror_sum<kDppXor2>(d[i], d[2 + i], b1);  // + 2*b1
+  return mirror_sum<kDppXor1>(e[0], e[1], b0);  // + b0
+}
+
+// Sum of one value over the 64 lanes, in every lane.
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_move<kDppXor1>(v);
+  v += dpp_move<kDppXor2>(v);
+  v += dpp_move<kDppHalfMirror>(v);
+  v += dpp_move<kDppRowMirror>(v);
+  v = swap_sum<16>(v, v);
+  return swap_sum<32>(v, v);
+}
+
+// Sum over the 64 lanes of each of 32 values by recursive halving (every
+// step pairs lanes across one more bit: 32, 16, then l ^ 15, l ^ 7, l ^ 2
+// within rows, and l ^ 1): lanes 2k and 2k + 1 return the total of v[k].
+// 32 exchanges instead of 32 x 6 shuffles, none through LDS.
+template <typename T>
+__device__ __forceinline__ T wave_sum32(T (&v)[32], int lane) {
+  T a[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) a[i] = swap_sum<32>(v[i], v[16 + i]);  // value 16*b5 + i
+  T b[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) b[i] = swap_sum<16>(a[i], a[8 + i]);  // + 8*b4
+  const bool b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1, b1 = (lane >> 1) & 1;
+  T c[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) c[i] = mirror_sum<kDppRowMirror>(b[i], b[4 + i], b3);  // + 4*b3
+  T d[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) d[i] = mirror_sum<kDppHalfMirror>(c[i], c[2 + i], b2);  // + 2*b2
+  const T e = mirror_sum<kDppXor2>(d[0], d[1], b1);  // + b1
+  return e + dpp_move<kDppXor1>(e);
 }
 
 // Sum NV values over the 256-thread block into result[0..NV) (LDS), visible
-// to every thread after the final barrier.
+// to every thread after the final barrier. The wavefront sums run without
+// LDS; a wavefront's totals then go to smem[wave * NV + k].
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* smem, double* result) {
+  static_assert(NV <= 64, "at most one value per lane");
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  if constexpr (NV == 1) {
+    const double t = wave_sum(v[0]);
+    if (lane == 0) smem[wave] = t;
+  } else if constexpr (NV <= 32) {
+    double all[32];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
-  if (lane == 0)
+    for (int k = 0; k < 32; ++k) all[k] = k < NV ? v[k] : 0.0;
+    const double t = wave_sum32(all, lane);
+    if ((lane & 1) == 0 && (lane >> 1) < NV) smem[wave * NV + (lane >> 1)] = t;
+  } else {
+    double all[64];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) smem[wave * NV + k] = v[k];
+    for (int k = 0; k < 64; ++k) all[k] = k < NV ? v[k] : 0.0;
+    const double t = wave_sum64(all, lane);
+    if (lane < NV) smem[wave * NV + lane] = t;
+  }
   __syncthreads();
   for (int k = threadIdx.x; k < NV; k += blockDim.x) {
     double t = smem[k];
@@ -453,83 +571,11 @@ __device__ __forceinline__ long long lane_i64(long long v, int lane) {
   const int hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane);
   return (static_cast<long long>(hi) << 32) | lo;
 }
-
-// Lane exchanges without LDS: the gfx950 row swaps (v_permlane32_swap
-// exchanges the upper half of its first operand with the lower half of its
-// second, v_permlane16_swap the odd 16-lane rows of the first with the even
-// rows of the second) and DPP moves within 16-lane rows (row_mirror: lane
-// l <- l ^ 15, row_half_mirror: l <- l ^ 7, quad_perm).
-constexpr int kDppRowMirror = 0x140, kDppHalfMirror = 0x141;
-constexpr int kDppXor2 = 0x4e;  // quad_perm [2, 3, 0, 1]
-constexpr int kDppXor1 = 0xb1;  // quad_perm [1, 0, 3, 2]
-
-template <int Ctrl>
-__device__ __forceinline__ float dpp_move(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), Ctrl, 0xf, 0xf, false));
+__device__ __forceinline__ float lane_real(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
 }
-template <int Ctrl>
-__device__ __forceinline__ double dpp_move(double v) {
-  const long long b = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b & 0xffffffffll), Ctrl, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), Ctrl, 0xf, 0xf, false);
-  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
-}
-
-// a + b after swapping a's upper half-wavefront (Half = 32) or its odd
-// 16-lane rows (Half = 16) with the matching lanes of b: lanes whose Half bit
-// is clear get a(l) + a(l ^ Half), the others b(l ^ Half) + b(l).
-template <int Half>
-__device__ __forceinline__ float swap_sum(float a, float b) {
-  const unsigned ua = __builtin_bit_cast(unsigned, a), ub = __builtin_bit_cast(unsigned, b);
-  const auto r = Half == 32 ? __builtin_amdgcn_permlane32_swap(ua, ub, false, false)
-                            : __builtin_amdgcn_permlane16_swap(ua, ub, false, false);
-  return __builtin_bit_cast(float, static_cast<unsigned>(r[0])) + __builtin_bit_cast(float, static_cast<unsigned>(r[1]));
-}
-template <int Half>
-__device__ __forceinline__ double swap_sum(double a, double b) {
-  const unsigned long long ua = __builtin_bit_cast(unsigned long long, a), ub = __builtin_bit_cast(unsigned long long, b);
-  const unsigned alo = static_cast<unsigned>(ua), ahi = static_cast<unsigned>(ua >> 32);
-  const unsigned blo = static_cast<unsigned>(ub), bhi = static_cast<unsigned>(ub >> 32);
-  const auto lo = Half == 32 ? __builtin_amdgcn_permlane32_swap(alo, blo, false, false)
-                             : __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
-  const auto hi = Half == 32 ? __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false)
-                             : __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
-  const double na = __builtin_bit_cast(
-      double, (static_cast<unsigned long long>(static_cast<unsigned>(hi[0])) << 32) | static_cast<unsigned>(lo[0]));
-  const double nb = __builtin_bit_cast(
-      double, (static_cast<unsigned long long>(static_cast<unsigned>(hi[1])) << 32) | static_cast<unsigned>(lo[1]));
-  return na + nb;
-}
-
-// One halving step over a lane pairing l <-> l ^ mask (a DPP move): lanes
-// with `upper` set keep b, the others a, each plus the partner's copy.
-template <int Ctrl, typename T>
-__device__ __forceinline__ T mirror_sum(T a, T b, bool upper) {
-  const T keep = upper ? b : a, send = upper ? a : b;
-  return keep + dpp_move<Ctrl>(send);
-}
-
-// Sum over the 64 lanes of each of 32 values by recursive halving (every
-// step pairs lanes across one more bit: 32, 16, then l ^ 15, l ^ 7, l ^ 2
-// within rows, and l ^ 1): lanes 2k and 2k + 1 return the total of v[k].
-// 32 exchanges instead of 32 x 6 shuffles, none through LDS.
-template <typename T>
-__device__ __forceinline__ T wave_sum32(T (&v)[32], int lane) {
-  T a[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) a[i] = swap_sum<32>(v[i], v[16 + i]);  // value 16*b5 + i
-  T b[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) b[i] = swap_sum<16>(a[i], a[8 + i]);  // + 8*b4
-  const bool b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1, b1 = (lane >> 1) & 1;
-  T c[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) c[i] = mirror_sum<kDppRowMirror>(b[i], b[4 + i], b3);  // + 4*b3
-  T d[2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i) d[i] = mirror_sum<kDppHalfMirror>(c[i], c[2 + i], b2);  // + 2*b2
-  const T e = mirror_sum<kDppXor2>(d[0], d[1], b1);  // + b1
-  return e + dpp_move<kDppXor1>(e);
+__device__ __forceinline__ double lane_real(double v, int lane) {
+  return __builtin_bit_cast(double, lane_i64(__builtin_bit_cast(long long, v), lane));
 }
 
 // One S item on one wavefront: ROWS rows; a lane takes one quad of every
@@ -619,7 +665,8 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
     const ColGroup* __restrict__ items, const int* __restrict__ woff, const int* __restrict__ list,
     const ColEntry* __restrict__ entries, const Real* __restrict__ val, const In* __restrict__ y,
     double* __restrict__ out, const int* __restrict__ gorig, const double* base, const int* __restrict__ active) {
-  constexpr int kWaves = kColWaves, kBatch = 8;
+  // Rows per batch: 16 with FP32 values, 8 with FP64 (the same bytes).
+  constexpr int kWaves = kColWaves, kBatch = sizeof(Real) == 4 ? 16 : 8;
   static_assert(kBatch <= kWave, "a batch's entries are held one per lane");
   __shared__ Acc part[kWaves - 1][12][kWave];
   const int lane = static_cast<int>(threadIdx.x) % kWave;
@@ -657,17 +704,26 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
       for (int e = w0; e < w1; e += kBatch) {
         ColEntry nxt = cur;
         if (e + kBatch < w1) nxt = entries[min(e + kBatch + slot, w1 - 1)];
-        Acc yr[kBatch][3];
+        // Lane k's S-product row, read with the batch's values (a vector
+        // load, so it needs no scalar-cache round trip of its own).
+        Acc ymine[3];
+        load_vec4(y, static_cast<size_t>(cur.yrow), ymine);
         Real v[kBatch][4];
         bool use[kBatch];
 #pragma unroll
         for (int k = 0; k < kBatch; ++k) {
           const long long rowoff = lane_i64(cur.rowoff, k);
-          const int col0 = lane_int(cur.col0, k), quads = lane_int(cur.quads, k), yrow = lane_int(cur.yrow, k);
+          const int col0 = lane_int(cur.col0, k), quads = lane_int(cur.quads, k);
           const int qi = (c - col0) >> 2;
           use[k] = e + k < w1 && c >= col0 && qi < quads;
-          load_vec4(y, static_cast<size_t>(yrow), yr[k]);
           load_quad<false>(val + rowoff + 4 * static_cast<long long>(use[k] ? qi : 0), v[k]);
+        }
+        Acc yr[kBatch][3];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+          yr[k][0] = lane_real(ymine[0], k);
+          yr[k][1] = lane_real(ymine[1], k);
+          yr[k][2] = lane_real(ymine[2], k);
         }
 #pragma unroll
         for (int k = 0; k < kBatch; ++k) {
@@ -1136,7 +1192,8 @@ __global__ __launch_bounds__(kBlock) void k_element_operator(DeviceState d) {
 
 // Rows of the assembled operator: workgroup (chunk, slot), one thread per
 // row of the vertex chunk: block (i, j) = sum over the incident elements of
-// i, in incidence order, of K_e[k][l] with idx[e][k] = i and idx[e][l] = j;
+// i, in incidence order, of K_e[k][l] with idx[e][k] = i and idx[e][l] = j
+// (the element blocks listed for it by the host, four loads in flight);
 // minus A_ij I in the defect mode.
 __global__ __launch_bounds__(kBlock) void k_operator_assemble(DeviceState d) {
   const int ci = blockIdx.x, s = blockIdx.y;
@@ -1145,22 +1202,27 @@ __global__ __launch_bounds__(kBlock) void k_operator_assemble(DeviceState d) {
   const Chunk c = d.vchunks[ci];
   const int i = c.begin + threadIdx.x;
   const size_t base = static_cast<size_t>(d.kbase[ci]);
-  const int K = d.K;
   const size_t blk = base + static_cast<size_t>(s) * kBlock + threadIdx.x;
   double acc[9];
 #pragma unroll
   for (int t = 0; t < 9; ++t) acc[t] = 0.0;
   const int j = i < c.end ? d.kcol[blk] : -1;
   if (j >= 0) {
-    for (int a = d.inc_off[i]; a < d.inc_off[i + 1]; ++a) {
-      const int ent = d.inc_ent[a];
-      const int e = ent / K;
-      for (int l = 0; l < K; ++l) {
-        if (d.idx[e * K + l] != j) continue;
-        const double* kb = d.ke + (static_cast<size_t>(ent) * K + l) * 9;
+    const int p1 = d.kmap_off[blk + 1];
+    for (int p = d.kmap_off[blk]; p < p1; p += 4) {
+      double kb[4][9];
 #pragma unroll
-        for (int t = 0; t < 9; ++t) acc[t] += kb[t];
+      for (int u = 0; u < 4; ++u) {
+        const double* src = d.ke + static_cast<size_t>(d.kmap_src[min(p + u, p1 - 1)]) * 9;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) kb[u][t] = src[t];
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (p + u < p1) {
+#pragma unroll
+          for (int t = 0; t < 9; ++t) acc[t] += kb[u][t];
+        }
     }
     if (d.defect) {
       const double a = d.kdiag[blk];
@@ -1186,14 +1248,35 @@ __global__ __launch_bounds__(kBlock) void k_adj_rhs(DeviceState d) {
     const int* col = d.kcol + base + threadIdx.x;
     const double* val = d.kval + 9 * base + threadIdx.x;
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-    for (int s = 0; s < w; ++s) {
-      const int j = col[static_cast<size_t>(s) * kBlock];
-      if (j < 0) break;
-      const V3 m = ld3(d.sol, j);
-      const double* v = val + static_cast<size_t>(s) * 9 * kBlock;
-      t0 += v[0 * kBlock] * m.x + v[1 * kBlock] * m.y + v[2 * kBlock] * m.z;
-      t1 += v[3 * kBlock] * m.x + v[4 * kBlock] * m.y + v[5 * kBlock] * m.z;
-      t2 += v[6 * kBlock] * m.x + v[7 * kBlock] * m.y + v[8 * kBlock] * m.z;
+    // Slots in batches of 8 column loads, then 4 slots at a time with all
+    // their vertex and block loads issued before their products; padding
+    // (column -1, zero block) ends the row.
+    for (int s0 = 0; s0 < w; s0 += 8) {
+      int j[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) j[u] = s0 + u < w ? col[static_cast<size_t>(s0 + u) * kBlock] : -1;
+      if (j[0] < 0) break;
+#pragma unroll
+      for (int h = 0; h < 8; h += 4) {
+        V3 m[4];
+        double v[4][9];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int slot = min(s0 + h + u, w - 1);
+          m[u] = ld3(d.sol, j[h + u] >= 0 ? j[h + u] : i);
+          const double* vb = val + static_cast<size_t>(slot) * 9 * kBlock;
+#pragma unroll
+          for (int t = 0; t < 9; ++t) v[u][t] = vb[t * kBlock];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (j[h + u] >= 0) {
+            t0 += v[u][0] * m[u].x + v[u][1] * m[u].y + v[u][2] * m[u].z;
+            t1 += v[u][3] * m[u].x + v[u][4] * m[u].y + v[u][5] * m[u].z;
+            t2 += v[u][6] * m[u].x + v[u][7] * m[u].y + v[u][8] * m[u].z;
+          }
+        }
+      }
     }
     store_rhs(d, i, V3{t0, t1, t2} + ld3(d.arhs, i));
   });

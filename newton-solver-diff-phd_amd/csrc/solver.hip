@@ -443,6 +443,11 @@ void build_solver(dphd_solver* s, double dt) {
   std::vector<int> kwidth(nvc_host), kbase(nvc_host);
   std::vector<int> kcol;
   std::vector<double> kdiag;
+  // Per operator block (i, j), in block order: the element blocks K_e[k][l]
+  // (index (e*K + k)*K + l) with idx[e][k] = i and idx[e][l] = j, in the
+  // incidence order of i and then l ascending.
+  std::vector<int> kmap_off(1, 0), kmap_src;
+  if (static_cast<int64_t>(E) * K * K > INT32_MAX) throw std::invalid_argument("element blocks exceed int32 indexing");
   {
     std::vector<std::vector<int>> pattern(kBlock);
     int64_t blocks = 0;
@@ -467,6 +472,32 @@ void build_solver(dphd_solver* s, double dt) {
       blocks += static_cast<int64_t>(width) * kBlock;
       kcol.resize(blocks, -1);
       if (defect) kdiag.resize(blocks, 0.0);
+      // Element blocks of the chunk's operator blocks (local block index
+      // t * kBlock + lane), counted, then placed.
+      std::vector<int> local(static_cast<size_t>(width) * kBlock + 1, 0);
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int i = c.begin; i < c.end; ++i) {
+          const std::vector<int>& row = pattern[i - c.begin];
+          const size_t lane = static_cast<size_t>(i - c.begin);
+          for (int a = inc_off[i]; a < inc_off[i + 1]; ++a) {
+            const int ent = inc_ent[a], e = ent / K;
+            for (int l = 0; l < K; ++l) {
+              const int j = s->indices[static_cast<size_t>(e) * K + l];
+              const size_t t = static_cast<size_t>(std::lower_bound(row.begin(), row.end(), j) - row.begin());
+              const size_t lb = t * kBlock + lane;
+              if (pass == 0)
+                ++local[lb + 1];
+              else
+                kmap_src[kmap_off[kbase[ci] + lb] + local[lb]++] = ent * K + l;
+            }
+          }
+        }
+        if (pass == 0) {
+          for (size_t lb = 0; lb + 1 < local.size(); ++lb) kmap_off.push_back(kmap_off.back() + local[lb + 1]);
+          kmap_src.resize(kmap_off.back());
+          std::fill(local.begin(), local.end(), 0);
+        }
+      }
       for (int i = c.begin; i < c.end; ++i) {
         const std::vector<int>& row = pattern[i - c.begin];
         const size_t lane = static_cast<size_t>(i - c.begin);
@@ -543,8 +574,8 @@ void build_solver(dphd_solver* s, double dt) {
   {
     // Cost of an item in memory round trips of its wavefront: an S item
     // streams its rows in passes of 64 quads, a wavefront of an S^T group
-    // takes its share of the group's rows in batches of 8; plus the item's
-    // reduction and store.
+    // takes its share of the group's rows in batches of 16 (FP32 values) or
+    // 8 (FP64); plus the item's reduction and store.
     std::vector<double> cost;
     std::vector<int> body, off, list;
     for (const RowItem& it : s_items) {
@@ -557,7 +588,8 @@ void build_solver(dphd_solver* s, double dt) {
     cost.clear();
     body.clear();
     for (const ColGroup& g : t_items) {
-      cost.push_back(1.0 + (g.e1 - g.e0 + kColWaves * 8 - 1) / (kColWaves * 8));
+      const int batch = kColWaves * (defect ? 16 : 8);
+      cost.push_back(1.0 + (g.e1 - g.e0 + batch - 1) / batch);
       body.push_back(g.body);
     }
     balanced_lists(cost, body, B, d.g_t, 1, off, list);
@@ -619,6 +651,8 @@ void build_solver(dphd_solver* s, double dt) {
   d.kwidth = dupload(o, kwidth);
   d.kbase = dupload(o, kbase);
   d.kcol = dupload(o, kcol);
+  d.kmap_off = dupload(o, kmap_off);
+  d.kmap_src = dupload(o, kmap_src);
   if (defect) d.kdiag = dupload(o, kdiag);
   d.kblocks = static_cast<int>(kcol.size());
   d.kmaxwidth = std::max(1, *std::max_element(kwidth.begin(), kwidth.end()));
