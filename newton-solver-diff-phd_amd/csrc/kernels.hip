@@ -38,6 +38,21 @@ __device__ __forceinline__ void st3(double* p, int i, V3 v) {
   p[3 * i + 2] = v.z;
 }
 
+// parts[stride*c + off] summed over the chunks c0 <= c < c1 in chunk order,
+// with the loads issued eight at a time (one round trip per eight chunks).
+__device__ __forceinline__ double chunk_sum(const double* parts, int stride, int off, int c0, int c1) {
+  double t = 0.0;
+  for (int c = c0; c < c1; c += 8) {
+    double p[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) p[u] = parts[static_cast<size_t>(stride) * min(c + u, c1 - 1) + off];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (c + u < c1) t += p[u];
+  }
+  return t;
+}
+
 // Lane exchanges without LDS: the gfx950 row swaps (v_permlane32_swap
 // exchanges the upper half of its first operand with the lower half of its
 // second, v_permlane16_swap the odd 16-lane rows of the first with the even
@@ -702,12 +717,9 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
       ColEntry cur{};
       if (w0 < w1) cur = entries[min(w0 + slot, w1 - 1)];
       for (int e = w0; e < w1; e += kBatch) {
-        ColEntry nxt = cur;
-        if (e + kBatch < w1) nxt = entries[min(e + kBatch + slot, w1 - 1)];
-        // Lane k's S-product row, read with the batch's values (a vector
-        // load, so it needs no scalar-cache round trip of its own).
-        Acc ymine[3];
-        load_vec4(y, static_cast<size_t>(cur.yrow), ymine);
+        // The batch's values first, then lane k's S-product row (a vector
+        // load, no scalar-cache round trip of its own) and the next batch's
+        // entries: all in flight together.
         Real v[kBatch][4];
         bool use[kBatch];
 #pragma unroll
@@ -718,6 +730,10 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
           use[k] = e + k < w1 && c >= col0 && qi < quads;
           load_quad<false>(val + rowoff + 4 * static_cast<long long>(use[k] ? qi : 0), v[k]);
         }
+        Acc ymine[3];
+        load_vec4(y, static_cast<size_t>(cur.yrow), ymine);
+        // (After the last batch this re-reads the share's last entry, unused.)
+        const ColEntry nxt = entries[min(e + kBatch + slot, w1 - 1)];
         Acc yr[kBatch][3];
 #pragma unroll
         for (int k = 0; k < kBatch; ++k) {
@@ -747,17 +763,23 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
         for (int w = 0; w < kWaves - 1; ++w)
 #pragma unroll
           for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] += part[w][t][lane];
+        // A body's slots are a multiple of 4, so the lane's four columns
+        // are all the body's or none; their vertices in one load, then the
+        // base rows of all four together.
+        if (c < gend) {
+          const int4 o4 = *reinterpret_cast<const int4*>(gorig + c);
+          const int o[4] = {o4.x, o4.y, o4.z, o4.w};
+          V3 add[4] = {};
+          if (base) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int colx = c + u;
-          if (colx < gend) {
-            const int o = gorig[colx];
-            if (o >= 0) {
-              V3 r{static_cast<double>(acc[u][0]), static_cast<double>(acc[u][1]), static_cast<double>(acc[u][2])};
-              if (base) r = r + ld3(base, o);
-              st3(out, o, r);
-            }
+            for (int u = 0; u < 4; ++u) add[u] = ld3(base, o[u] >= 0 ? o[u] : 0);
           }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (o[u] >= 0)
+              st3(out, o[u],
+                  V3{static_cast<double>(acc[u][0]), static_cast<double>(acc[u][1]), static_cast<double>(acc[u][2])} +
+                      add[u]);
         }
       }
       __syncthreads();
@@ -854,11 +876,7 @@ __device__ void gram_solve(const DeviceState& d, int b, const int* iters, bool w
                            int* meta, int* status) {
   const int pb = parity_base(d, iters, b);
   const int c0 = d.vcb[b], c1 = d.vcb[b + 1];
-  for (int k = threadIdx.x; k < kGramEntries; k += blockDim.x) {
-    double t = 0.0;
-    for (int c = c0; c < c1; ++c) t += d.gram_part[kGramEntries * c + k];
-    red[k] = t;
-  }
+  for (int k = threadIdx.x; k < kGramEntries; k += blockDim.x) red[k] = chunk_sum(d.gram_part, kGramEntries, k, c0, c1);
   __syncthreads();
   if (threadIdx.x == 0) {
     const int W = d.W, cap = W + 1;
@@ -975,11 +993,7 @@ __global__ void k_conv_check(DeviceState d, StepScalars s) {
   if (!d.active[b]) return;
   __shared__ double red[4];
   const int c0 = d.vcb[b], c1 = d.vcb[b + 1];
-  if (threadIdx.x < 4) {
-    double t = 0.0;
-    for (int c = c0; c < c1; ++c) t += d.conv_part[4 * c + threadIdx.x];
-    red[threadIdx.x] = t;
-  }
+  if (threadIdx.x < 4) red[threadIdx.x] = chunk_sum(d.conv_part, 4, threadIdx.x, c0, c1);
   __syncthreads();
   if (threadIdx.x != 0) return;
   const int k = d.iter[b];
@@ -1316,9 +1330,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_clip_partial(DeviceState d) {
 // Every workgroup recomputes it in the same order; the body's first chunk
 // reports the safeguard.
 __device__ __forceinline__ double clip_scale(const DeviceState& d, const double* parts, int b, bool reporter) {
-  double t = 0.0;
-  for (int c = d.vcb[b]; c < d.vcb[b + 1]; ++c) t += parts[c];
-  const double len = sqrt(t);
+  const double len = sqrt(chunk_sum(parts, 1, 0, d.vcb[b], d.vcb[b + 1]));
   const double limit = 1.0e6;
   if (isfinite(len) && len > limit) {
     if (reporter && threadIdx.x == 0) atomicOr(d.status, 16);
@@ -1413,11 +1425,8 @@ __global__ void k_adj_check(DeviceState d, StepScalars s) {
   const int b = blockIdx.x;
   if (!d.aactive[b]) return;
   if (threadIdx.x != 0) return;
-  double t0 = 0.0, t1 = 0.0;
-  for (int c = d.vcb[b]; c < d.vcb[b + 1]; ++c) {
-    t0 += d.conv_part[4 * c];
-    t1 += d.conv_part[4 * c + 1];
-  }
+  const double t0 = chunk_sum(d.conv_part, 4, 0, d.vcb[b], d.vcb[b + 1]);
+  const double t1 = chunk_sum(d.conv_part, 4, 1, d.vcb[b], d.vcb[b + 1]);
   const int k = d.aiter[b];
   d.aiter[b] = k + 1;
   if (k > 0 && sqrt(t0) < s.rel_tol * sqrt(t1) + s.abs_tol) {
