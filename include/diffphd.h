@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPHD_ABI_VERSION 3
+#define DPHD_ABI_VERSION 4
 
 enum {
   DPHD_OK = 0,
@@ -169,6 +169,12 @@ int dphd_mesh_contact_force(dphd_solver* s, const double* position, const double
  * (SolverDiffPhD.mesh_contact_pairs, DP/solver.py:350-353); -1 without mesh
  * contact. Synchronizes. */
 int dphd_mesh_contact_pairs(dphd_solver* s, int* pairs);
+/* The same pairs by direction: counts[0] vertices of faces_a against faces_b,
+ * counts[1] vertices of faces_b against faces_a (the two per-step pair counts
+ * the reference's parity harness records from MeshContact.bary_ab / bary_ba,
+ * tools/parity/port_cases.py:444-447); -1 each without mesh contact.
+ * Synchronizes. */
+int dphd_mesh_contact_pair_counts(dphd_solver* s, int* counts);
 /* Mutable between steps (DP/solver.py:206-209). */
 int dphd_set_tolerances(dphd_solver* s, int max_iterations, double abs_tol, double rel_tol);
 /* Element weights, assembly and factor per body (DP/solver.py:491-552). */

@@ -47,7 +47,7 @@ struct MeshContactDev {
   const int* inc_b_off = nullptr;
   const int* inc_b = nullptr;
   double* force = nullptr;    // [n*3]
-  int* pairs = nullptr;       // pairs within the contact radius in the last evaluation
+  int* pairs = nullptr;       // [2] pairs within the contact radius in the last evaluation (A-B, B-A)
 };
 
 // Evaluate the force of positions/velocities given as float32 (state arrays,

@@ -136,7 +136,10 @@ __device__ __forceinline__ PairForce pair_force(const Vec3Src& pos, const Vec3Sr
   return out;
 }
 
-__global__ void k_mc_zero(int* pairs) { *pairs = 0; }
+__global__ void k_mc_zero(int* pairs) {
+  pairs[0] = 0;
+  pairs[1] = 0;
+}
 
 __global__ __launch_bounds__(256) void k_mc_rows(MeshContactSide side, Vec3Src pos, Vec3Src vel, double radius,
                                                  double kf, double mu, int* pairs) {
@@ -876,11 +879,12 @@ void launch_mesh_contact(const MeshContactDev& mc, const float* pos32, const flo
                          hipStream_t stream) {
   const Vec3Src pos{pos32, pos64}, vel{vel32, vel64};
   hipLaunchKernelGGL(k_mc_zero, dim3(1), dim3(1), 0, stream, mc.pairs);
-  for (const MeshContactSide* side : {&mc.ab, &mc.ba}) {
+  for (int d = 0; d < 2; ++d) {
+    const MeshContactSide* side = d == 0 ? &mc.ab : &mc.ba;
     const long long waves = static_cast<long long>(side->nv) * side->row_words;
     if (waves == 0) continue;
     hipLaunchKernelGGL(k_mc_rows, dim3(blocks_for(waves * 64)), dim3(256), 0, stream, *side, pos, vel, mc.radius,
-                       mc.kf, mc.mu, mc.pairs);
+                       mc.kf, mc.mu, mc.pairs + d);
     hipLaunchKernelGGL(k_mc_transpose, dim3(blocks_for(static_cast<long long>(side->nf) * side->col_words)),
                        dim3(256), 0, stream, *side);
   }

@@ -764,7 +764,7 @@ void build_mesh_contact(dphd_solver* s) {
   mc.inc_b_off = dupload(o, ib_off);
   mc.inc_b = dupload(o, ib);
   mc.force = dzeros<double>(o, 3 * static_cast<size_t>(n));
-  mc.pairs = dzeros<int>(o, 1);
+  mc.pairs = dzeros<int>(o, 2);
 }
 
 // Device state of the frictional boundary and its panel A^{-1} e_j of the
@@ -1301,7 +1301,20 @@ int dphd_mesh_contact_pairs(dphd_solver* s, int* pairs) {
     if (!s->built || !s->has_mesh_contact) return DPHD_OK;
     HIP_CHECK(hipSetDevice(s->device));
     HIP_CHECK(hipDeviceSynchronize());
-    HIP_CHECK(hipMemcpy(pairs, s->mc.pairs, sizeof(int), hipMemcpyDeviceToHost));
+    int counts[2] = {0, 0};
+    HIP_CHECK(hipMemcpy(counts, s->mc.pairs, sizeof(counts), hipMemcpyDeviceToHost));
+    *pairs = counts[0] + counts[1];
+  })
+}
+
+int dphd_mesh_contact_pair_counts(dphd_solver* s, int* counts) {
+  DPHD_TRY({
+    if (!s || !counts) throw std::invalid_argument("dphd_mesh_contact_pair_counts: invalid arguments");
+    counts[0] = counts[1] = -1;
+    if (!s->built || !s->has_mesh_contact) return DPHD_OK;
+    HIP_CHECK(hipSetDevice(s->device));
+    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipMemcpy(counts, s->mc.pairs, 2 * sizeof(int), hipMemcpyDeviceToHost));
   })
 }
 

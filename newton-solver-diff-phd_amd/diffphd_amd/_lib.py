@@ -79,6 +79,7 @@ SIGNATURES = {
     "dphd_set_mesh_contact": ([_P, _D, _D, _D, _D, _D, _I, _P, _I, _P], _I),
     "dphd_mesh_contact_force": ([_P, _P, _P, _P, _P], _I),
     "dphd_mesh_contact_pairs": ([_P, ctypes.POINTER(_I)], _I),
+    "dphd_mesh_contact_pair_counts": ([_P, _P], _I),
     "dphd_set_frictional_boundary": ([_P, _I, _P, _I, _P, _D, _D, _I, _I, _I, _P], _I),
     "dphd_boundary_parameters": ([_P, _P, _P], _I),
     "dphd_contact_state": ([_P, _P, _P, _P, _P, _P], _I),
@@ -101,7 +102,7 @@ SIGNATURES = {
 _lib = None
 
 
-ABI_VERSION = 3  # DPHD_ABI_VERSION of include/diffphd.h
+ABI_VERSION = 4  # DPHD_ABI_VERSION of include/diffphd.h
 
 
 def lib():

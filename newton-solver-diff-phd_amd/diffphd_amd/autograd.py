@@ -32,6 +32,8 @@ class StepFunction(torch.autograd.Function):
             qd_out,
             checkpoint,
         )
+        # The reference's solver._x holds the last step's converged iterate, taped or not.
+        solver._x.copy_(checkpoint)
         ctx.solver = solver
         ctx.version = solver._version
         ctx.has_dirichlet = dirichlet is not None

@@ -483,7 +483,7 @@ class SolverDiffPhD:
         return out
 
     def solution(self) -> torch.Tensor:
-        """FP64 converged iterate of the last untracked step (the reference's ``solver._x``)."""
+        """FP64 converged iterate of the last step (the reference's ``solver._x``)."""
         return self._x
 
     def _native_step(self, dt, q, qd, f, dirichlet, targets, q_out, qd_out, checkpoint):
@@ -602,6 +602,15 @@ class SolverDiffPhD:
         out = ctypes.c_int()
         check(lib().dphd_mesh_contact_pairs(self._h, ctypes.byref(out)))
         return torch.tensor([max(out.value, 0)], dtype=torch.int32)
+
+    @property
+    def mesh_contact_pair_counts(self):
+        """(faces_a vertices vs faces_b, faces_b vertices vs faces_a) pairs of the last step, or None."""
+        if self._mesh_contact is None:
+            return None
+        out = np.zeros(2, dtype=np.int32)
+        check(lib().dphd_mesh_contact_pair_counts(self._h, out.ctypes.data))
+        return tuple(max(int(c), 0) for c in out)
 
     def mesh_contact_force(self, position: torch.Tensor, velocity: torch.Tensor) -> torch.Tensor:
         """The mesh contact force [N] of float64 positions and velocities (MeshContact.evaluate)."""
