@@ -10,6 +10,8 @@
 // device arrays; the pairs within the contact radius are found by a hit
 // bitmask per direction, row-major [vertex][face word] (one 64-bit word per
 // 64 faces, built by wavefront ballots) and its transpose [face][vertex word].
+// Broad phase: each face word's bounding box (widened by the radius) is
+// built per evaluation, and a vertex outside it skips the word's 64 tests.
 // A particle's force then sums the forces of its own hit pairs in face order
 // and subtracts the reactions of the hit pairs on the faces it is a corner
 // of in vertex order; hit pairs are recomputed where they are summed, so no
@@ -32,6 +34,7 @@ struct MeshContactSide {
   uint64_t* cols = nullptr;   // [nf * col_words] hits by face
   int row_words = 0;          // ceil(nf / 64)
   int col_words = 0;          // ceil(nv / 64)
+  double* boxes = nullptr;    // [row_words * 6] per 64-face word: lower, upper corner (widened)
 };
 
 struct MeshContactDev {

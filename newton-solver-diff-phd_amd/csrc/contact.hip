@@ -141,18 +141,60 @@ __global__ void k_mc_zero(int* pairs) {
   pairs[1] = 0;
 }
 
+// Broad phase: the bounds of each word of 64 faces (faces in index order),
+// widened by the contact radius times 1 + 1e-6. A vertex outside a word's
+// box is farther than the radius from every face of the word in exact
+// arithmetic, and the margin covers the rounding of the closest-point
+// distance, so skipping the word drops no pair the exact test would hit.
+__global__ __launch_bounds__(256) void k_mc_boxes(MeshContactSide side, Vec3Src pos, double reach) {
+  const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
+  if (wave >= side.row_words) return;  // uniform per wavefront
+  const int f = static_cast<int>(wave) * 64 + lane;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  bool finite = true;
+  if (f < side.nf) {
+    for (int k = 0; k < 3; ++k) {
+      const V3 v = pos(side.faces[3 * f + k]);
+      const double c[3] = {v.x, v.y, v.z};
+      for (int d = 0; d < 3; ++d) {
+        finite = finite && isfinite(c[d]);
+        lo[d] = fmin(lo[d], c[d]);
+        hi[d] = fmax(hi[d], c[d]);
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+      hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+    }
+  // A non-finite corner leaves the word unbounded (never skipped), so the
+  // exact test sees it as before.
+  const bool all_finite = __ballot(!finite) == 0;
+  if (lane < 3) side.boxes[6 * wave + lane] = all_finite ? lo[lane] - reach : -INFINITY;
+  else if (lane < 6) side.boxes[6 * wave + lane] = all_finite ? hi[lane - 3] + reach : INFINITY;
+}
+
 __global__ __launch_bounds__(256) void k_mc_rows(MeshContactSide side, Vec3Src pos, Vec3Src vel, double radius,
                                                  double kf, double mu, int* pairs) {
   const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
   const int lane = threadIdx.x & 63;
   if (wave >= static_cast<long long>(side.nv) * side.row_words) return;  // uniform per wavefront
   const int s = static_cast<int>(wave / side.row_words), word = static_cast<int>(wave % side.row_words);
+  const size_t at = static_cast<size_t>(s) * side.row_words + word;
+  const V3 p = pos(side.verts[s]);
+  const double* box = side.boxes + 6 * word;
+  if (p.x < box[0] || p.y < box[1] || p.z < box[2] || p.x > box[3] || p.y > box[4] || p.z > box[5]) {
+    if (lane == 0) side.rows[at] = 0;
+    return;
+  }
   const int f = word * 64 + lane;
   bool hit = false;
   if (f < side.nf) hit = pair_force(pos, vel, side.verts[s], side.faces + 3 * f, radius, side.kn, kf, mu).hit;
   const uint64_t mask = __ballot(hit);
   if (lane == 0) {
-    side.rows[static_cast<size_t>(s) * side.row_words + word] = mask;
+    side.rows[at] = mask;
     if (mask) atomicAdd(pairs, __popcll(mask));
   }
 }
@@ -883,6 +925,8 @@ void launch_mesh_contact(const MeshContactDev& mc, const float* pos32, const flo
     const MeshContactSide* side = d == 0 ? &mc.ab : &mc.ba;
     const long long waves = static_cast<long long>(side->nv) * side->row_words;
     if (waves == 0) continue;
+    hipLaunchKernelGGL(k_mc_boxes, dim3(blocks_for(side->row_words * 64LL)), dim3(256), 0, stream, *side, pos,
+                       mc.radius * (1.0 + 1.0e-6));
     hipLaunchKernelGGL(k_mc_rows, dim3(blocks_for(waves * 64)), dim3(256), 0, stream, *side, pos, vel, mc.radius,
                        mc.kf, mc.mu, mc.pairs + d);
     hipLaunchKernelGGL(k_mc_transpose, dim3(blocks_for(static_cast<long long>(side->nf) * side->col_words)),

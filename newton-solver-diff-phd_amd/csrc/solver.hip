@@ -753,6 +753,7 @@ void build_mesh_contact(dphd_solver* s) {
     sd.col_words = (nv + 63) / 64;
     sd.rows = dzeros<uint64_t>(o, static_cast<size_t>(nv) * sd.row_words);
     sd.cols = dzeros<uint64_t>(o, static_cast<size_t>(nf) * sd.col_words);
+    sd.boxes = dzeros<double>(o, static_cast<size_t>(sd.row_words) * 6);
     return sd;
   };
   mc.ab = side(dva, static_cast<int>(va.size()), dfb, fb, s->mc_params[1]);

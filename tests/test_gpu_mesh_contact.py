@@ -94,3 +94,37 @@ def test_invalid_mesh_contact_is_rejected(cuda):
     ):
         with pytest.raises(ValueError):
             _solver(model, mesh_contact=options)
+
+
+def test_broad_phase_keeps_every_pair_of_a_wide_scene(cuda):
+    """A 12 x 12 slab under a 4 x 4 plate: most of the slab's face words lie out of the plate's reach
+    (and the other way round), so the face-word boxes skip them; forces and pair counts must equal
+    the oracle's brute-force evaluation on perturbed states."""
+    from diffphd_amd import meshes, parity
+
+    print("[case] mesh contact broad phase", flush=True)
+    ra, ta, _, _, da = parity.grid(height=0.0, dims=(12, 12, 1), cell=0.01, fix_bottom=True)
+    rb, tb, _, _, db = parity.grid(height=0.0, dims=(4, 4, 1), cell=0.01, origin=(0.035, 0.045, 0.0105))
+    vertices = np.concatenate([ra, rb]).astype(np.float32)
+    tets = np.concatenate([ta, tb + len(ra)])
+    free = ~np.concatenate([da, db])
+    faces = parity.boundary_faces(tets)
+    in_b = faces >= len(ra)
+    faces_a, faces_b = faces[~in_b.any(axis=1)], faces[in_b.all(axis=1)]
+    assert len(faces_a) > 4 * 64 and len(faces_b) > 64
+    materials = np.tile(np.array([[2.0e4, 5.0e4, 0.0]], np.float32), (len(tets), 1))
+    model = tet_model(cuda, tets, meshes.tet_poses(vertices, tets), materials, free, vertices, (0.0, 0.0, 0.0))
+    options = (0.004, 60.0, 150.0, 3.0, 0.5)
+    solver = _solver(model, mesh_contact=(*options, faces_a, faces_b))
+    solver.prepare(0.004)
+    rng = np.random.default_rng(21)
+    for trial in range(4):
+        pos = vertices.astype(np.float64) + 0.001 * rng.standard_normal(vertices.shape)
+        vel = 0.1 * rng.standard_normal(vertices.shape)
+        got = solver.mesh_contact_force(torch.as_tensor(pos, device=cuda), torch.as_tensor(vel, device=cuda))
+        ref, pairs = O.mesh_contact_force(pos, vel, faces_a, faces_b, *options)
+        assert pairs > 0
+        scale = max(np.abs(ref).max(), 1e-300)
+        np.testing.assert_allclose(got.cpu().numpy(), ref, rtol=0, atol=1e-13 * scale, err_msg=f"trial {trial}")
+        ab, ba = solver.mesh_contact_pair_counts
+        assert ab + ba == pairs == int(solver.mesh_contact_pairs[0])
