@@ -1,10 +1,10 @@
 """Parity fixtures of the reference's harness (schema ``newton.diff_phd_parity.v1``).
 
-The reference judges a port against the executed C++ DiffPhD by exchanging NPZ
+The reference compares a port with the executed C++ DiffPhD by exchanging NPZ
 files (docs/validation.md:451-472): a fixture holds one scene's inputs (SI
 units, canonical particle order, positively oriented Tet4, every value rounded
 to float32 and stored as float64), each side writes its outputs under fixed
-names, and tools/parity/compare.py judges them with fixed thresholds. This
+names, and tools/parity/compare.py checks them against fixed thresholds. This
 module is the port side of that exchange for this solver:
 
 * ``run(fixture)`` steps the fixture's scene on the GPU solver, back-propagates

@@ -104,7 +104,7 @@ def run(fx):
     return result
 
 
-def judge(fx, port, ref):
+def compare_results(fx, port, ref):
     """Failed checks of ``port`` against ``ref`` under compare.py's thresholds (empty when all pass):
     positions and velocities per step, iterations +-1, convergence, active set and multipliers with
     contact, gradients on free particles (1e-5 relative L2 without active contact, 1e-4 with) and the

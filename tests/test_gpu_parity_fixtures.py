@@ -1,5 +1,5 @@
 """The parity harness's named scenes (schema newton.diff_phd_parity.v1) on the GPU against the oracle,
-judged with the thresholds of the reference's tools/parity/compare.py:23-46: positions and velocities
+checked with the thresholds of the reference's tools/parity/compare.py:23-46: positions and velocities
 within 10 (rel_tol |q| + abs_tol) plus the float32 carry-over of earlier steps, iterations within 1,
 equal convergence, equal contact active sets, multipliers within 5e-6 relative, gradients within 1e-5
 (1e-4 with active contact) relative on the free particles."""
@@ -34,7 +34,7 @@ def test_fixture_outputs_meet_the_harness_thresholds(cuda, name):
     fx = _cases()[name]
     port = parity.run(fx, device=cuda)
     ref = parity_oracle.run(fx)
-    failed = parity_oracle.judge(fx, port, ref)
+    failed = parity_oracle.compare_results(fx, port, ref)
     assert not failed, failed
     np.testing.assert_array_equal(port["status_bits"], ref["status_bits"])
     assert abs(port["loss"] - ref["loss"]) <= 1e-6 * max(1.0, abs(ref["loss"]))
