@@ -1204,6 +1204,51 @@ __global__ __launch_bounds__(kBlock) void k_element_operator(DeviceState d) {
   }
 }
 
+// The Tet4 case (one sample, four corners): each block K_e[k][l] goes to LDS
+// as soon as it is formed (no block array in registers), and the workgroup's
+// 256 rows of 36 values, one contiguous range of ke, are then written with
+// consecutive lanes on consecutive addresses.
+__global__ __launch_bounds__(kBlock) void k_element_operator_tet4(DeviceState d) {
+  __shared__ double stage[kBlock * 36];
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int first = blockIdx.x * blockDim.x;
+  const int rows = min(kBlock, d.E * 4 - first);
+  if (tid < d.E * 4) {
+    const int e = tid / 4, k = tid % 4;
+    const size_t stride = static_cast<size_t>(d.E);
+    V3 tg[4];
+    tet_grads(d.poses, e, tg);
+    const double gkv[3] = {tg[k].x, tg[k].y, tg[k].z};
+    double p[9][3];
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) t += gkv[j] * d.jac[(c * 9 + 3 * a + j) * stride + e];
+        p[c][a] = t;
+      }
+    double* out = stage + threadIdx.x * 36;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const double glv[3] = {tg[l].x, tg[l].y, tg[l].z};
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int be = 0; be < 3; ++be) {
+          double t = 0.0;
+#pragma unroll
+          for (int n = 0; n < 3; ++n) t += glv[n] * p[be + 3 * n][a];
+          out[9 * l + 3 * a + be] = t;
+        }
+    }
+  }
+  __syncthreads();
+  double* dst = d.ke + static_cast<size_t>(first) * 36;
+  for (int t = threadIdx.x; t < rows * 36; t += kBlock) dst[t] = stage[t];
+}
+
 // Rows of the assembled operator: workgroup (chunk, slot), one thread per
 // row of the vertex chunk: block (i, j) = sum over the incident elements of
 // i, in incidence order, of K_e[k][l] with idx[e][k] = i and idx[e][l] = j
@@ -1817,7 +1862,10 @@ void launch_adjoint_setup(const DeviceState& d, const StepScalars& s, const doub
   const int total = d.E * d.Q;
   hipLaunchKernelGGL(k_jacobian, dim3((total + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d, x, targets);
   const int rows = d.E * d.K;
-  hipLaunchKernelGGL(k_element_operator, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d);
+  if (d.Q == 1)
+    hipLaunchKernelGGL(k_element_operator_tet4, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d);
+  else
+    hipLaunchKernelGGL(k_element_operator, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d);
   hipLaunchKernelGGL(k_operator_assemble, dim3(d.nvc, d.kmaxwidth), dim3(kBlock), 0, stream, d);
 }
 
