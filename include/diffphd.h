@@ -140,8 +140,9 @@ int dphd_set_mesh_contact(dphd_solver* s, double contact_radius, double kn_left,
  * vertices below `split` touch faces [left_count, face_count), the others
  * faces [0, left_count) of `faces` [face_count,3]); `count` registered
  * vertices [count] int32; Coulomb friction and restitution coefficients.
- * Host arrays. kind < 0 removes it. Needs a single-body solver with the
- * FP64 factor (DPHD_ENOTIMPL otherwise, at dphd_build). */
+ * Host arrays. kind < 0 removes it. Needs a single-body solver
+ * (DPHD_ENOTIMPL otherwise, at dphd_build); works with either factor
+ * precision. */
 int dphd_set_frictional_boundary(dphd_solver* s, int kind, const double* parameters, int count, const int* vertices,
                                  double friction, double restitution, int split, int left_count, int face_count,
                                  const int* faces);

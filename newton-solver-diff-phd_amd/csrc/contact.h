@@ -98,6 +98,12 @@ struct FrictionDev {
   int split = 0, left_count = 0, face_count = 0;
   const int* faces = nullptr;
   double* surface = nullptr;  // [n*3] the iterate the boundary last saw (restitution)
+  // FP32-factor mode: the solve is x + S^T S (b - A x), so the coupled forces
+  // of the previous iteration (fprev [K*3]) enter b, only their change goes
+  // through the panel, and freec [K*3] holds the free solution A^{-1} b at
+  // the candidates (the iterate's value minus the previous forces' response).
+  int defect = 0;
+  double *fprev = nullptr, *freec = nullptr;
 };
 
 // Columns A^{-1} e_j for registered vertices [j0, j0 + 3) from out [n*3]

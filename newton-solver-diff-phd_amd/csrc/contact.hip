@@ -397,6 +397,7 @@ __global__ __launch_bounds__(256) void k_fc_begin(FrictionDev fc, const double* 
     t1 = V3{best.x / lb, best.y / lb, best.z / lb};
     fc.candidate[c] = 1;
   }
+  if (fc.defect) st3(fc.fprev, c, V3{0, 0, 0});
   const V3 u = cross3(n, t1);
   const double l2 = sqrt(dot(u, u));
   st3(fc.normal, c, n);
@@ -608,8 +609,8 @@ __device__ void ldlt_solve_block(double* m, int n, double* b, int* perm, double*
 
 // One Newton step on the multipliers and the friction update
 // (FrictionalContact._newton / iterate, contact.py:865-987), one workgroup.
-__global__ __launch_bounds__(256) void k_fc_newton(FrictionDev fc, const double* x, const double* solved,
-                                                   const double* position, const int* active) {
+__global__ __launch_bounds__(256) void k_fc_newton(FrictionDev fc, const double* x, const double* position,
+                                                   const int* active) {
 #pragma clang fp contract(off)
   if (!*active) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -646,7 +647,7 @@ __global__ __launch_bounds__(256) void k_fc_newton(FrictionDev fc, const double*
     const int node = fc.nodes[c];
     const V3 n = ld3(fc.normal, c);
     const double w = fc.omega[c];
-    fc.rhs[c] = -fc.resid[c] + w * dot(n, ld3(x, node)) - w * dot(n, ld3(solved, node) + ld3(fc.response, c));
+    fc.rhs[c] = -fc.resid[c] + w * dot(n, ld3(x, node)) - w * dot(n, ld3(fc.freec, c) + ld3(fc.response, c));
   }
   __syncthreads();
   // contact_system: Omega W Omega + E over the coupled block.
@@ -684,7 +685,7 @@ __global__ __launch_bounds__(256) void k_fc_newton(FrictionDev fc, const double*
         continue;
       }
       const int node = fc.nodes[c];
-      const V3 step = ld3(solved, node) + ld3(fc.response, c) - ld3(position, node);
+      const V3 step = ld3(fc.freec, c) + ld3(fc.response, c) - ld3(position, node);
       const V3 n = ld3(fc.normal, c), t1 = ld3(fc.t1, c), t2 = ld3(fc.t2, c);
       const V3 tangential = step - dot(step, n) * n;
       const double u1 = dot(tangential, t1), u2 = dot(tangential, t2);
@@ -711,19 +712,53 @@ __global__ __launch_bounds__(256) void k_fc_newton(FrictionDev fc, const double*
   fc_forces(fc, true);
 }
 
+// The free solution A^{-1} b at the candidates: the solve's value there, less
+// (FP32-factor mode) the response B f_prev of the forces already in b.
+__global__ __launch_bounds__(256) void k_fc_free(FrictionDev fc, const double* solved, const int* active) {
+#pragma clang fp contract(off)
+  if (!*active) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= fc.K) return;
+  V3 v = ld3(solved, fc.nodes[c]);
+  if (fc.defect) {
+    for (int j = 0; j < fc.K; ++j) v = v - fc.blockT[static_cast<size_t>(j) * fc.K + c] * ld3(fc.fprev, j);
+  }
+  st3(fc.freec, c, v);
+}
+
 // contact_solution: x = A^{-1} b + sum over coupled candidates of A^{-1} e_j f_j.
+// In the FP32-factor mode b already holds the previous coupled forces, so
+// only their change is added: x = solve + sum_j A^{-1} e_j (f_j - f_prev_j).
 __global__ __launch_bounds__(256) void k_fc_solution(FrictionDev fc, double* xr, const int* active) {
 #pragma clang fp contract(off)
   if (!*active) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= fc.n) return;
-  const int A = *fc.nactive;
   V3 v = ld3(xr, i);
-  for (int k = 0; k < A; ++k) {
-    const int j = fc.order[k];
-    v = v + fc.panel[static_cast<size_t>(j) * fc.n + i] * ld3(fc.force, j);
+  if (fc.defect) {
+    for (int j = 0; j < fc.K; ++j) {
+      const V3 f = fc.slot[j] >= 0 ? ld3(fc.force, j) : V3{0, 0, 0};
+      const V3 delta = f - ld3(fc.fprev, j);
+      if (delta.x != 0.0 || delta.y != 0.0 || delta.z != 0.0)
+        v = v + fc.panel[static_cast<size_t>(j) * fc.n + i] * delta;
+    }
+  } else {
+    const int A = *fc.nactive;
+    for (int k = 0; k < A; ++k) {
+      const int j = fc.order[k];
+      v = v + fc.panel[static_cast<size_t>(j) * fc.n + i] * ld3(fc.force, j);
+    }
   }
   st3(xr, i, v);
+}
+
+// FP32-factor mode: the coupled forces of this iteration, for the next
+// iteration's right-hand side.
+__global__ __launch_bounds__(256) void k_fc_keep(FrictionDev fc, const int* active) {
+  if (!*active) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= fc.K) return;
+  st3(fc.fprev, c, fc.slot[c] >= 0 ? ld3(fc.force, c) : V3{0, 0, 0});
 }
 
 // apply_restitution / apply_mesh_restitution (contact.py:587-660).
@@ -965,8 +1000,10 @@ void launch_contact_iteration(const FrictionDev& fc, const double* x, double* xr
   if (fc.kind == 2 && fc.restitution > 0.0)
     hipLaunchKernelGGL(k_fc_surface, dim3(blocks_for(3LL * fc.n)), dim3(256), 0, stream, fc, x, active);
   hipLaunchKernelGGL(k_fc_scalars, dim3(fc.K), dim3(64), 0, stream, fc, x, active);
-  hipLaunchKernelGGL(k_fc_newton, dim3(1), dim3(256), 0, stream, fc, x, xr, position, active);
+  hipLaunchKernelGGL(k_fc_free, dim3(blocks_for(fc.K)), dim3(256), 0, stream, fc, xr, active);
+  hipLaunchKernelGGL(k_fc_newton, dim3(1), dim3(256), 0, stream, fc, x, position, active);
   hipLaunchKernelGGL(k_fc_solution, dim3(blocks_for(fc.n)), dim3(256), 0, stream, fc, xr, active);
+  if (fc.defect) hipLaunchKernelGGL(k_fc_keep, dim3(blocks_for(fc.K)), dim3(256), 0, stream, fc, active);
 }
 
 void launch_contact_finish(const FrictionDev& fc, const double* position, const float* velocity, double dt, double* x,

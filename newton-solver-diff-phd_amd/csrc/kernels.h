@@ -177,6 +177,11 @@ struct DeviceState {
   // kmap_off[b + 1]) (indices into ke's blocks, in the order they are added).
   int *kmap_off = nullptr, *kmap_src = nullptr;
   double* kval = nullptr;    // [kblocks*9]
+  // FP32-factor mode with a frictional boundary: each vertex's candidate
+  // (-1 if none) and the previous iteration's coupled forces, which the
+  // right-hand side b - A x includes.
+  const int* fc_vslot = nullptr;
+  const double* fc_fprev = nullptr;
   int kblocks = 0;
   int kmaxwidth = 0;         // the widest chunk's slots
   double *arhs = nullptr, *sol = nullptr;  // [N*3]
@@ -233,6 +238,11 @@ void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const do
                             float* gq_in, float* gqd_in, float* gf_in, double* gmat, double* gact, float* gdir,
                             const double* z, bool any_pinned, hipStream_t stream);
 void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, hipStream_t stream);
+// A^{-1} rhs to FP64 accuracy in the FP32-factor mode: S^T S rhs, then
+// `sweeps` refinements out += S^T S (rhs - A out) with the FP64 A (r, corr
+// are [N*3] work vectors); the plain products in the FP64 mode.
+void launch_refined_inverse(const DeviceState& d, const double* rhs, double* out, double* r, double* corr, int sweeps,
+                            hipStream_t stream);
 // Workgroups of the S and S^T product kernels the current device holds at
 // once (the kernel's occupancy per compute unit times the compute units).
 hipError_t product_capacity(bool f32, int* rows_blocks, int* cols_blocks);

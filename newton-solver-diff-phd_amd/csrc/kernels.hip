@@ -500,6 +500,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d) {
         V3 ax{0, 0, 0};
         for (int q = d.a_off[i]; q < d.a_off[i + 1]; ++q) ax = ax + d.a_val[q] * ld3(d.x, d.a_col[q]);
         r = r - ax;
+        if (d.fc_vslot) {
+          const int c = d.fc_vslot[i];
+          if (c >= 0) r = r + ld3(d.fc_fprev, c);
+        }
       }
       st3(d.p, i, t);
       store_rhs(d, i, r);
@@ -785,6 +789,21 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
       __syncthreads();
     }
   }
+}
+
+// r = rhs - A x with the FP64 constant matrix (FP32-factor mode), one thread
+// per vertex, all three coordinates.
+__global__ __launch_bounds__(kBlock) void k_residual_a(DeviceState d, const double* rhs, const double* x, double* r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.N) return;
+  V3 ax{0, 0, 0};
+  for (int q = d.a_off[i]; q < d.a_off[i + 1]; ++q) ax = ax + d.a_val[q] * ld3(x, d.a_col[q]);
+  st3(r, i, ld3(rhs, i) - ax);
+}
+
+__global__ __launch_bounds__(kBlock) void k_add3(int n, const double* x, double* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) st3(y, i, ld3(y, i) + ld3(x, i));
 }
 
 // rhs [N*3] FP64 (original order) -> the 4-wide postorder right-hand side.
@@ -1918,6 +1937,18 @@ void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, 
     hipLaunchKernelGGL(k_pack4<double>, dim3(grid), dim3(kBlock), 0, stream, d.N, d.gpost, rhs,
                        static_cast<double*>(d.rhs4));
   spmv_dispatch(d, out, nullptr, d.ones, stream);
+}
+
+void launch_refined_inverse(const DeviceState& d, const double* rhs, double* out, double* r, double* corr, int sweeps,
+                            hipStream_t stream) {
+  launch_apply_inverse(d, rhs, out, stream);
+  if (!d.defect) return;
+  const int grid = (d.N + kBlock - 1) / kBlock;
+  for (int k = 0; k < sweeps; ++k) {
+    hipLaunchKernelGGL(k_residual_a, dim3(grid), dim3(kBlock), 0, stream, d, rhs, out, r);
+    launch_apply_inverse(d, r, corr, stream);
+    hipLaunchKernelGGL(k_add3, dim3(grid), dim3(kBlock), 0, stream, d.N, corr, out);
+  }
 }
 
 void launch_hex_determinants(const DeviceState& d, const float* q, double* out, hipStream_t stream) {

@@ -772,7 +772,6 @@ void build_mesh_contact(dphd_solver* s) {
 // registered vertices (FrictionalContact.__init__ / build, contact.py:702-800).
 void build_friction(dphd_solver* s) {
   if (s->B != 1) throw std::invalid_argument("a frictional boundary needs a single-body solver");
-  if (s->d.defect) throw std::domain_error("a frictional boundary needs the FP64 factor");
   auto& o = s->owned;
   const int K = static_cast<int>(s->fc_nodes.size()), n = s->N;
   const size_t k1 = std::max(K, 1);
@@ -809,6 +808,15 @@ void build_friction(dphd_solver* s) {
   fc.compact = dzeros<double>(o, k1);
   fc.work = dzeros<double>(o, k1);
   fc.candidate = dzeros<int>(o, k1);
+  fc.freec = dzeros<double>(o, 3 * k1);
+  fc.fprev = dzeros<double>(o, 3 * k1);
+  fc.defect = s->d.defect;
+  if (s->d.defect) {
+    std::vector<int> vslot(n, -1);
+    for (int c = 0; c < K; ++c) vslot[s->fc_nodes[c]] = c;
+    s->d.fc_vslot = dupload(o, vslot);
+    s->d.fc_fprev = fc.fprev;
+  }
   if (fc.kind == 2) {
     fc.radius = s->fc_radius;
     fc.split = s->fc_split;
@@ -849,10 +857,15 @@ void build_friction(dphd_solver* s) {
   if (K == 0) return;
   double* rhs = dzeros<double>(o, 3 * static_cast<size_t>(n));
   double* out = dzeros<double>(o, 3 * static_cast<size_t>(n));
+  double* r = dzeros<double>(o, 3 * static_cast<size_t>(n));
+  double* corr = dzeros<double>(o, 3 * static_cast<size_t>(n));
   hipStream_t st = nullptr;
   for (int j0 = 0; j0 < K; j0 += 3) {
     launch_contact_units(fc, j0, rhs, st);
-    launch_apply_inverse(s->d, rhs, out, st);
+    // With the FP32 factor the columns are refined against the FP64 A. The
+    // step's fixed point does not depend on them (the forces enter the
+    // defect residual), only its convergence rate does.
+    launch_refined_inverse(s->d, rhs, out, r, corr, 3, st);
     launch_contact_store(fc, j0, out, st);
   }
   launch_contact_block(fc, st);

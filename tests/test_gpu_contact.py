@@ -16,7 +16,7 @@ from test_contact import GRID_CASES, MESH_CASES
 pytestmark = pytest.mark.gpu
 
 
-def _solver(cuda, name):
+def _solver(cuda, name, factor_precision=64):
     from diffphd_amd import SolverDiffPhD
 
     g = golden(f"contact_{name}")
@@ -30,7 +30,8 @@ def _solver(cuda, name):
     model = tet_model(cuda, tets, poses, materials, free, vertices, (0.0, 0.0, -9.81))
     solver = SolverDiffPhD(
         model, density=1000.0, frictional_boundary=boundary, frictional_vertices=g["nodes"],
-        friction=float(g["friction"]), restitution=float(g["restitution"]), **options,
+        friction=float(g["friction"]), restitution=float(g["restitution"]), factor_precision=factor_precision,
+        **options,
     )
     return g, model, solver
 
@@ -64,6 +65,40 @@ def test_contact_rollout_matches_reference(cuda, name):
             np.testing.assert_allclose(mult, g["multiplier"][t], **tol, err_msg=f"step {t}")
 
 
+@pytest.mark.parametrize("name", GRID_CASES + MESH_CASES)
+def test_contact_rollout_with_the_fp32_factor(cuda, name):
+    """The FP32-factor mode: the previous iteration's coupled forces enter the defect residual
+    b + J^T lambda - A x, so the step's fixed point is the FP64 one and only the path differs:
+    iterations within 2 of the golden's, positions and multipliers to the FP64 tests' bounds."""
+    print(f"[case] frictional contact {name} fp32 factor", flush=True)
+    g, model, solver = _solver(cuda, name, factor_precision=32)
+    worst_x = worst_m = 0.0
+    for t in range(len(g["x"])):
+        s_in, s_out = model.state(), model.state()
+        s_in.particle_q = torch.as_tensor(g["q"][t], device=cuda)
+        s_in.particle_qd = torch.as_tensor(g["qd"][t], device=cuda)
+        solver.step(s_in, s_out, None, None, 0.004)
+        bits = solver.read_status()
+        assert (bits == 0) == bool(g["converged"][t]), (t, bits)
+        if not bool(g["converged"][t]):
+            continue
+        assert abs(int(solver.iterations[0]) - int(g["iterations"][t])) <= 2, (t, int(solver.iterations[0]))
+        x = solver.solution().cpu().numpy()
+        mult = solver.contact_multipliers.cpu().numpy()
+        worst_x = max(worst_x, float(np.abs(x - g["x"][t]).max()))
+        scale = max(np.abs(g["multiplier"][t]).max(), 1e-30)
+        worst_m = max(worst_m, float(np.abs(mult - g["multiplier"][t]).max() / scale))
+    print(f"[result] contact {name} fp32 factor: max position error {worst_x:.3e} m, multipliers {worst_m:.3e} of max",
+          flush=True)
+    if float(g["rel_tol"]) <= 1e-10:
+        assert worst_x <= 1e-9 and worst_m <= 1e-5
+    else:
+        # At rel 1e-6 (sticking mesh friction) the gate stops a slowly contracting iteration,
+        # so a different path stops elsewhere: positions to the FP32 mode's 1e-5 of |x|, and
+        # the sticking multipliers, resolved only to the gate, to 5e-2 of the largest.
+        assert worst_x <= 1e-5 * np.abs(g["x"]).max() and worst_m <= 5e-2
+
+
 def test_moving_sphere_parameters_apply_between_steps(cuda):
     print("[case] frictional contact moving sphere", flush=True)
     from diffphd_amd import SolverDiffPhD
@@ -94,10 +129,12 @@ def test_moving_sphere_parameters_apply_between_steps(cuda):
     a.check_status()
 
 
-@pytest.mark.parametrize("name", ["woodbury", "woodbury_friction"])
-def test_woodbury_projection_matches_dense_reference(cuda, name):
-    """Parameter gradients through the contact rows (tests/test_diff_phd_adjoint.py:339-416)."""
-    print(f"[case] woodbury {name}", flush=True)
+@pytest.mark.parametrize("name,precision", [("woodbury", 64), ("woodbury_friction", 64), ("woodbury", 32),
+                                            ("woodbury_friction", 32)])
+def test_woodbury_projection_matches_dense_reference(cuda, name, precision):
+    """Parameter gradients through the contact rows (tests/test_diff_phd_adjoint.py:339-416), with the
+    FP64 and the FP32 factor (whose forward and adjoint fixed points are the FP64 ones)."""
+    print(f"[case] woodbury {name} fp{precision}", flush=True)
     from diffphd_amd import SolverDiffPhD
 
     g = golden(f"contact_{name}")
@@ -107,6 +144,7 @@ def test_woodbury_projection_matches_dense_reference(cuda, name):
         model, density=1000.0, frictional_boundary=("planar", (0.0, 0.0, 1.0, 0.0)), frictional_vertices=g["nodes"],
         friction=float(g["friction"]), muscle_directions=g["directions"], muscle_stiffness=4.0e4,
         max_iterations=int(g["max_iterations"]), abs_tol=float(g["abs_tol"]), rel_tol=float(g["rel_tol"]),
+        factor_precision=precision,
     )
     control = model.control()
     control.tet_activations = torch.as_tensor(g["contraction"], device=cuda).requires_grad_(True)
