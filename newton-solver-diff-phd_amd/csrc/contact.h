@@ -98,6 +98,11 @@ struct FrictionDev {
   int split = 0, left_count = 0, face_count = 0;
   const int* faces = nullptr;
   double* surface = nullptr;  // [n*3] the iterate the boundary last saw (restitution)
+  // Nearest-face search bounds: each side's faces in words of 64 (left words
+  // [0, left_words), right words [left_words, words)), box [words][6] = the
+  // corners' min xyz and max xyz at the positions the search reads.
+  int left_words = 0, words = 0;
+  double* face_box = nullptr;
   // FP32-factor mode: the solve is x + S^T S (b - A x), so the coupled forces
   // of the previous iteration (fprev [K*3]) enter b, only their change goes
   // through the panel, and freec [K*3] holds the free solution A^{-1} b at

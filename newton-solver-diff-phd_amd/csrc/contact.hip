@@ -301,30 +301,116 @@ __device__ __forceinline__ V3 boundary_normal(const FrictionDev& fc, V3 x) {
 }
 
 // The first face of minimal distance on the surface opposite `node`
-// (mesh_closest, contact.py:98-137): closest point, unit face normal, dist2.
+// The opposing side of a mesh-boundary vertex: faces [first, last) in words
+// [w0, w1). Vertices below `split` touch the right faces.
+struct FaceRange {
+  int first, last, w0, w1;
+};
+
+__device__ __forceinline__ FaceRange opposing_faces(const FrictionDev& fc, int node) {
+  if (node >= fc.split) return FaceRange{0, fc.left_count, 0, fc.left_words};
+  return FaceRange{fc.left_count, fc.face_count, fc.left_words, fc.words};
+}
+
+__device__ __forceinline__ int word_first_face(const FrictionDev& fc, int w) {
+  return w < fc.left_words ? w * 64 : fc.left_count + (w - fc.left_words) * 64;
+}
+
+// Squared distance from p to word w's box: a lower bound of dist2 for every
+// face of the word (each triangle lies in the box of its corners). An
+// unbounded box gives 0.
+__device__ __forceinline__ double box_bound2(const FrictionDev& fc, int w, V3 p) {
+#pragma clang fp contract(off)
+  const double* box = fc.face_box + 6 * static_cast<size_t>(w);
+  const double dx = fmax(fmax(box[0] - p.x, p.x - box[3]), 0.0);
+  const double dy = fmax(fmax(box[1] - p.y, p.y - box[4]), 0.0);
+  const double dz = fmax(fmax(box[2] - p.z, p.z - box[5]), 0.0);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// A word can be left out when its box lies farther from p than a face
+// already found, with a margin (1e-6 relative, 1e-9 of the coordinates'
+// scale) far above the rounding of the closest-point arithmetic, so no face
+// the search leaves out could tie or beat the one it keeps.
+__device__ __forceinline__ bool beyond(double bound2, double best2, V3 p) {
+  const double scale = 1.0 + fmax(fabs(p.x), fmax(fabs(p.y), fabs(p.z)));
+  return sqrt(bound2) > sqrt(best2) * (1.0 + 1.0e-6) + 1.0e-9 * scale;
+}
+
+// Builds face_box from x, one wavefront per word. A non-finite corner leaves
+// the word's box unbounded, so no search leaves it out. Skipped when *active
+// is zero (active may be null: always run).
+__global__ __launch_bounds__(64) void k_fc_face_boxes(FrictionDev fc, const double* x, const int* active) {
+  if (active && !*active) return;
+  const int w = blockIdx.x, lane = threadIdx.x;
+  const int f = word_first_face(fc, w) + lane;
+  const int last = w < fc.left_words ? fc.left_count : fc.face_count;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  bool finite = true;
+  if (f < last) {
+    for (int k = 0; k < 3; ++k) {
+      const V3 v = ld3(x, fc.faces[3 * f + k]);
+      for (int d = 0; d < 3; ++d) {
+        const double c = vget(v, d);
+        finite = finite && isfinite(c);
+        lo[d] = fmin(lo[d], c);
+        hi[d] = fmax(hi[d], c);
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
+      hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
+    }
+  const bool unbounded = __any(!finite);
+  if (lane == 0) {
+    double* box = fc.face_box + 6 * static_cast<size_t>(w);
+    for (int d = 0; d < 3; ++d) {
+      box[d] = unbounded ? -INFINITY : lo[d];
+      box[3 + d] = unbounded ? INFINITY : hi[d];
+    }
+  }
+}
+
+// (mesh_closest, contact.py:98-137): closest point, unit face normal, dist2
+// of the first face of minimal distance. The word nearest by box is searched
+// first; the others only where their box can hold a face as near.
 __device__ void mesh_closest(const FrictionDev& fc, V3 p, int node, const double* x, V3& closest, V3& normal,
                              double& best) {
 #pragma clang fp contract(off)
-  int first = fc.left_count, last = fc.face_count;
-  if (node >= fc.split) {
-    first = 0;
-    last = fc.left_count;
-  }
+  const FaceRange r = opposing_faces(fc, node);
   best = INFINITY;
   closest = V3{0, 0, 0};
   normal = V3{0, 0, 1};
-  for (int f = first; f < last; ++f) {
-    const int* face = fc.faces + 3 * f;
-    const V3 a = ld3(x, face[0]), b = ld3(x, face[1]), c = ld3(x, face[2]);
-    V3 point, w;
-    double dist2;
-    closest_point_triangle(p, a, b, c, point, w, dist2);
-    if (dist2 < best) {
-      best = dist2;
-      closest = point;
-      const V3 n = cross3(b - a, c - a);
-      const double length = sqrt(dot(n, n));
-      normal = length > kEps ? V3{n.x / length, n.y / length, n.z / length} : V3{0, 0, 1};
+  int best_face = r.last;
+  int nearest = r.w1;
+  double nearest2 = INFINITY;
+  for (int w = r.w0; w < r.w1; ++w) {
+    const double b2 = box_bound2(fc, w, p);
+    if (nearest == r.w1 || b2 < nearest2) {
+      nearest = w;
+      nearest2 = b2;
+    }
+  }
+  for (int k = -1; k < r.w1 - r.w0; ++k) {
+    const int w = k < 0 ? nearest : r.w0 + k;
+    if (w == r.w1 || (k >= 0 && (w == nearest || beyond(box_bound2(fc, w, p), best, p)))) continue;
+    const int f0 = word_first_face(fc, w), f1 = min(f0 + 64, r.last);
+    for (int f = f0; f < f1; ++f) {
+      const int* face = fc.faces + 3 * f;
+      const V3 a = ld3(x, face[0]), b = ld3(x, face[1]), c = ld3(x, face[2]);
+      V3 point, wts;
+      double dist2;
+      closest_point_triangle(p, a, b, c, point, wts, dist2);
+      if (dist2 < best || (dist2 == best && f < best_face)) {
+        best = dist2;
+        best_face = f;
+        closest = point;
+        const V3 n = cross3(b - a, c - a);
+        const double length = sqrt(dot(n, n));
+        normal = length > kEps ? V3{n.x / length, n.y / length, n.z / length} : V3{0, 0, 1};
+      }
     }
   }
 }
@@ -454,21 +540,52 @@ __global__ __launch_bounds__(64) void k_fc_scalars(FrictionDev fc, const double*
       }
       return;
     }
-    int first = fc.left_count, last = fc.face_count;
-    if (node >= fc.split) {
-      first = 0;
-      last = fc.left_count;
+    // The word nearest by box first, then every other word whose box can
+    // hold a face as near as the nearest word's best; the first face of
+    // minimal distance wins over all faces searched.
+    const FaceRange r = opposing_faces(fc, node);
+    const int last = r.last;
+    double nearest2 = INFINITY;
+    int nearest = r.w1;
+    for (int w = r.w0 + lane; w < r.w1; w += 64) {
+      const double b2 = box_bound2(fc, w, p);
+      if (nearest == r.w1 || b2 < nearest2) {
+        nearest = w;
+        nearest2 = b2;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(nearest2, off, 64);
+      const int ow = __shfl_xor(nearest, off, 64);
+      if (ow < r.w1 && (nearest == r.w1 || ob < nearest2 || (ob == nearest2 && ow < nearest))) {
+        nearest2 = ob;
+        nearest = ow;
+      }
     }
     double best = INFINITY;
     int best_face = last;
-    for (int f = first + lane; f < last; f += 64) {
+    auto search = [&](int w) {
+      const int f = word_first_face(fc, w) + lane;
+      if (f >= last) return;
       const int* face = fc.faces + 3 * f;
-      V3 point, w;
+      V3 point, wts;
       double dist2;
-      closest_point_triangle(p, ld3(x, face[0]), ld3(x, face[1]), ld3(x, face[2]), point, w, dist2);
-      if (dist2 < best) {
+      closest_point_triangle(p, ld3(x, face[0]), ld3(x, face[1]), ld3(x, face[2]), point, wts, dist2);
+      if (dist2 < best || (dist2 == best && f < best_face)) {
         best = dist2;
         best_face = f;
+      }
+    };
+    if (nearest < r.w1) search(nearest);
+    double bound = best;
+    for (int off = 32; off > 0; off >>= 1) bound = fmin(bound, __shfl_xor(bound, off, 64));
+    for (int base = r.w0; base < r.w1; base += 64) {
+      const int w = base + lane;
+      const bool open = w < r.w1 && w != nearest && !beyond(box_bound2(fc, w, p), bound, p);
+      unsigned long long mask = __ballot(open);
+      while (mask) {
+        search(base + __builtin_ctzll(mask));
+        mask &= mask - 1;
       }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -989,6 +1106,8 @@ void launch_contact_block(const FrictionDev& fc, hipStream_t stream) {
 
 void launch_contact_begin(const FrictionDev& fc, const double* position, double dt, hipStream_t stream) {
   if (fc.K == 0) return;
+  if (fc.kind == 2 && fc.words > 0)
+    hipLaunchKernelGGL(k_fc_face_boxes, dim3(fc.words), dim3(64), 0, stream, fc, position, nullptr);
   hipLaunchKernelGGL(k_fc_begin, dim3(blocks_for(fc.K)), dim3(256), 0, stream, fc, position);
   hipLaunchKernelGGL(k_fc_delassus, dim3(blocks_for(static_cast<long long>(fc.K) * fc.K)), dim3(256), 0, stream, fc,
                      dt);
@@ -999,6 +1118,8 @@ void launch_contact_iteration(const FrictionDev& fc, const double* x, double* xr
   if (fc.K == 0) return;
   if (fc.kind == 2 && fc.restitution > 0.0)
     hipLaunchKernelGGL(k_fc_surface, dim3(blocks_for(3LL * fc.n)), dim3(256), 0, stream, fc, x, active);
+  if (fc.kind == 2 && fc.words > 0)
+    hipLaunchKernelGGL(k_fc_face_boxes, dim3(fc.words), dim3(64), 0, stream, fc, x, active);
   hipLaunchKernelGGL(k_fc_scalars, dim3(fc.K), dim3(64), 0, stream, fc, x, active);
   hipLaunchKernelGGL(k_fc_free, dim3(blocks_for(fc.K)), dim3(256), 0, stream, fc, xr, active);
   hipLaunchKernelGGL(k_fc_newton, dim3(1), dim3(256), 0, stream, fc, x, position, active);
@@ -1009,6 +1130,8 @@ void launch_contact_iteration(const FrictionDev& fc, const double* x, double* xr
 void launch_contact_finish(const FrictionDev& fc, const double* position, const float* velocity, double dt, double* x,
                            hipStream_t stream) {
   if (fc.K == 0 || fc.restitution <= 0.0) return;
+  if (fc.kind == 2 && fc.words > 0)
+    hipLaunchKernelGGL(k_fc_face_boxes, dim3(fc.words), dim3(64), 0, stream, fc, fc.surface, nullptr);
   hipLaunchKernelGGL(k_fc_restitution, dim3(blocks_for(fc.K)), dim3(256), 0, stream, fc, position, velocity, dt, x);
 }
 

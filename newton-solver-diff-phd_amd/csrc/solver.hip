@@ -824,6 +824,9 @@ void build_friction(dphd_solver* s) {
     fc.face_count = static_cast<int>(s->fc_faces.size() / 3);
     fc.faces = dupload(o, s->fc_faces.empty() ? std::vector<int>{0, 0, 0} : s->fc_faces);
     fc.surface = dzeros<double>(o, 3 * static_cast<size_t>(n));
+    fc.left_words = (fc.left_count + 63) / 64;
+    fc.words = fc.left_words + (fc.face_count - fc.left_count + 63) / 64;
+    fc.face_box = dzeros<double>(o, 6 * static_cast<size_t>(std::max(fc.words, 1)));
   }
   // Woodbury rows of the backward and the recorded contact state they read.
   s->rec_mult = dzeros<double>(o, k1);

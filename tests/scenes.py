@@ -165,6 +165,24 @@ def two_blocks(gap):
     return vertices, tets, poses, materials, free, split, surface_faces(upper_t), surface_faces(lower_t + split)
 
 
+
+def stacked_blocks(gap, cell=0.025):
+    """A free 6x3x2 block ``gap`` above a 10x6x3 block with a pinned bottom (cells of ``cell``), offset
+    half a cell in x and y: 144 upper and 432 lower surface faces, several 64-face words per side.
+
+    Returns vertices, tets, poses, materials, free, split, faces_left (upper) and faces_right (lower).
+    """
+    upper_v, upper_t, _ = meshes.kuhn_tet_grid(6, 3, 2, cell, origin=(cell / 2, cell / 2, 3 * cell + gap))
+    lower_v, lower_t, _ = meshes.kuhn_tet_grid(10, 6, 3, cell)
+    split = len(upper_v)
+    vertices = np.concatenate([upper_v, lower_v]).astype(np.float32)
+    tets = np.concatenate([upper_t, lower_t + split]).astype(np.int32)
+    poses = meshes.tet_poses(vertices.astype(np.float64), tets)
+    materials = np.tile([2.0e4, 5.0e4, 0.0], (len(tets), 1)).astype(np.float32)
+    free = np.ones(len(vertices), bool)
+    free[split:] = lower_v[:, 2] > 0.0
+    return vertices, tets, poses, materials, free, split, surface_faces(upper_t), surface_faces(lower_t + split)
+
 # ---------------------------------------------------------------------------
 # The benchmark configurations (their inputs as the metric defines them), at
 # full or reduced size. Hex8 x-cells i with L // 3 < i <= 2 (L // 3) are the
