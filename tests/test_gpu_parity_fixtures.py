@@ -43,6 +43,22 @@ def test_fixture_outputs_meet_the_harness_thresholds(cuda, name):
         assert port["mesh_contact_force"].shape == (int(fx["steps"]), len(fx["q"]), 3)
 
 
+
+@pytest.mark.parametrize("name", NAMES)
+def test_fixture_outputs_with_the_fp32_factor(cuda, name):
+    # The FP32-factor mode has the FP64 mode's fixed point (defect correction against the FP64 matrix)
+    # and, on these scenes, the same iteration counts as the oracle: every threshold applies.
+    print(f"[case] parity fixture {name}, fp32 factor", flush=True)
+    fx = _cases()[name]
+    port = parity.run(fx, device=cuda, factor_precision=32)
+    ref = parity_oracle.run(fx)
+    failed = parity_oracle.compare_results(fx, port, ref)
+    assert not failed, failed
+    diff = np.abs(np.asarray(port["iterations"]) - np.asarray(ref["iterations"]))
+    print(f"[result] fixture {name} fp32 factor: largest iteration difference {int(diff.max())} "
+          f"(oracle max {int(np.max(ref['iterations']))})", flush=True)
+    assert abs(port["loss"] - ref["loss"]) <= 1e-6 * max(1.0, abs(ref["loss"]))
+
 def test_command_line_writes_fixture_and_port_files(cuda, tmp_path):
     print("[case] parity command line", flush=True)
     parity.main(["--out", str(tmp_path), "--cases", "mesh_contact_poke", "--device", str(cuda)])
