@@ -159,6 +159,11 @@ struct DeviceState {
   // iteration control
   int* iter = nullptr;      // [B]
   int* active = nullptr;    // [B]
+  // Work-list prefixes of the active bodies, [B + 1] each: pref[b] = the
+  // chunks of the active bodies before b, pref[B] = their total; vertex
+  // chunks and element chunks of the forward loop, vertex chunks of the
+  // adjoint loop. Written by the seed and check kernels.
+  int *pref_fv = nullptr, *pref_fe = nullptr, *pref_av = nullptr;
   int* status = nullptr;    // [1]
   int* any_active = nullptr;  // mapped host int
   // adjoint
@@ -222,6 +227,9 @@ struct FrictionDev;
 void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const double* targets,
                               const FrictionDev* fc, hipStream_t stream);
 void launch_any_active(const DeviceState& d, const int* active, hipStream_t stream);
+// Work-list prefixes after the active flags were set from outside a check
+// kernel (adjoint: pref_av from aactive, else pref_fv and pref_fe from active).
+void launch_active_prefix(const DeviceState& d, bool adjoint, hipStream_t stream);
 void launch_write_state(const DeviceState& d, const StepScalars& s, float* q, float* qd, double* checkpoint,
                         hipStream_t stream);
 

@@ -256,13 +256,33 @@ __device__ __forceinline__ ItemRange xcd_items(int total, int workers, int worke
                    (g / 8) * workers};
 }
 
-// Loop `fn(item, body)` over the active items; the whole block iterates
-// together (block-uniform bounds), so fn may use block barriers.
-template <typename Fn>
-__device__ __forceinline__ void for_active(const int* active, const int* base, int B, Fn fn) {
+// Writes the prefix of `active` over chunks `base` to out [B + 1]; every
+// thread of the workgroup calls it (dynamic LDS of B + 1 ints).
+__device__ void write_prefix(const int* active, const int* base, int B, int* out) {
   extern __shared__ int pref[];
-  const int total = active_prefix(active, base, B, pref);
+  __syncthreads();
+  active_prefix(active, base, B, pref);
+  for (int b = threadIdx.x; b <= B; b += blockDim.x) out[b] = pref[b];
+}
+
+__global__ __launch_bounds__(kBlock) void k_active_prefix(const int* active, const int* base0, int* out0,
+                                                          const int* base1, int* out1, int B) {
+  write_prefix(active, base0, B, out0);
+  if (base1) write_prefix(active, base1, B, out1);
+}
+
+// Loop `fn(item, body)` over the active items listed by the prefix `pref_g`
+// [B + 1] (chunks `base`); the whole block iterates together (block-uniform
+// bounds), so fn may use block barriers. A workgroup without items reads one
+// value; the others copy the prefix to LDS for the body search.
+template <typename Fn>
+__device__ __forceinline__ void for_active(const int* pref_g, const int* base, int B, Fn fn) {
+  extern __shared__ int pref[];
+  const int total = pref_g[B];
   const ItemRange r = xcd_items(total, 1, 0);
+  if (r.begin + r.first >= r.end) return;
+  for (int b = threadIdx.x; b <= B; b += blockDim.x) pref[b] = pref_g[b];
+  __syncthreads();
   for (int v = r.begin + r.first; v < r.end; v += r.stride) {
     const int b = find_body(pref, B, v);
     fn(base[b] + (v - pref[b]), b);
@@ -463,7 +483,7 @@ __device__ __forceinline__ void local_hex8_chunk(const DeviceState& d, int ci, i
 }
 
 __global__ __launch_bounds__(kBlock) void k_local(DeviceState d, const double* targets) {
-  for_active(d.active, d.ecb, d.B, [&](int ci, int b) {
+  for_active(d.pref_fe, d.ecb, d.B, [&](int ci, int b) {
     if (d.Q == 1)
       local_tet4_chunk(d, ci, b, targets);
     else
@@ -477,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_local(DeviceState d, const double* t
 __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d) {
   __shared__ double smem[2 * kBlock / kWave];
   __shared__ double res[2];
-  for_active(d.active, d.vcb, d.B, [&](int ci, int b) {
+  for_active(d.pref_fv, d.vcb, d.B, [&](int ci, int b) {
     const Chunk c = d.vchunks[ci];
     const int i = c.begin + threadIdx.x;
     double part[2] = {0.0, 0.0};
@@ -958,7 +978,7 @@ __device__ __forceinline__ V3 anderson_value(const DeviceState& d, int i, V3 raw
 __global__ __launch_bounds__(kBlock) void k_push_gram(DeviceState d) {
   __shared__ double smem[kGramEntries * kBlock / kWave];
   __shared__ double res[kGramEntries];
-  for_active(d.active, d.vcb, d.B, [&](int ci, int b) {
+  for_active(d.pref_fv, d.vcb, d.B, [&](int ci, int b) {
     const Chunk c = d.vchunks[ci];
     const int i = c.begin + threadIdx.x;
     double acc[kGramEntries];
@@ -983,7 +1003,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_conv(DeviceState d) {
   __shared__ double red[kGramEntries];
   __shared__ double gamma[kMaxWindow];
   __shared__ int meta[3];
-  for_active(d.active, d.vcb, d.B, [&](int ci, int b) {
+  for_active(d.pref_fv, d.vcb, d.B, [&](int ci, int b) {
     if (d.W > 0) {
       gram_solve(d, b, d.iter, ci == d.vcb[b], red, gamma, meta, d.status);
     } else if (threadIdx.x == 0) {
@@ -1007,24 +1027,28 @@ __global__ __launch_bounds__(kBlock) void k_apply_conv(DeviceState d) {
   });
 }
 
-__global__ void k_conv_check(DeviceState d, StepScalars s) {
-  const int b = blockIdx.x;
-  if (!d.active[b]) return;
-  __shared__ double red[4];
-  const int c0 = d.vcb[b], c1 = d.vcb[b + 1];
-  if (threadIdx.x < 4) red[threadIdx.x] = chunk_sum(d.conv_part, 4, threadIdx.x, c0, c1);
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  const int k = d.iter[b];
-  const bool step_small = sqrt(red[0]) <= s.rel_tol * sqrt(red[1]) + s.abs_tol;
-  const bool rhs_small = sqrt(red[2]) <= s.rel_tol * sqrt(red[3]) + s.abs_tol;
-  d.iter[b] = k + 1;
-  if (k > 0 && step_small && rhs_small) {
-    d.active[b] = 0;
-  } else if (k + 1 >= s.max_iterations) {
-    d.active[b] = 0;
-    atomicOr(d.status, 1);
+// Every body's convergence test (one thread per body, one workgroup), then
+// the work-list prefixes of the bodies still active.
+__global__ __launch_bounds__(kBlock) void k_conv_check(DeviceState d, StepScalars s) {
+  for (int b = threadIdx.x; b < d.B; b += kBlock) {
+    if (!d.active[b]) continue;
+    const int c0 = d.vcb[b], c1 = d.vcb[b + 1];
+    double red[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[q] = chunk_sum(d.conv_part, 4, q, c0, c1);
+    const int k = d.iter[b];
+    const bool step_small = sqrt(red[0]) <= s.rel_tol * sqrt(red[1]) + s.abs_tol;
+    const bool rhs_small = sqrt(red[2]) <= s.rel_tol * sqrt(red[3]) + s.abs_tol;
+    d.iter[b] = k + 1;
+    if (k > 0 && step_small && rhs_small) {
+      d.active[b] = 0;
+    } else if (k + 1 >= s.max_iterations) {
+      d.active[b] = 0;
+      atomicOr(d.status, 1);
+    }
   }
+  write_prefix(d.active, d.vcb, d.B, d.pref_fv);
+  write_prefix(d.active, d.ecb, d.B, d.pref_fe);
 }
 
 // ---------------------------------------------------------------------------
@@ -1317,7 +1341,7 @@ __global__ __launch_bounds__(kBlock) void k_operator_assemble(DeviceState d) {
 // work = K mu + rhs (gather with accumulate, adjoint.py:939), or
 // rhs + (K - A) mu in the defect mode; one thread per row.
 __global__ __launch_bounds__(kBlock) void k_adj_rhs(DeviceState d) {
-  for_active(d.aactive, d.vcb, d.B, [&](int ci, int) {
+  for_active(d.pref_av, d.vcb, d.B, [&](int ci, int) {
     const Chunk c = d.vchunks[ci];
     const int i = c.begin + threadIdx.x;
     if (i >= c.end) return;
@@ -1372,7 +1396,7 @@ __device__ __forceinline__ V3 scrub(V3 v, bool& scrubbed) {
 __global__ __launch_bounds__(kBlock) void k_adj_clip_partial(DeviceState d) {
   __shared__ double smem[kBlock / kWave];
   __shared__ double res[1];
-  for_active(d.aactive, d.vcb, d.B, [&](int ci, int) {
+  for_active(d.pref_av, d.vcb, d.B, [&](int ci, int) {
     const Chunk c = d.vchunks[ci];
     const int i = c.begin + threadIdx.x;
     double part[1] = {0.0};
@@ -1406,7 +1430,7 @@ __device__ __forceinline__ double clip_scale(const DeviceState& d, const double*
 __global__ __launch_bounds__(kBlock) void k_adj_push(DeviceState d) {
   __shared__ double smem[(kGramEntries + 1) * kBlock / kWave];
   __shared__ double res[kGramEntries + 1];
-  for_active(d.aactive, d.vcb, d.B, [&](int ci, int b) {
+  for_active(d.pref_av, d.vcb, d.B, [&](int ci, int b) {
     const double sc = clip_scale(d, d.clip_part, b, ci == d.vcb[b]);
     const Chunk c = d.vchunks[ci];
     const int i = c.begin + threadIdx.x;
@@ -1442,7 +1466,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_apply(DeviceState d) {
   __shared__ double red[kGramEntries];
   __shared__ double gamma[kMaxWindow];
   __shared__ int meta[3];
-  for_active(d.aactive, d.vcb, d.B, [&](int ci, int b) {
+  for_active(d.pref_av, d.vcb, d.B, [&](int ci, int b) {
     if (d.W > 0) {
       gram_solve(d, b, d.aiter, ci == d.vcb[b], red, gamma, meta, d.status);
     } else if (threadIdx.x == 0) {
@@ -1469,7 +1493,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_apply(DeviceState d) {
 __global__ __launch_bounds__(kBlock) void k_adj_finish(DeviceState d) {
   __shared__ double smem[kBlock / kWave];
   __shared__ double res[1];
-  for_active(d.aactive, d.vcb, d.B, [&](int ci, int b) {
+  for_active(d.pref_av, d.vcb, d.B, [&](int ci, int b) {
     const double sc = clip_scale(d, d.clip_part2, b, ci == d.vcb[b]);
     const Chunk c = d.vchunks[ci];
     const int i = c.begin + threadIdx.x;
@@ -1485,21 +1509,24 @@ __global__ __launch_bounds__(kBlock) void k_adj_finish(DeviceState d) {
   });
 }
 
-__global__ void k_adj_check(DeviceState d, StepScalars s) {
-  const int b = blockIdx.x;
-  if (!d.aactive[b]) return;
-  if (threadIdx.x != 0) return;
-  const double t0 = chunk_sum(d.conv_part, 4, 0, d.vcb[b], d.vcb[b + 1]);
-  const double t1 = chunk_sum(d.conv_part, 4, 1, d.vcb[b], d.vcb[b + 1]);
-  const int k = d.aiter[b];
-  d.aiter[b] = k + 1;
-  if (k > 0 && sqrt(t0) < s.rel_tol * sqrt(t1) + s.abs_tol) {
-    d.aactive[b] = 0;
-  } else if (k + 1 >= s.max_iterations) {
-    d.aactive[b] = 0;
-    // Woodbury columns are truncated by design; only the backbone reports.
-    if (s.report) atomicOr(d.status, 8);
+// Every body's adjoint convergence test (one thread per body, one
+// workgroup), then the work-list prefix of the bodies still active.
+__global__ __launch_bounds__(kBlock) void k_adj_check(DeviceState d, StepScalars s) {
+  for (int b = threadIdx.x; b < d.B; b += kBlock) {
+    if (!d.aactive[b]) continue;
+    const double t0 = chunk_sum(d.conv_part, 4, 0, d.vcb[b], d.vcb[b + 1]);
+    const double t1 = chunk_sum(d.conv_part, 4, 1, d.vcb[b], d.vcb[b + 1]);
+    const int k = d.aiter[b];
+    d.aiter[b] = k + 1;
+    if (k > 0 && sqrt(t0) < s.rel_tol * sqrt(t1) + s.abs_tol) {
+      d.aactive[b] = 0;
+    } else if (k + 1 >= s.max_iterations) {
+      d.aactive[b] = 0;
+      // Woodbury columns are truncated by design; only the backbone reports.
+      if (s.report) atomicOr(d.status, 8);
+    }
   }
+  write_prefix(d.aactive, d.vcb, d.B, d.pref_av);
 }
 
 // Routing guard per body (adjoint.py:226-261): one block loops over the body.
@@ -1852,6 +1879,17 @@ hipError_t product_capacity(bool f32, int* rows_blocks, int* cols_blocks) {
 void launch_begin_step(const DeviceState& d, const StepScalars& s, const float* q, const float* qd, const float* f,
                        const float* dirichlet, hipStream_t stream) {
   hipLaunchKernelGGL(k_begin_step, dim3(d.nvc), dim3(kBlock), 0, stream, d, s, q, qd, f, dirichlet);
+  launch_active_prefix(d, false, stream);
+}
+
+void launch_active_prefix(const DeviceState& d, bool adjoint, hipStream_t stream) {
+  const size_t lds = prefix_lds(d);
+  if (adjoint)
+    hipLaunchKernelGGL(k_active_prefix, dim3(1), dim3(kBlock), lds, stream, d.aactive, d.vcb, d.pref_av, nullptr,
+                       nullptr, d.B);
+  else
+    hipLaunchKernelGGL(k_active_prefix, dim3(1), dim3(kBlock), lds, stream, d.active, d.vcb, d.pref_fv, d.ecb,
+                       d.pref_fe, d.B);
 }
 
 void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const double* targets,
@@ -1863,7 +1901,7 @@ void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const 
   if (fc) launch_contact_iteration(*fc, d.x, d.xr, d.q0, d.active, stream);
   if (d.W > 0) hipLaunchKernelGGL(k_push_gram, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   hipLaunchKernelGGL(k_apply_conv, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
-  hipLaunchKernelGGL(k_conv_check, dim3(d.B), dim3(64), 0, stream, d, s);
+  hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(kBlock), lds, stream, d, s);
 }
 
 void launch_any_active(const DeviceState& d, const int* active, hipStream_t stream) {
@@ -1878,6 +1916,7 @@ void launch_write_state(const DeviceState& d, const StepScalars& s, float* q, fl
 void launch_adjoint_setup(const DeviceState& d, const StepScalars& s, const double* x, const double* targets,
                           const float* gq_out, const float* gqd_out, hipStream_t stream) {
   hipLaunchKernelGGL(k_adj_seed, dim3(d.nvc), dim3(kBlock), 0, stream, d, s.dt, gq_out, gqd_out);
+  launch_active_prefix(d, true, stream);
   const int total = d.E * d.Q;
   hipLaunchKernelGGL(k_jacobian, dim3((total + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d, x, targets);
   const int rows = d.E * d.K;
@@ -1896,7 +1935,7 @@ void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStr
   hipLaunchKernelGGL(k_adj_push, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   hipLaunchKernelGGL(k_adj_apply, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   hipLaunchKernelGGL(k_adj_finish, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
-  hipLaunchKernelGGL(k_adj_check, dim3(d.B), dim3(64), 0, stream, d, s);
+  hipLaunchKernelGGL(k_adj_check, dim3(1), dim3(kBlock), lds, stream, d, s);
 }
 
 void launch_adjoint_route(const DeviceState& d, hipStream_t stream) {

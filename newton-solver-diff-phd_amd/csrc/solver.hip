@@ -640,6 +640,9 @@ void build_solver(dphd_solver* s, double dt) {
   d.sol = dzeros<double>(o, v3);
   d.aiter = dzeros<int>(o, B);
   d.aactive = dzeros<int>(o, B);
+  d.pref_fv = dzeros<int>(o, B + 1);
+  d.pref_fe = dzeros<int>(o, B + 1);
+  d.pref_av = dzeros<int>(o, B + 1);
   d.ascale = dzeros<double>(o, 2 * static_cast<size_t>(B));
   d.route = dzeros<double>(o, 2 * static_cast<size_t>(B));
   d.gmat_tmp = dzeros<double>(o, 2 * static_cast<size_t>(E));
@@ -959,6 +962,7 @@ const double* woodbury(dphd_solver* s) {
   }
   for (int k = 0; k < count; ++k) {
     launch_woodbury_seed(w, k, d.arhs, d.sol, d.aiter, d.aactive, d.aa, st);
+    launch_active_prefix(d, true, st);
     run_loop(s, kWoodbury);
     HIP_CHECK(hipMemcpyAsync(w.columns + static_cast<size_t>(k) * 3 * s->N, d.sol, vb, hipMemcpyDeviceToDevice, st));
   }
