@@ -53,6 +53,27 @@ __device__ __forceinline__ double chunk_sum(const double* parts, int stride, int
   return t;
 }
 
+// chunk_sum of components 0 .. N-1 at once (each in chunk order, as
+// chunk_sum adds it), all 8 x N loads of a round in flight together.
+template <int N>
+__device__ __forceinline__ void chunk_sums(const double* parts, int stride, int c0, int c1, double (&t)[N]) {
+#pragma unroll
+  for (int q = 0; q < N; ++q) t[q] = 0.0;
+  for (int c = c0; c < c1; c += 8) {
+    double p[8][N];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int q = 0; q < N; ++q) p[u][q] = parts[static_cast<size_t>(stride) * min(c + u, c1 - 1) + q];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (c + u < c1) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) t[q] += p[u][q];
+      }
+  }
+}
+
 // Lane exchanges without LDS: the gfx950 row swaps (v_permlane32_swap
 // exchanges the upper half of its first operand with the lower half of its
 // second, v_permlane16_swap the odd 16-lane rows of the first with the even
@@ -1034,8 +1055,7 @@ __global__ __launch_bounds__(kBlock) void k_conv_check(DeviceState d, StepScalar
     if (!d.active[b]) continue;
     const int c0 = d.vcb[b], c1 = d.vcb[b + 1];
     double red[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) red[q] = chunk_sum(d.conv_part, 4, q, c0, c1);
+    chunk_sums<4>(d.conv_part, 4, c0, c1, red);
     const int k = d.iter[b];
     const bool step_small = sqrt(red[0]) <= s.rel_tol * sqrt(red[1]) + s.abs_tol;
     const bool rhs_small = sqrt(red[2]) <= s.rel_tol * sqrt(red[3]) + s.abs_tol;
@@ -1514,8 +1534,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_finish(DeviceState d) {
 __global__ __launch_bounds__(kBlock) void k_adj_check(DeviceState d, StepScalars s) {
   for (int b = threadIdx.x; b < d.B; b += kBlock) {
     if (!d.aactive[b]) continue;
-    const double t0 = chunk_sum(d.conv_part, 4, 0, d.vcb[b], d.vcb[b + 1]);
-    const double t1 = chunk_sum(d.conv_part, 4, 1, d.vcb[b], d.vcb[b + 1]);
+    double t[2];
+    chunk_sums<2>(d.conv_part, 4, d.vcb[b], d.vcb[b + 1], t);
+    const double t0 = t[0], t1 = t[1];
     const int k = d.aiter[b];
     d.aiter[b] = k + 1;
     if (k > 0 && sqrt(t0) < s.rel_tol * sqrt(t1) + s.abs_tol) {
