@@ -1316,9 +1316,13 @@ __global__ __launch_bounds__(kBlock) void k_element_operator_tet4(DeviceState d)
 // row of the vertex chunk: block (i, j) = sum over the incident elements of
 // i, in incidence order, of K_e[k][l] with idx[e][k] = i and idx[e][l] = j
 // (the element blocks listed for it by the host, four loads in flight);
-// minus A_ij I in the defect mode.
+// minus A_ij I in the defect mode. Workgroup g runs on XCD g % 8; the slots
+// of one chunk are consecutive workgroups of one XCD, so the element rows
+// (four blocks, one per slot) a chunk reads are shared in that XCD's L2.
 __global__ __launch_bounds__(kBlock) void k_operator_assemble(DeviceState d) {
-  const int ci = blockIdx.x, s = blockIdx.y;
+  const int g = static_cast<int>(blockIdx.x), x = g % 8, r = g / 8;
+  const int ci = (r / d.kmaxwidth) * 8 + x, s = r % d.kmaxwidth;
+  if (ci >= d.nvc) return;
   const int w = d.kwidth[ci];
   if (s >= w) return;
   const Chunk c = d.vchunks[ci];
@@ -1945,7 +1949,8 @@ void launch_adjoint_setup(const DeviceState& d, const StepScalars& s, const doub
     hipLaunchKernelGGL(k_element_operator_tet4, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d);
   else
     hipLaunchKernelGGL(k_element_operator, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d);
-  hipLaunchKernelGGL(k_operator_assemble, dim3(d.nvc, d.kmaxwidth), dim3(kBlock), 0, stream, d);
+  const long long groups = 8LL * ((d.nvc + 7) / 8) * d.kmaxwidth;
+  hipLaunchKernelGGL(k_operator_assemble, dim3(static_cast<unsigned>(groups)), dim3(kBlock), 0, stream, d);
 }
 
 void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStream_t stream) {
