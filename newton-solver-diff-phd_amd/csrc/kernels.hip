@@ -1115,10 +1115,13 @@ __global__ __launch_bounds__(kBlock) void k_jacobian(DeviceState d, const double
     m = ld3(d.mdir, e);
     target = targets ? targets[e] : 1.0;
   }
-  double jac[81];
-  sample_jacobian(f, d.material, d.jwc[e], d.jwv[e], d.jwn[e], nh, wm, m, target, jac);
+  // Each column is stored as it is formed (81 values held at once spill).
   const size_t stride = static_cast<size_t>(total);
-  for (int t = 0; t < 81; ++t) d.jac[t * stride + sample] = jac[t];
+  double* out = d.jac + sample;
+  sample_jacobian_columns(f, d.material, d.jwc[e], d.jwv[e], d.jwn[e], nh, wm, m, target, [&](int c, const M3& dp) {
+#pragma unroll
+    for (int t = 0; t < 9; ++t) out[(c * 9 + t) * stride] = dp.a[t / 3][t % 3];
+  });
 }
 
 // ds = sum_c dF_c J_c for one sample.

@@ -637,9 +637,11 @@ DPHD_HD int sample_stress(const M3& f, const M3& fd, SampleWeights w, NhParams n
 }
 
 // The 9 columns d(w P)/dF . E_c, c = b + 3 n (Eigen column-major flatten),
-// of one sample (DP/adjoint.py:92-137). jac[c][i][j] row-major 3x3 each.
-DPHD_HD void sample_jacobian(const M3& f, int material, double wc, double wv, double wn, NhParams nh, double wm,
-                             V3 mdir, double mtarget, double* jac /* 81 */) {
+// of one sample (DP/adjoint.py:92-137), handed to sink(c, column) one at a
+// time, so a caller can store each column without holding all 81 values.
+template <typename Sink>
+DPHD_HD void sample_jacobian_columns(const M3& f, int material, double wc, double wv, double wn, NhParams nh,
+                                     double wm, V3 mdir, double mtarget, Sink&& sink) {
   M3 u, v;
   V3 sigma;
   unsigned_svd(f, u, sigma, v);
@@ -661,9 +663,17 @@ DPHD_HD void sample_jacobian(const M3& f, int material, double wc, double wv, do
       dp = add(dp, scale(wn, nh_prox_jvp(u, sigma, v, st, df, nh.mu, nh.lame, nh.penalty)));
     }
     if (wm != 0.0) dp = add(dp, scale(wm, outer(muscle_jvp(f, mdir, mtarget, df), mdir)));
+    sink(c, dp);
+  }
+}
+
+// The same columns as jac[c][i][j], row-major 3x3 each.
+DPHD_HD void sample_jacobian(const M3& f, int material, double wc, double wv, double wn, NhParams nh, double wm,
+                             V3 mdir, double mtarget, double* jac /* 81 */) {
+  sample_jacobian_columns(f, material, wc, wv, wn, nh, wm, mdir, mtarget, [&](int c, const M3& dp) {
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) jac[c * 9 + i * 3 + j] = dp.a[i][j];
-  }
+  });
 }
 
 }  // namespace dphd
