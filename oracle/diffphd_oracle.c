@@ -831,15 +831,19 @@ int oracle_forward(const oracle_problem* p, double dt, const float* q, const flo
 /* ------------------------------------------------------------------------ */
 /* Backward pass (adjoint.py).                                              */
 
-static int safe_clip(const oracle_problem* p, double* v, int chunks, double* part) {
-  const int n = p->n;
+/* Safe clip of an adjoint vector (adjoint.py:187-225, _clip_partials and
+ * _clip_apply): Dirichlet rows zeroed, non-finite components replaced by 0
+ * (status 16), then, when limit > 0 and the norm is finite and above it, the
+ * whole vector scaled to the limit (status 16). The squared norm is summed in
+ * CHUNK-row partials, in chunk order. */
+static int clip_vector(int n, const unsigned char* free_rows, double* v, int chunks, double* part, double limit) {
   int st = 0;
   for (int c = 0; c < chunks; ++c) {
     double tot = 0.0;
     const int hi = (c + 1) * CHUNK < n ? (c + 1) * CHUNK : n;
     for (int i = c * CHUNK; i < hi; ++i) {
       double* x = v + 3 * i;
-      if (!p->free[i]) x[0] = x[1] = x[2] = 0.0;
+      if (!free_rows[i]) x[0] = x[1] = x[2] = 0.0;
       for (int d = 0; d < 3; ++d)
         if (!isfinite(x[d])) {
           x[d] = 0.0;
@@ -852,11 +856,49 @@ static int safe_clip(const oracle_problem* p, double* v, int chunks, double* par
   double total = 0.0;
   for (int c = 0; c < chunks; ++c) total += part[c];
   const double len = sqrt(total);
-  if (isfinite(len) && len > CLIP_NORM) {
-    for (int t = 0; t < 3 * n; ++t) v[t] = (CLIP_NORM / len) * v[t];
+  if (limit > 0.0 && isfinite(len) && len > limit) {
+    for (int t = 0; t < 3 * n; ++t) v[t] = (limit / len) * v[t];
     st |= 16;
   }
   return st;
+}
+
+static int safe_clip(const oracle_problem* p, double* v, int chunks, double* part) {
+  return clip_vector(p->n, p->free, v, chunks, part, CLIP_NORM);
+}
+
+/* Routing guard of a finished adjoint solve (adjoint.py:226-261) from the
+ * squared norms of its right-hand side and solution: diverged when
+ * |mu| >= 0.999 clip_norm (scale 0), else scale = min(1, (|g| + 1) / (m/h^2 |mu|));
+ * either intervention sets status 16. */
+static int routing_guard(double rhs_sq, double mu_sq, double inertia, double clip_norm, double* scale,
+                         int* diverged) {
+  const double mu_norm = sqrt(mu_sq);
+  *diverged = clip_norm > 0.0 && mu_norm >= 0.999 * clip_norm;
+  *scale = 1.0;
+  if (*diverged) {
+    *scale = 0.0;
+    return 16;
+  }
+  const double cap = sqrt(rhs_sq) + 1.0, inj = inertia * mu_norm;
+  if (inj > cap) {
+    *scale = cap / inj;
+    return 16;
+  }
+  return 0;
+}
+
+int oracle_safe_clip(int n, const unsigned char* free_rows, double* v, double limit) {
+  const int chunks = n > 0 ? (n + CHUNK - 1) / CHUNK : 1;
+  double* part = malloc(sizeof(double) * chunks);
+  const int st = clip_vector(n, free_rows, v, chunks, part, limit);
+  free(part);
+  return st;
+}
+
+int oracle_routing_guard(double rhs_sq, double mu_sq, double inertia, double clip_norm, double* scale,
+                         int* diverged) {
+  return routing_guard(rhs_sq, mu_sq, inertia, clip_norm, scale, diverged);
 }
 
 /* Fixed point (A - K) sol = rhs from a zero start with at most `limit`
@@ -1138,19 +1180,9 @@ int oracle_backward(const oracle_problem* p, double dt, const double* x, const d
     r0 += a0;
     r1 += a1;
   }
-  const double mu_norm = sqrt(r1);
-  const int diverged = mu_norm >= 0.999 * CLIP_NORM;
   double scale = 1.0;
-  if (diverged) {
-    scale = 0.0;
-    st |= 16;
-  } else {
-    const double cap = sqrt(r0) + 1.0, inj = inertia * mu_norm;
-    if (inj > cap) {
-      scale = cap / inj;
-      st |= 16;
-    }
-  }
+  int diverged = 0;
+  st |= routing_guard(r0, r1, inertia, CLIP_NORM, &scale, &diverged);
   /* Parameter gradients use z, the backbone projected off the active
    * contact rows; state gradients use mu (adjoint.py:1145-1148). */
   double* z = sol;

@@ -90,6 +90,16 @@ int oracle_backward(const oracle_problem* p, double dt, const double* x, const d
                     const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
                     float* gdirichlet, double* gactivations, double* mu_out, int* iterations, int* status);
 
+/* Safe clip of one adjoint vector v [n,3] in place with norm limit `limit`
+ * (adjoint.py _clip_partials / _clip_apply); returns the status bits (16 when
+ * a component was scrubbed or the norm clipped). */
+int oracle_safe_clip(int n, const unsigned char* free_rows, double* v, double limit);
+
+/* Routing guard (adjoint.py _routing_guard) from |rhs|^2 and |mu|^2: writes
+ * the state-gradient scale and the divergence flag; returns the status bits. */
+int oracle_routing_guard(double rhs_sq, double mu_sq, double inertia, double clip_norm, double* scale,
+                         int* diverged);
+
 void oracle_projections(int count, const double* f, const double* dirs, double mu, double lame, double penalty,
                         double fibre_target, double* sigma, double* recon, double* rotation, double* volume,
                         double* neo_hookean, double* fibre);

@@ -418,6 +418,24 @@ class OracleBody:
         return out
 
 
+def safe_clip(values, free, limit):
+    """The backward's safe clip of one [n,3] vector with norm limit ``limit``; returns (values, status)."""
+    v = np.array(values, dtype=np.float64, order="C").reshape(-1, 3)
+    free = np.ascontiguousarray(np.asarray(free).astype(np.uint8))
+    status = lib().oracle_safe_clip(len(v), _ptr(free), _ptr(v), ctypes.c_double(limit))
+    return v, status
+
+
+def routing_guard(rhs_sq, mu_sq, inertia, clip_norm):
+    """The backward's routing guard from squared norms; returns (scale, diverged, status)."""
+    scale, diverged = ctypes.c_double(), ctypes.c_int()
+    status = lib().oracle_routing_guard(
+        ctypes.c_double(rhs_sq), ctypes.c_double(mu_sq), ctypes.c_double(inertia), ctypes.c_double(clip_norm),
+        ctypes.byref(scale), ctypes.byref(diverged),
+    )
+    return scale.value, diverged.value, status
+
+
 def projections(f, directions, mu, lame, penalty, target):
     f = _c(f, np.float64)
     d = _c(directions, np.float64)
