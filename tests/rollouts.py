@@ -50,12 +50,15 @@ def oracle_rollout_grad(body, scene, records, gq_final, gqd_final=None, explicit
     gqd = np.zeros_like(gq) if gqd_final is None else np.asarray(gqd_final, np.float32)
     gmat = 0.0
     status = 0
+    iterations = []
     for out in reversed(records):
         bw = body.backward(scene["dt"], out["x"], gq, gqd, explicit_dirichlet=explicit_dirichlet)
         gmat = gmat + bw["materials"]
         gq, gqd = bw["q"], bw["qd"]
         status |= bw["status"]
-    return {"materials": gmat, "q": gq, "qd": gqd, "status": status}
+        iterations.append(bw["iterations"])
+    # Adjoint iterations per step, in backward (last step first) order.
+    return {"materials": gmat, "q": gq, "qd": gqd, "status": status, "iterations": iterations}
 
 
 def gpu_rollout(solver, model, dt, steps, q0, qd0, force=None, dirichlet=None, track=False, before=None):

@@ -262,3 +262,57 @@ def config5(cells=(16, 16, 8), steps=200):
     return dict(vertices=vertices.astype(np.float32), tets=tets.astype(np.int32), poses=poses, materials=materials,
                 free=free, surface=surface, targets=targets, sphere=sphere, dt=1.0e-3, steps=steps, density=1100.0,
                 friction=0.5, rel_tol=1.0e-6, abs_tol=1.0e-9)
+
+
+def config4_bodies(sizes, seed=11, cell=0.005):
+    """Tet4 cubes of k^3 cells (5 mm), one per entry of ``sizes``: bottom pinned, per-body E log-uniform in
+    [1e5, 1e7] Pa and Poisson ratio uniform in [0.3, 0.45], density 1000; each with its top-face vertices
+    and seeded targets N(rest, 1 mm) (the batched configuration's bodies)."""
+    rng = np.random.default_rng(seed)
+    young = np.exp(rng.uniform(np.log(1e5), np.log(1e7), len(sizes)))
+    poisson = rng.uniform(0.3, 0.45, len(sizes))
+    target_rng = np.random.default_rng(seed + 1)
+    bodies = []
+    for b, k in enumerate(sizes):
+        v, t, p = meshes.kuhn_tet_grid(int(k), int(k), int(k), cell, origin=(0.1 * (b % 16), 0.1 * (b // 16), 0.0))
+        mu, lam = meshes.lame(young[b], poisson[b])
+        mats = np.zeros((len(t), 3), np.float32)
+        mats[:, 0], mats[:, 1] = mu, lam
+        top = np.flatnonzero(v[:, 2] == v[:, 2].max())
+        bodies.append(dict(k=int(k), v=v.astype(np.float32), t=t, p=p, m=mats, free=v[:, 2] > v[:, 2].min(),
+                           top=top, target=v[top] + target_rng.normal(0.0, 1e-3, (len(top), 3)),
+                           young=float(young[b]), poisson=float(poisson[b])))
+    return bodies
+
+
+def packed_bodies_model(bodies, device, requires_grad=True):
+    """One model holding every body (vertex and element offsets per body); returns (model, voff, eoff)."""
+    voff, eoff = [0], [0]
+    for b in bodies:
+        voff.append(voff[-1] + len(b["v"]))
+        eoff.append(eoff[-1] + len(b["t"]))
+    model = Model(
+        particle_q=np.concatenate([b["v"] for b in bodies]),
+        particle_qd=np.zeros((voff[-1], 3), np.float32),
+        particle_inv_mass=np.concatenate([b["free"] for b in bodies]).astype(np.float32),
+        particle_world=np.zeros(voff[-1], np.int32),
+        gravity=np.array([[0.0, 0.0, -9.81]]),
+        tet_indices=np.concatenate([b["t"] + voff[i] for i, b in enumerate(bodies)]),
+        tet_poses=np.concatenate([b["p"] for b in bodies]),
+        tet_materials=np.concatenate([b["m"] for b in bodies]),
+        device=device,
+        requires_grad=requires_grad,
+        body_vertex_offsets=voff,
+        body_element_offsets=eoff,
+    )
+    return model, voff, eoff
+
+
+def lame_parameter_gradients(g_mu_lambda, young, poisson):
+    """(dL/dlog E, dL/dnu) of one body with uniform (E, nu) from its per-element (mu, lambda) gradients."""
+    g = np.asarray(g_mu_lambda, np.float64).sum(axis=0)
+    mu = young / (2.0 * (1.0 + poisson))
+    lam = young * poisson / ((1.0 + poisson) * (1.0 - 2.0 * poisson))
+    dmu = -mu / (1.0 + poisson)
+    dlam = young * (1.0 + 2.0 * poisson**2) / ((1.0 + poisson) ** 2 * (1.0 - 2.0 * poisson) ** 2)
+    return g[0] * mu + g[1] * lam, g[0] * dmu + g[1] * dlam
