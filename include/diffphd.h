@@ -206,6 +206,21 @@ int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle
                   const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
                   float* gdirichlet, double* gactivations, void* stream);
 
+/* dphd_backward without device allocation. No counterpart in the reference:
+ * for callers that cannot call hipMalloc (tight serving loops, harnesses
+ * that forbid allocation). Same arguments, results and status bits as
+ * dphd_backward, whose first call allocates the adjoint's Jacobian, element
+ * blocks and assembled operator; here they live in a static 64 MiB device
+ * buffer per GPU (up to roughly 30,000 Tet4 or 6,000 Hex8 elements per
+ * solver; larger solvers get DPHD_EINVAL with the bytes they need), and the
+ * adjoint iterations are launched directly instead of through a graph.
+ * The buffer is shared by every solver on a GPU: static backward calls
+ * that overlap on one GPU race. Parameter gradients with a frictional
+ * boundary (whose Woodbury columns are allocated) give DPHD_ENOTIMPL. */
+int dphd_backward_static(dphd_solver* s, const double* checkpoint, const double* muscle_targets, const float* gq_out,
+                         const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
+                         float* gdirichlet, double* gactivations, void* stream);
+
 /* Status word (DP/solver.py:380-390): OR of all bodies since the last clear. */
 int dphd_read_status(dphd_solver* s, int clear, int* bits);
 /* Iterations of the last forward (adjoint = 0) or backward (adjoint = 1)

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -69,6 +70,41 @@ int grid_for(int items) {
   int g = std::max(1, std::min(items, 2048));
   if (g >= 8) g &= ~7;
   return g;
+}
+
+// Device scratch of dphd_backward_static, for callers that cannot allocate:
+// the adjoint's Jacobian, element blocks and assembled operator of one
+// backward (every value is written before it is read, so nothing needs
+// clearing). One array per GPU, held until the process ends, shared by every
+// solver on that GPU: overlapping static backward calls on one GPU race.
+constexpr size_t kScratchBytes = size_t{64} << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+// Device address of each GPU's copy of `scratch`, looked up once per device
+// with that device current (the host-side name is not a device address).
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double* scratch_ptr[kMaxDevices] = {};
+hipError_t scratch_status[kMaxDevices] = {};
+bool scratch_looked_up[kMaxDevices] = {};
+
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (scratch_looked_up[device]) return;
+  void* ptr = nullptr;
+  scratch_status[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  scratch_ptr[device] = scratch_status[device] == hipSuccess ? static_cast<double*>(ptr) : nullptr;
+  scratch_looked_up[device] = true;
+}
+
+double* scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) throw std::runtime_error("dphd_backward_static: device index out of range");
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_looked_up[device] || !scratch_ptr[device])
+    throw std::runtime_error(std::string("dphd_backward_static: no device address of the static scratch: ") +
+                             (scratch_looked_up[device] ? hipGetErrorString(scratch_status[device]) : "not looked up"));
+  return scratch_ptr[device];
 }
 
 }  // namespace
@@ -942,6 +978,24 @@ int run_loop(dphd_solver* s, int mode) {
   return launched;
 }
 
+// The adjoint loop of dphd_backward_static: the kernels of the captured
+// adjoint graph launched directly (a graph instantiation may allocate), a
+// chunk of iterations at a time, polling the mapped flag after each chunk.
+void run_adjoint_direct(dphd_solver* s, const DeviceState& d) {
+  StepScalars sc = scalars(s);
+  sc.max_iterations = loop_limit(s, kAdjoint);
+  sc.report = 1;
+  const int C = chunk_size(s);
+  const int max_chunks = (sc.max_iterations + C - 1) / C;
+  for (int launched = 0; launched < max_chunks; ++launched) {
+    for (int c = 0; c < C; ++c) launch_adjoint_iteration(d, sc, s->stream);
+    launch_any_active(d, d.aactive, s->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    if (!*reinterpret_cast<volatile int*>(s->any_host)) break;
+  }
+}
+
 // Woodbury projection of the adjoint for the parameter gradients; returns z
 // (Adjoint._woodbury, adjoint.py:1272-1354). d.sol holds mu and d.arhs the
 // seed on entry and again on return.
@@ -1029,6 +1083,7 @@ int dphd_create(const dphd_config* config, int device, dphd_solver** out) {
   }
   DPHD_TRY({
     HIP_CHECK(hipSetDevice(device));
+    dphd::scratch_setup(device);
     auto* s = new dphd_solver;
     s->cfg = c;
     s->device = device;
@@ -1467,6 +1522,64 @@ int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle
     if ((gmaterials || gactivations) && s->has_friction && s->fc.K > 0) z = dphd::woodbury(s);
     dphd::launch_adjoint_outputs(s->d, sc, checkpoint, gqd_out, gq_in, gqd_in, gf_in, gmaterials, gactivations,
                                  gdirichlet, z, s->any_pinned, s->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(s->ev_out, s->stream));
+    HIP_CHECK(hipStreamWaitEvent(user, s->ev_out, 0));
+  })
+}
+
+int dphd_backward_static(dphd_solver* s, const double* checkpoint, const double* muscle_targets, const float* gq_out,
+                         const float* gqd_out, float* gq_in, float* gqd_in, float* gf_in, double* gmaterials,
+                         float* gdirichlet, double* gactivations, void* stream) {
+  if (!s || !s->built) {
+    set_error("dphd_backward_static: call dphd_build first");
+    return DPHD_ESTATE;
+  }
+  if (!checkpoint || !gq_in || !gqd_in || !gf_in) {
+    set_error("dphd_backward_static: null pointer");
+    return DPHD_EINVAL;
+  }
+  if (!s->planar_idx.empty()) {
+    set_error("dphd_backward_static: planar vertex energies have no adjoint (the reference records none for Hex8)");
+    return DPHD_ENOTIMPL;
+  }
+  if (gactivations && (s->Q != 1 || !s->d.has_muscles)) {
+    set_error("dphd_backward_static: activation gradients need Tet4 elements with muscle fibres");
+    return DPHD_EINVAL;
+  }
+  if ((gmaterials || gactivations) && s->has_friction && s->fc.K > 0) {
+    set_error("dphd_backward_static: the Woodbury projection of parameter gradients with a frictional boundary "
+              "allocates its columns; use dphd_backward");
+    return DPHD_ENOTIMPL;
+  }
+  DPHD_TRY({
+    HIP_CHECK(hipSetDevice(s->device));
+    // Jacobian, element blocks and assembled operator, each from a 256-byte boundary.
+    auto round = [](size_t count) { return (count + 31) / 32 * 32; };
+    const size_t jac_n = round(static_cast<size_t>(s->E) * s->Q * 81);
+    const size_t ke_n = round(static_cast<size_t>(s->E) * s->K * s->K * 9);
+    const size_t kval_n = round(static_cast<size_t>(s->d.kblocks) * 9);
+    const size_t need = (jac_n + ke_n + kval_n) * sizeof(double);
+    if (need > dphd::kScratchBytes)
+      throw std::invalid_argument("dphd_backward_static: this solver's adjoint needs " + std::to_string(need) +
+                                  " bytes of scratch (Jacobian, element blocks, assembled operator), above the " +
+                                  std::to_string(dphd::kScratchBytes) + "-byte static buffer; use dphd_backward");
+    double* base = dphd::scratch_address(s->device);
+    dphd::DeviceState d = s->d;
+    d.jac = base;
+    d.ke = base + jac_n;
+    d.kval = base + jac_n + ke_n;
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    HIP_CHECK(hipEventRecord(s->ev_in, user));
+    HIP_CHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+    if (s->d.has_muscles && muscle_targets)
+      HIP_CHECK(hipMemcpyAsync(s->targets, muscle_targets, sizeof(double) * s->E, hipMemcpyDeviceToDevice, s->stream));
+    const dphd::StepScalars sc = dphd::scalars(s);
+    dphd::launch_adjoint_setup(d, sc, checkpoint, s->targets, gq_out, gqd_out, s->stream);
+    dphd::run_adjoint_direct(s, d);
+    dphd::launch_adjoint_route(d, s->stream);
+    dphd::launch_adjoint_outputs(d, sc, checkpoint, gqd_out, gq_in, gqd_in, gf_in, gmaterials, gactivations,
+                                 gdirichlet, nullptr, s->any_pinned, s->stream);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(s->ev_out, s->stream));
     HIP_CHECK(hipStreamWaitEvent(user, s->ev_out, 0));

@@ -90,6 +90,7 @@ SIGNATURES = {
     "dphd_body_nnz": ([_P, _P], _I),
     "dphd_step": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "dphd_backward": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "dphd_backward_static": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "dphd_read_status": ([_P, _I, ctypes.POINTER(_I)], _I),
     "dphd_iterations": ([_P, _I, _P], _I),
     "dphd_synchronize": ([_P], _I),
