@@ -146,54 +146,92 @@ def environment(device):
     return info
 
 
-def check(fixture):
-    """Reasons the fixture cannot be run; empty when it can (compare.py:63-101, plus this solver's
-    own limits: a frictional boundary needs one body, which a fixture always is)."""
-    fx = fixture
-    reasons = []
-    if str(_value(fx.get("schema", ""))) != SCHEMA:
-        reasons.append("schema")
+def _exact_f32(values):
+    a = np.asarray(values, dtype=np.float64)
+    return np.array_equal(a, a.astype(np.float32).astype(np.float64))
+
+
+def _tet_rows(fx):
     tets = np.asarray(fx["tets"])
+    return tets.ndim == 2 and tets.shape[1] == 4
+
+
+def _tets_in_range(fx):
+    tets = np.asarray(fx["tets"])
+    return len(tets) == 0 or (tets.min() >= 0 and tets.max() < len(np.asarray(fx["rest_vertices"])))
+
+
+def _positively_oriented(fx):
     rest = np.asarray(fx["rest_vertices"], dtype=np.float64)
-    if tets.ndim != 2 or tets.shape[1] != 4:
-        reasons.append("only Tet4 topology")
-    elif len(tets) and (tets.min() < 0 or tets.max() >= len(rest)):
-        reasons.append("tet indices out of range")
-    else:
-        p = rest[tets]
-        dm = np.stack([p[:, j] - p[:, 0] for j in (1, 2, 3)], axis=-1)
-        if np.any(np.linalg.det(dm) <= 0):
-            reasons.append("tetrahedra must be positively oriented")
-    if str(_value(fx["material"])) not in ("pd", "neo_hookean"):
-        reasons.append("material")
-    if str(_value(fx["contact_kind"])) not in ("none", "planar", "spherical", "mesh"):
-        reasons.append("contact kind")
-    if "mesh_contact_parameters" in fx:
-        parameters = np.asarray(fx["mesh_contact_parameters"], dtype=np.float64)
-        faces = [np.asarray(fx["mesh_contact_faces_a"]), np.asarray(fx["mesh_contact_faces_b"])]
-        if parameters.shape != (5,) or not np.all(np.isfinite(parameters)) or np.any(parameters < 0):
-            reasons.append("mesh_contact parameters (contact_radius, kn_left, kn_right, kf, mu)")
-        if any(f.ndim != 2 or f.shape[1] != 3 or not len(f) or f.min() < 0 or f.max() >= len(fx["q"]) for f in faces):
-            reasons.append("mesh_contact faces")
-    if "activations" in fx and "a = 1 - u" not in str(_value(fx.get("activation_convention", ""))):
-        reasons.append("activation convention")
-    steps = int(_value(fx["steps"]))
-    if "dirichlet_positions" in fx:
-        targets = np.asarray(fx["dirichlet_positions"], dtype=np.float64)
-        if targets.shape != (steps, len(fx["q"]), 3) or not np.all(np.isfinite(targets)):
-            reasons.append("dirichlet_positions shape or finite values")
-        elif not np.array_equal(targets, targets.astype(np.float32).astype(np.float64)):
-            reasons.append("dirichlet_positions is not float32-representable")
-    if "velocity_damping" in fx:
-        damping = np.asarray(fx["velocity_damping"], dtype=np.float64)
-        if damping.size != 1 or not np.all(np.isfinite(damping)) or not np.all((0 <= damping) & (damping <= 1)):
-            reasons.append("velocity_damping must be a finite scalar in [0,1]")
-    for key in FLOAT32_KEYS:
-        a = np.asarray(fx[key], dtype=np.float64)
-        if not np.array_equal(a, a.astype(np.float32).astype(np.float64)):
-            reasons.append(f"{key} is not float32-representable (Newton stores float32)")
-    if np.asarray(fx["f_ext"]).shape != (steps, len(fx["q"]), 3):
-        reasons.append("f_ext must be (steps, N, 3)")
+    tets = np.asarray(fx["tets"])
+    edges = rest[tets[:, 1:]] - rest[tets[:, :1]]  # [E, 3 edges, 3]
+    return bool(np.all(np.linalg.det(edges) > 0.0))
+
+
+def _mesh_contact_faces_ok(fx):
+    n = len(np.asarray(fx["q"]))
+    for key in ("mesh_contact_faces_a", "mesh_contact_faces_b"):
+        faces = np.asarray(fx[key])
+        if faces.ndim != 2 or faces.shape[1] != 3 or len(faces) == 0 or faces.min() < 0 or faces.max() >= n:
+            return False
+    return True
+
+
+def _per_step(fx, key):
+    return np.asarray(fx[key]).shape == (int(_value(fx["steps"])), len(np.asarray(fx["q"])), 3)
+
+
+def _nonnegative_finite(values, shape):
+    a = np.asarray(values, dtype=np.float64)
+    return a.shape == shape and bool(np.all(np.isfinite(a))) and bool(np.all(a >= 0.0))
+
+
+MATERIALS = ("pd", "neo_hookean")
+
+# What a fixture of schema newton.diff_phd_parity.v1 must satisfy before it is
+# run (docs/validation.md:451-472), with what this solver runs (Tet4, "pd" or
+# "neo_hookean", the four contact kinds). Each rule: the reason reported when
+# it fails, the keys that make it apply (an optional input is checked only
+# when present), the rules that must pass first, and the predicate.
+_RULES = [
+    ("schema", "schema", (), (), lambda fx: str(_value(fx.get("schema", ""))) == SCHEMA),
+    ("tet_rows", "only Tet4 topology", ("tets",), (), _tet_rows),
+    ("tet_range", "tet indices out of range", ("tets", "rest_vertices"), ("tet_rows",), _tets_in_range),
+    ("orientation", "tetrahedra must be positively oriented", ("tets", "rest_vertices"), ("tet_range",),
+     _positively_oriented),
+    ("material", "material", ("material",), (), lambda fx: str(_value(fx["material"])) in MATERIALS),
+    ("contact", "contact kind", ("contact_kind",), (),
+     lambda fx: str(_value(fx["contact_kind"])) in ("none", "planar", "spherical", "mesh")),
+    ("mesh_params", "mesh_contact parameters (contact_radius, kn_left, kn_right, kf, mu)",
+     ("mesh_contact_parameters",), (), lambda fx: _nonnegative_finite(fx["mesh_contact_parameters"], (5,))),
+    ("mesh_faces", "mesh_contact faces", ("mesh_contact_parameters",), (), _mesh_contact_faces_ok),
+    ("convention", "activation convention", ("activations",), (),
+     lambda fx: "a = 1 - u" in str(_value(fx.get("activation_convention", "")))),
+    ("targets", "dirichlet_positions shape or finite values", ("dirichlet_positions",), (),
+     lambda fx: _per_step(fx, "dirichlet_positions") and bool(np.all(np.isfinite(fx["dirichlet_positions"])))),
+    ("targets_f32", "dirichlet_positions is not float32-representable", ("dirichlet_positions",), ("targets",),
+     lambda fx: _exact_f32(fx["dirichlet_positions"])),
+    ("damping", "velocity_damping must be a finite scalar in [0,1]", ("velocity_damping",), (),
+     lambda fx: np.size(fx["velocity_damping"]) == 1
+     and 0.0 <= float(np.asarray(fx["velocity_damping"], dtype=np.float64).reshape(-1)[0]) <= 1.0),
+    ("forces", "f_ext must be (steps, N, 3)", ("f_ext",), (), lambda fx: _per_step(fx, "f_ext")),
+] + [
+    (f"{key}_f32", f"{key} is not float32-representable (Newton stores float32)", (key,), (),
+     lambda fx, key=key: _exact_f32(fx[key]))
+    for key in FLOAT32_KEYS
+]
+
+
+def check(fixture):
+    """Reasons the fixture cannot be run (the schema's rules above, in order); empty when it can."""
+    passed, reasons = set(), []
+    for name, reason, keys, after, ok in _RULES:
+        if not all(k in fixture for k in keys) or not all(a in passed for a in after):
+            continue
+        if ok(fixture):
+            passed.add(name)
+        else:
+            reasons.append(reason)
     return reasons
 
 

@@ -57,7 +57,10 @@ def test_fixture_outputs_with_the_fp32_factor(cuda, name):
     diff = np.abs(np.asarray(port["iterations"]) - np.asarray(ref["iterations"]))
     print(f"[result] fixture {name} fp32 factor: largest iteration difference {int(diff.max())} "
           f"(oracle max {int(np.max(ref['iterations']))})", flush=True)
+    # Every status bit, the adjoint's (8 not converged, 16 safeguard) included, as the oracle's.
+    np.testing.assert_array_equal(port["status_bits"], ref["status_bits"])
     assert abs(port["loss"] - ref["loss"]) <= 1e-6 * max(1.0, abs(ref["loss"]))
+
 
 def test_command_line_writes_fixture_and_port_files(cuda, tmp_path):
     print("[case] parity command line", flush=True)

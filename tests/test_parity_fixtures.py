@@ -50,6 +50,16 @@ def test_schema_checks_reject_what_the_harness_rejects():
         "q is not float32-representable": {"q": fx["q"] + 1e-12},
         "activation convention": {"activations": np.zeros((2, len(fx["tets"]))), "activation_convention": "a = u"},
         "velocity_damping": {"velocity_damping": 1.5},
+        "f_ext must be (steps, N, 3)": {"f_ext": fx["f_ext"][:1]},
+        "dirichlet_positions shape": {"dirichlet_positions": np.zeros((1, 2, 3))},
+        "dirichlet_positions is not float32-representable": {
+            "dirichlet_positions": np.broadcast_to(fx["q"], (int(fx["steps"]),) + fx["q"].shape) + 1e-12},
+        "mesh_contact parameters": {"mesh_contact_parameters": np.array([1e-3, -1.0, 0.0, 0.0, 0.0]),
+                                    "mesh_contact_faces_a": fx["tets"][:, :3], "mesh_contact_faces_b": fx["tets"][:, 1:]},
+        "mesh_contact faces": {"mesh_contact_parameters": np.zeros(5), "mesh_contact_faces_a": fx["tets"][:, :3] + 10**6,
+                               "mesh_contact_faces_b": fx["tets"][:, 1:]},
+        "tet indices out of range": {"tets": fx["tets"] + len(fx["rest_vertices"])},
+        "only Tet4 topology": {"tets": fx["tets"][:, :3]},
     }
     for reason, change in cases.items():
         reasons = parity.check({**fx, **change})
