@@ -13,8 +13,13 @@ through them, all bodies in one batched solver (`value`, steps/s), and the
 same bodies as 256 independent single-body solvers run one after another
 (the plain per-body GPU path). Roofline: the factor products S and S^T
 of all bodies (A^{-1} = S^T S, the dominant kernels), timed live with HIP
-events; algorithmic bytes count the reference's CSR form. CPU baseline: the
-FP64 C oracle, one step of every body on the host cores.
+events; algorithmic bytes count the reference's CSR form. Gradient check:
+one body of every distinct size against single-body FP64 solves of the same
+rollout at a tight gate, at the metric's tolerance and at a gradient-
+accurate one (relative 1e-10, absolute 0), whose batched and plain rates
+are reported too. CPU baseline: the FP64 C oracle on the host cores, the
+same rollout of one body per distinct size, weighted by the bodies of that
+size.
 """
 
 from __future__ import annotations
@@ -62,7 +67,8 @@ def make_bodies(count, kmin, kmax, cell=0.005):
         free = v[:, 2] > v[:, 2].min()
         top = np.flatnonzero(v[:, 2] == v[:, 2].max())
         target = v[top] + target_rng.normal(0.0, 1e-3, (len(top), 3))
-        bodies.append(dict(k=int(k), v=v.astype(np.float32), t=t, p=p, m=mats, free=free, top=top, target=target))
+        bodies.append(dict(k=int(k), v=v.astype(np.float32), t=t, p=p, m=mats, free=free, top=top, target=target,
+                           young=float(young[b]), poisson=float(poisson[b])))
     return bodies
 
 
@@ -191,6 +197,116 @@ def copy_ceiling(dev):
     return gbps
 
 
+def stratified_sample(bodies):
+    """One body of every distinct size k (the first of each), and how many bodies each stands for."""
+    first, count = {}, {}
+    for i, b in enumerate(bodies):
+        first.setdefault(b["k"], i)
+        count[b["k"]] = count.get(b["k"], 0) + 1
+    ks = sorted(first)
+    return [first[k] for k in ks], [count[k] for k in ks]
+
+
+def body_gradients(b, gmat, gq, gqd):
+    """(dL/dlog E, dL/dnu) of a body with uniform E and Poisson ratio, and its q0 / v0 gradients."""
+    g = np.asarray(gmat, np.float64)[:, :2].sum(axis=0)
+    young, nu = b["young"], b["poisson"]
+    mu = young / (2.0 * (1.0 + nu))
+    lam = young * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    dmu = -mu / (1.0 + nu)
+    dlam = young * (1.0 + 2.0 * nu**2) / ((1.0 + nu) ** 2 * (1.0 - 2.0 * nu) ** 2)
+    return {"log_e": float(g[0] * mu + g[1] * lam), "nu": float(g[0] * dmu + g[1] * dlam),
+            "q0": np.asarray(gq, np.float64), "v0": np.asarray(gqd, np.float64)}
+
+
+def per_body_gradients(model, q0, qd0, voff, eoff, bodies, pick):
+    gmat = model.tet_materials.grad.detach().cpu().numpy()
+    gq = q0.grad.detach().cpu().numpy()
+    gqd = qd0.grad.detach().cpu().numpy()
+    return {i: body_gradients(bodies[i], gmat[eoff[i]:eoff[i + 1]], gq[voff[i]:voff[i + 1]], gqd[voff[i]:voff[i + 1]])
+            for i in pick}
+
+
+def gradient_errors(got, ref):
+    """Relative error of each gradient against the reference over the sampled bodies: ``vector`` the L2 error
+    of all sampled bodies' values together (log E and nu one entry per body), ``worst_body`` the largest
+    per-body relative error (|a - b| / |b| for the scalars, which are tiny for nearly static stiff bodies)."""
+    out = {}
+    for key in ("log_e", "nu", "q0", "v0"):
+        a = np.concatenate([np.ravel(got[i][key]) for i in ref])
+        b = np.concatenate([np.ravel(ref[i][key]) for i in ref])
+        worst = max(float(np.linalg.norm(np.ravel(got[i][key]) - np.ravel(ref[i][key])) /
+                          max(np.linalg.norm(np.ravel(ref[i][key])), 1e-300)) for i in ref)
+        out[key] = {"vector": float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)), "worst_body": worst}
+    out["max_vector"] = max(v["vector"] for v in out.values())
+    return out
+
+
+def single_body_rollouts(bodies, pick, dev, steps, dt, **solver_kw):
+    """Each picked body in a single-body solver (the plain per-body GPU path): its gradients and rollout time."""
+    import torch
+
+    from diffphd_amd import SolverDiffPhD
+
+    grads, seconds, status = {}, {}, 0
+    for i in pick:
+        b = bodies[i]
+        m1, v1, e1 = packed_model([b], dev)
+        s1 = SolverDiffPhD(m1, density=1000.0, anderson_window=5, **solver_kw)
+        s1.prepare(dt)
+        i1, t1 = loss_index([b], v1, dev)
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        _loss, q0, qd0 = rollout(s1, m1, steps, dt, i1, t1)
+        torch.cuda.synchronize()
+        seconds[i] = time.perf_counter() - start
+        status |= s1.read_status()
+        grads[i] = per_body_gradients(m1, q0, qd0, v1, e1, [b], [0])[0]
+        del s1
+    return grads, seconds, status
+
+
+def timed_batched(model, dev, dt, idx, tgt, args, rel_tol, abs_tol):
+    """Batched solver of every body: warm-up steps, then the timed K-step rollout; a further untimed rollout
+    logs the iteration counts and loop counters (logging synchronizes, so it stays out of the timed one)."""
+    import torch
+
+    from diffphd_amd import SolverDiffPhD
+
+    t0 = time.time()
+    solver = SolverDiffPhD(
+        model, density=1000.0, rel_tol=rel_tol, abs_tol=abs_tol, anderson_window=5, independent_bodies=True,
+        factor_precision=args.precision, iteration_chunk=args.chunk,
+    )
+    solver.prepare(dt)
+    setup = time.time() - t0
+    for _ in range(args.warmup):
+        rollout(solver, model, 1, dt, idx, tgt)
+    torch.cuda.synchronize()
+    solver.read_status()
+    model.tet_materials.grad = None
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    loss, q0, qd0 = rollout(solver, model, args.steps, dt, idx, tgt)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    status = solver.read_status()
+    grads_model = model.tet_materials.grad.detach().clone()
+    before = solver.loop_stats()
+    solver.iteration_log = []
+    model.tet_materials.grad = None
+    rollout(solver, model, args.steps, dt, idx, tgt)
+    log_ = solver.iteration_log
+    solver.iteration_log = None
+    after = solver.loop_stats()
+    model.tet_materials.grad = grads_model
+    fwd = np.array([it for kind, it in log_ if kind == "forward"])
+    adj = np.array([it for kind, it in log_ if kind == "adjoint"])
+    loops = {m: {k: (after[m][k] - before[m][k]) / args.steps for k in after[m]} for m in after}
+    return dict(solver=solver, setup=setup, elapsed=elapsed, status=status, loss=float(loss.detach()), q0=q0,
+                qd0=qd0, fwd=fwd, adj=adj, loops=loops)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,7 +321,9 @@ def main():
     ap.add_argument("--plain-bodies", type=int, default=None, help="bodies timed on the plain path (default all)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-bodies", type=int, default=0, help="0 = every body")
+    ap.add_argument("--no-gradient-check", action="store_true")
+    ap.add_argument("--accurate-rel", type=float, default=1e-10,
+                    help="relative tolerance (absolute 0) of the gradient-accurate setting; 0 skips it")
     ap.add_argument("--probe-reps", type=int, default=20)
     ap.add_argument("--probe-only", action="store_true",
                     help="only the factor-product probe and the copy calibration (for counter runs)")
@@ -229,41 +347,25 @@ def main():
     model, voff, eoff = packed_model(bodies, dev)
     idx, tgt = loss_index(bodies, voff, dev)
     n_total, e_total = voff[-1], eoff[-1]
-    log(f"[bench] {len(bodies)} bodies, {n_total} vertices, {e_total} tets")
-
-    t0 = time.time()
-    solver = SolverDiffPhD(
-        model, density=1000.0, rel_tol=1e-6, anderson_window=5, independent_bodies=True,
-        factor_precision=args.precision, iteration_chunk=args.chunk,
-    )
-    solver.prepare(dt)
-    log(f"[bench] batched setup {time.time() - t0:.1f}s, nnz(S) {solver.info.factor_nnz}")
     s_bytes = 8 if args.precision == 64 else 4
+    log(f"[bench] {len(bodies)} bodies, {n_total} vertices, {e_total} tets")
     if args.probe_only:
+        solver = SolverDiffPhD(model, density=1000.0, rel_tol=1e-6, anderson_window=5, independent_bodies=True,
+                               factor_precision=args.precision, iteration_chunk=args.chunk)
+        solver.prepare(dt)
         z_all = float(np.sum(solver.body_factor_nnz()))
         probe = factor_probe(solver, dev, n_total, z_all, s_bytes, args)
         copy_gbps = copy_ceiling(dev)
         print(json.dumps({"probe_launch_us": probe["launch_s"] * 1e6, "copy_ceiling_gbps": copy_gbps}))
         return
 
-    for _ in range(args.warmup):
-        rollout(solver, model, 1, dt, idx, tgt)
-    torch.cuda.synchronize()
-    solver.read_status()
-    model.tet_materials.grad = None
-
-    solver.iteration_log = []
-    torch.cuda.synchronize()
-    start = time.perf_counter()
-    loss, q0, qd0 = rollout(solver, model, args.steps, dt, idx, tgt)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - start
-    status = solver.read_status()
-    log(f"[bench] batched rollout {elapsed:.3f}s for {args.steps} fwd+adj steps, loss {float(loss.detach()):.6e}, status {status}")
-    fwd_iters = np.array([it for kind, it in solver.iteration_log if kind == "forward"])
-    adj_iters = np.array([it for kind, it in solver.iteration_log if kind == "adjoint"])
-    solver.iteration_log = None
-    grad_ok = bool(torch.isfinite(model.tet_materials.grad).all()) and bool(torch.isfinite(qd0.grad).all())
+    run = timed_batched(model, dev, dt, idx, tgt, args, 1e-6, 1e-9)
+    solver, elapsed = run["solver"], run["elapsed"]
+    log(f"[bench] batched setup {run['setup']:.1f}s, nnz(S) {solver.info.factor_nnz}; rollout {elapsed:.3f}s for "
+        f"{args.steps} fwd+adj steps, loss {run['loss']:.6e}, status {run['status']}, loops/step {run['loops']}")
+    fwd_iters, adj_iters = run["fwd"], run["adj"]
+    grad_ok = bool(torch.isfinite(model.tet_materials.grad).all()) and bool(torch.isfinite(run["qd0"].grad).all())
+    pick, weight = stratified_sample(bodies)
 
     # Algorithmic bytes of the rollout from the per-body iteration counts.
     n_b = np.diff(voff)
@@ -278,6 +380,50 @@ def main():
 
     probe = factor_probe(solver, dev, n_total, z_total, s_bytes, args)
     copy_gbps = copy_ceiling(dev)
+
+    # Gradient accuracy: the sampled bodies' gradients against single-body FP64 solves at a tight gate
+    # (relative 1e-12, absolute 0) of the same rollout.
+    grad_check = None
+    accurate = None
+    if not args.no_gradient_check:
+        batched_grads = per_body_gradients(model, run["q0"], run["qd0"], voff, eoff, bodies, pick)
+        ref, _sec, ref_status = single_body_rollouts(bodies, pick, dev, args.steps, dt, rel_tol=1e-12, abs_tol=0.0,
+                                                     max_iterations=20000, factor_precision=64)
+        err = gradient_errors(batched_grads, ref)
+        grad_check = {
+            "reference": "single-body FP64 solves of the same rollout at rel 1e-12, abs 0 (status "
+                         f"{ref_status}); bodies {[bodies[i]['k'] for i in pick]} (k)",
+            "rel_err_at_metric_tolerance": err,
+            "target": 1e-4,
+            "target_met_at_metric_tolerance": bool(err["max_vector"] <= 1e-4),
+        }
+        log(f"[bench] gradient error at rel 1e-6 / abs 1e-9 vs the FP64 tight solves: {err}")
+        if args.accurate_rel > 0:
+            del solver
+            run_a = timed_batched(model, dev, dt, idx, tgt, args, args.accurate_rel, 0.0)
+            acc_grads = per_body_gradients(model, run_a["q0"], run_a["qd0"], voff, eoff, bodies, pick)
+            err_a = gradient_errors(acc_grads, ref)
+            _g, sec_a, _st = single_body_rollouts(bodies, pick, dev, args.steps, dt, rel_tol=args.accurate_rel,
+                                                 abs_tol=0.0, factor_precision=args.precision)
+            plain_a = args.steps / sum(sec_a[i] * w for i, w in zip(pick, weight))
+            value_a = args.steps / run_a["elapsed"]
+            accurate = {
+                "rel_tol": args.accurate_rel,
+                "abs_tol": 0.0,
+                "steps_per_s": value_a,
+                "status": run_a["status"],
+                "plain_steps_per_s_estimate": plain_a,
+                "plain_estimate": "one single-body solve per distinct k, weighted by the bodies of that k",
+                "speedup_vs_plain": value_a / plain_a,
+                "gradient_rel_err": err_a,
+                "target_met": bool(err_a["max_vector"] <= 1e-4),
+                "iterations": {"forward_mean": float(run_a["fwd"].mean()), "adjoint_mean": float(run_a["adj"].mean()),
+                               "adjoint_max": int(run_a["adj"].max())},
+            }
+            grad_check["accurate_setting"] = accurate
+            log(f"[bench] accurate setting rel {args.accurate_rel:g} abs 0: {value_a:.2f} steps/s, plain estimate "
+                f"{plain_a:.3f}, gradient error {err_a}")
+            del run_a
 
     # Plain per-body path: one single-body solver per body, run one after another.
     plain = None
@@ -308,7 +454,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(bodies, z_b, dt, args.cpu_sample_bodies, solver)
+        cpu = cpu_baseline(bodies, pick, weight, dt, args.steps)
 
     # HBM traffic per launch of the probe kernels from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs (calibrated against a plain copy kernel).
@@ -318,6 +464,7 @@ def main():
         traffic = json.loads(traffic_file.read_text())["traffic_bytes_per_launch"]
 
     value = args.steps / elapsed
+    achieved = probe["launch_bytes"] / probe["launch_s"] / 1e9
     result = {
         "metric": METRIC,
         "value": value,
@@ -349,43 +496,55 @@ def main():
             "adjoint_mean": float(adj_iters.mean()),
             "adjoint_max": int(adj_iters.max()),
         },
+        "loops_per_step": run["loops"],
         "rollout_hbm_gbps": rollout_gbps,
-        "status": status,
+        "status": run["status"],
         "gradients_finite": grad_ok,
+        "gradient_check": grad_check,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_factor_rows + k_factor_cols (S and S^T products of all bodies), per launch",
-            "achieved": probe["launch_bytes"] / probe["launch_s"] / 1e9,
+            "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
-            "frac": probe["launch_bytes"] / probe["launch_s"] / 1e9 / HBM_PEAK_GBPS,
+            "frac": achieved / HBM_PEAK_GBPS,
+            "frac_basis": "algorithmic bytes: the reference's CSR form (values + 4-byte column indices + offsets "
+                          "+ vectors); the block layout reads no column indices",
             "traffic": traffic,
+            "frac_traffic": None if traffic is None else traffic / probe["launch_s"] / 1e9 / HBM_PEAK_GBPS,
             "launch_us": probe["launch_s"] * 1e6,
             "bytes_per_launch": probe["launch_bytes"],
             "stored_bytes_per_launch": probe["stored_bytes"],
             "stored_gbps": probe["stored_bytes"] / probe["launch_s"] / 1e9,
+            "frac_stored": probe["stored_bytes"] / probe["launch_s"] / 1e9 / HBM_PEAK_GBPS,
             "copy_ceiling_gbps": copy_gbps,
-            "frac_of_copy": probe["launch_bytes"] / probe["launch_s"] / 1e9 / copy_gbps,
+            "frac_of_copy": achieved / copy_gbps,
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(result))
 
 
-def cpu_baseline(bodies, z_b, dt, sample_count, solver):
-    """FP64 C oracle on the host cores: one fwd+adj step from rest of every body (or of a sample scaled by
-    factor size to the whole batch)."""
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(bodies, pick, weight, dt, steps):
+    """FP64 C oracle on the host cores: the timed rollout (K steps from rest, then the backward sweep of the
+    top-face loss) of one body of every distinct size, weighted by the bodies of that size."""
     from diffphd_amd import _lib
     from oracle import oracle
 
     threads = min(16, os.cpu_count() or 1)
-    if sample_count <= 0 or sample_count >= len(bodies):
-        pick = np.arange(len(bodies))
-    else:
-        order = np.argsort([b["k"] for b in bodies])
-        pick = order[np.linspace(0, len(order) - 1, sample_count).astype(int)]
+    weighted = 0.0
     seconds = 0.0
-    for i in pick:
+    for i, w in zip(pick, weight):
         b = bodies[i]
         body = oracle.OracleBody(
             "tet4", b["t"], poses=b["p"], materials=b["m"], density=1000.0, inverse_mass=b["free"].astype(np.float32),
@@ -395,20 +554,29 @@ def cpu_baseline(bodies, z_b, dt, sample_count, solver):
         q = b["v"].astype(np.float32)
         qd = np.zeros_like(q)
         t0 = time.perf_counter()
-        fw = body.forward(dt, q, qd)
+        records = []
+        for _ in range(steps):
+            fw = body.forward(dt, q, qd)
+            records.append(fw)
+            q, qd = fw["q"], fw["qd"]
         gq = np.zeros_like(q)
-        gq[b["top"]] = 2.0 * (fw["q"][b["top"]] - b["target"])
-        body.backward(dt, fw["x"], gq, None)
-        seconds += time.perf_counter() - t0
-    scale = float(z_b.sum() / z_b[pick].sum())
-    per_step = seconds * scale
-    what = "all" if len(pick) == len(bodies) else f"{len(pick)} (k={[bodies[i]['k'] for i in pick]}), scaled by nnz(S) x{scale:.1f}"
+        gq[b["top"]] = 2.0 * (q[b["top"]] - b["target"])
+        gqd = np.zeros_like(q)
+        for fw in reversed(records):
+            bw = body.backward(dt, fw["x"], gq, gqd)
+            gq, gqd = bw["q"], bw["qd"]
+        t = time.perf_counter() - t0
+        seconds += t
+        weighted += w * t
     return {
-        "value": 1.0 / per_step,
+        "value": steps / weighted,
         "unit": "steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{what} of {len(bodies)} bodies, one fwd+adj step each from rest, {seconds:.1f} s of oracle time",
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(),
+        "sample": f"the timed rollout ({steps} fwd steps from rest + backward sweep) of one body per distinct k "
+                  f"({len(pick)} bodies), each weighted by the bodies of its k; {seconds:.1f} s of oracle time",
     }
 
 

@@ -183,6 +183,10 @@ int dphd_build(dphd_solver* s, double dt);
 int dphd_get_info(const dphd_solver* s, dphd_info* info);
 /* nnz(S) of every body after dphd_build (per_body [B]); sizes the byte model. */
 int dphd_body_nnz(const dphd_solver* s, int64_t* per_body);
+/* Loop counters since the last dphd_build (diagnostics; no reference
+ * counterpart): stats[6] = for the forward, adjoint and Woodbury loops in
+ * turn, the iteration chunks launched and the host waits between them. */
+int dphd_loop_stats(const dphd_solver* s, int64_t* stats);
 
 /* One forward step of all bodies (DP/solver.py:689-786). q/qd/f are [N,3]
  * float32 device arrays; dirichlet is [N,3] float32 or NULL

@@ -161,9 +161,10 @@ struct WoodburyDev {
 // guard's (scale, diverged) of body 0.
 void launch_woodbury_rows(const WoodburyDev& w, const double* mu, const double* route, hipStream_t stream);
 // Seed the adjoint solve of the k-th selected row's column: rhs = J_r^T,
-// solution zero, iteration state of body 0 reset.
-void launch_woodbury_seed(const WoodburyDev& w, int k, double* rhs, double* sol, int* aiter, int* aactive, int* aa,
-                          hipStream_t stream);
+// solution (sol, and the mixed iterate xr with its clip scale 1) zero,
+// iteration state of body 0 reset.
+void launch_woodbury_seed(const WoodburyDev& w, int k, double* rhs, double* sol, double* xr, double* fscale,
+                          int* aiter, int* aactive, int* aa, hipStream_t stream);
 // G alpha = J mu and z = mu - sum_k alpha_k column_k, then the safe clip of z.
 void launch_woodbury_project(const WoodburyDev& w, const double* mu, const uint8_t* free, double* z, int* status,
                              hipStream_t stream);

@@ -991,19 +991,21 @@ __global__ __launch_bounds__(256) void k_wb_rows(WoodburyDev w, const double* mu
   if (tid == 0) *w.count = s_base;
 }
 
-__global__ __launch_bounds__(256) void k_wb_seed(WoodburyDev w, int k, double* rhs, double* sol, int* aiter,
-                                                 int* aactive, int* aa) {
+__global__ __launch_bounds__(256) void k_wb_seed(WoodburyDev w, int k, double* rhs, double* sol, double* xr,
+                                                 double* fscale, int* aiter, int* aactive, int* aa) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     *aiter = 0;
     *aactive = 1;
     aa[0] = aa[1] = 0;
+    *fscale = 1.0;
   }
   if (i >= w.n) return;
   const int r = w.order[k];
   const V3 v = i == w.row_node[r] ? ld3(w.row_vec, r) : V3{0, 0, 0};
   st3(rhs, i, v);
   st3(sol, i, V3{0, 0, 0});
+  st3(xr, i, V3{0, 0, 0});
 }
 
 // _gram (adjoint.py:404-425) and the Eigen LDLT of G over the selected rows.
@@ -1143,9 +1145,10 @@ void launch_woodbury_rows(const WoodburyDev& w, const double* mu, const double* 
   hipLaunchKernelGGL(k_wb_rows, dim3(1), dim3(256), 0, stream, w, mu, route);
 }
 
-void launch_woodbury_seed(const WoodburyDev& w, int k, double* rhs, double* sol, int* aiter, int* aactive, int* aa,
-                          hipStream_t stream) {
-  hipLaunchKernelGGL(k_wb_seed, dim3(blocks_for(w.n)), dim3(256), 0, stream, w, k, rhs, sol, aiter, aactive, aa);
+void launch_woodbury_seed(const WoodburyDev& w, int k, double* rhs, double* sol, double* xr, double* fscale,
+                          int* aiter, int* aactive, int* aa, hipStream_t stream) {
+  hipLaunchKernelGGL(k_wb_seed, dim3(blocks_for(w.n)), dim3(256), 0, stream, w, k, rhs, sol, xr, fscale, aiter,
+                     aactive, aa);
 }
 
 void launch_woodbury_project(const WoodburyDev& w, const double* mu, const uint8_t* free, double* z, int* status,

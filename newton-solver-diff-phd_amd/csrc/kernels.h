@@ -154,8 +154,12 @@ struct DeviceState {
   int g_vert = 0, g_elem = 0;
   double* gram_part = nullptr;  // [nvc*kGramEntries]
   double* conv_part = nullptr;  // [nvc*4]
-  double* clip_part = nullptr;  // [nvc]
-  double* clip_part2 = nullptr; // [nvc]
+  double* clip_part2 = nullptr; // [nvc] squared norm of the adjoint's mixed iterate per chunk
+  // Squared norm of the adjoint's raw map output per S^T column group
+  // [t_nitems] (written by the S^T product's adjoint epilogue); the groups
+  // of body b are tcb[b] .. tcb[b + 1] - 1.
+  double* t_norm = nullptr;
+  int* tcb = nullptr;
   // iteration control
   int* iter = nullptr;      // [B]
   int* active = nullptr;    // [B]
@@ -192,7 +196,9 @@ struct DeviceState {
   double *arhs = nullptr, *sol = nullptr;  // [N*3]
   int* aiter = nullptr;      // [B]
   int* aactive = nullptr;    // [B]
-  double* ascale = nullptr;  // [B*2] clip scales
+  // [B] safe-clip scale of the adjoint's last mixed iterate: mu = fscale * xr
+  // until the next iteration (or the loop's end) stores it in sol.
+  double* fscale = nullptr;
   double* route = nullptr;   // [B*2] routing scale, diverged flag
   double* gmat_tmp = nullptr;  // [E*2]
   double* gact_tmp = nullptr;  // [E]
@@ -236,6 +242,8 @@ void launch_write_state(const DeviceState& d, const StepScalars& s, float* q, fl
 void launch_adjoint_setup(const DeviceState& d, const StepScalars& s, const double* x, const double* targets,
                           const float* gq_out, const float* gqd_out, hipStream_t stream);
 void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStream_t stream);
+// After an adjoint loop: mu = fscale * xr of every body into sol.
+void launch_adjoint_finalize(const DeviceState& d, hipStream_t stream);
 // Routing guard of a finished adjoint solve (scale and divergence flag per body).
 void launch_adjoint_route(const DeviceState& d, hipStream_t stream);
 // State, Dirichlet, material (gmat) and activation (gact, Tet4 fibres)

@@ -716,6 +716,17 @@ __global__ __launch_bounds__(kBlock) void k_factor_rows(const RowItem* __restric
   }
 }
 
+// The adjoint's safe-clip pass over the S^T output (adjoint.py:184-204,
+// _clip_partials): with `free` set, Dirichlet rows are stored as zero,
+// non-finite components as zero (status 16), and each column group's
+// squared norm goes to norm[item] (its lanes' columns in order, then the
+// wavefront sum), which the next kernel sums over the body's groups.
+struct AdjointEpilogue {
+  const uint8_t* free = nullptr;
+  double* norm = nullptr;
+  int* status = nullptr;
+};
+
 // S^T product: workgroup g takes the column groups t_list[woff[g] ..
 // woff[g + 1]) whose body is active (lane l of every wavefront holds the
 // l-th group of a window of 64); its kColWaves wavefronts split a group's
@@ -724,7 +735,8 @@ template <typename Real, typename Acc, typename In>
 __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
     const ColGroup* __restrict__ items, const int* __restrict__ woff, const int* __restrict__ list,
     const ColEntry* __restrict__ entries, const Real* __restrict__ val, const In* __restrict__ y,
-    double* __restrict__ out, const int* __restrict__ gorig, const double* base, const int* __restrict__ active) {
+    double* __restrict__ out, const int* __restrict__ gorig, const double* base, const int* __restrict__ active,
+    AdjointEpilogue adj) {
   // Rows per batch: 16 with FP32 values, 8 with FP64 (the same bytes).
   constexpr int kWaves = kColWaves, kBatch = sizeof(Real) == 4 ? 16 : 8;
   static_assert(kBatch <= kWave, "a batch's entries are held one per lane");
@@ -734,9 +746,11 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
   const int l0 = woff[blockIdx.x], l1 = woff[blockIdx.x + 1];
   for (int g0 = l0; g0 < l1; g0 += kWave) {
     ColGroup mine{};
+    int mine_id = 0;
     bool on = false;
     if (g0 + lane < l1) {
-      mine = items[list[g0 + lane]];
+      mine_id = list[g0 + lane];
+      mine = items[mine_id];
       on = active[mine.body] != 0;
     }
     // The same in every wavefront of the workgroup, so the barriers below
@@ -811,6 +825,8 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
         // A body's slots are a multiple of 4, so the lane's four columns
         // are all the body's or none; their vertices in one load, then the
         // base rows of all four together.
+        double sq = 0.0;
+        bool scrubbed = false;
         if (c < gend) {
           const int4 o4 = *reinterpret_cast<const int4*>(gorig + c);
           const int o[4] = {o4.x, o4.y, o4.z, o4.w};
@@ -819,12 +835,33 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
 #pragma unroll
             for (int u = 0; u < 4; ++u) add[u] = ld3(base, o[u] >= 0 ? o[u] : 0);
           }
+          uint8_t keep[4] = {1, 1, 1, 1};
+          if (adj.free) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) keep[u] = o[u] >= 0 ? adj.free[o[u]] : 0;
+          }
 #pragma unroll
           for (int u = 0; u < 4; ++u)
-            if (o[u] >= 0)
-              st3(out, o[u],
-                  V3{static_cast<double>(acc[u][0]), static_cast<double>(acc[u][1]), static_cast<double>(acc[u][2])} +
-                      add[u]);
+            if (o[u] >= 0) {
+              V3 v = V3{static_cast<double>(acc[u][0]), static_cast<double>(acc[u][1]),
+                        static_cast<double>(acc[u][2])} + add[u];
+              if (adj.free) {
+                if (!keep[u]) v = V3{0.0, 0.0, 0.0};
+                if (!isfinite(v.x)) { v.x = 0.0; scrubbed = true; }
+                if (!isfinite(v.y)) { v.y = 0.0; scrubbed = true; }
+                if (!isfinite(v.z)) { v.z = 0.0; scrubbed = true; }
+                sq += dot(v, v);
+              }
+              st3(out, o[u], v);
+            }
+        }
+        if (adj.free) {
+          const double total = wave_sum(sq);
+          const bool any_scrubbed = __ballot(scrubbed) != 0;
+          if (lane == 0) {
+            adj.norm[lane_int(mine_id, j)] = total;
+            if (any_scrubbed) atomicOr(adj.status, 16);
+          }
         }
       }
       __syncthreads();
@@ -1081,6 +1118,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_seed(DeviceState d, double dt, c
     d.aiter[b] = 0;
     d.aactive[b] = 1;
     d.aa[2 * b] = d.aa[2 * b + 1] = 0;
+    d.fscale[b] = 1.0;
   }
   const int i = c.begin + threadIdx.x;
   if (i >= c.end) return;
@@ -1088,6 +1126,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_seed(DeviceState d, double dt, c
     const double vel = (gqd ? static_cast<double>(gqd[3 * i + k]) : 0.0) / dt;
     d.arhs[3 * i + k] = (gq ? static_cast<double>(gq[3 * i + k]) : 0.0) + vel;
     d.sol[3 * i + k] = 0.0;
+    d.xr[3 * i + k] = 0.0;
   }
 }
 
@@ -1366,12 +1405,16 @@ __global__ __launch_bounds__(kBlock) void k_operator_assemble(DeviceState d) {
 }
 
 // work = K mu + rhs (gather with accumulate, adjoint.py:939), or
-// rhs + (K - A) mu in the defect mode; one thread per row.
+// rhs + (K - A) mu in the defect mode; one thread per row. mu is the last
+// iteration's mixed iterate times its safe-clip scale (fscale * xr), which
+// this kernel also stores as the row's mu (sol).
 __global__ __launch_bounds__(kBlock) void k_adj_rhs(DeviceState d) {
-  for_active(d.pref_av, d.vcb, d.B, [&](int ci, int) {
+  for_active(d.pref_av, d.vcb, d.B, [&](int ci, int b) {
     const Chunk c = d.vchunks[ci];
     const int i = c.begin + threadIdx.x;
     if (i >= c.end) return;
+    const double sc = d.fscale[b];
+    st3(d.sol, i, sc * ld3(d.xr, i));
     const int w = d.kwidth[ci];
     const size_t base = static_cast<size_t>(d.kbase[ci]);
     const int* col = d.kcol + base + threadIdx.x;
@@ -1392,7 +1435,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_rhs(DeviceState d) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int slot = min(s0 + h + u, w - 1);
-          m[u] = ld3(d.sol, j[h + u] >= 0 ? j[h + u] : i);
+          m[u] = sc * ld3(d.xr, j[h + u] >= 0 ? j[h + u] : i);
           const double* vb = val + static_cast<size_t>(slot) * 9 * kBlock;
 #pragma unroll
           for (int t = 0; t < 9; ++t) v[u][t] = vb[t * kBlock];
@@ -1418,47 +1461,25 @@ __device__ __forceinline__ V3 scrub(V3 v, bool& scrubbed) {
   return v;
 }
 
-// Safe-clip partial: Dirichlet entries zeroed, nonfinite components scrubbed
-// (adjoint.py:184-204).
-__global__ __launch_bounds__(kBlock) void k_adj_clip_partial(DeviceState d) {
-  __shared__ double smem[kBlock / kWave];
-  __shared__ double res[1];
-  for_active(d.pref_av, d.vcb, d.B, [&](int ci, int) {
-    const Chunk c = d.vchunks[ci];
-    const int i = c.begin + threadIdx.x;
-    double part[1] = {0.0};
-    if (i < c.end) {
-      V3 v = ld3(d.xr, i);
-      if (!d.free[i]) v = V3{0, 0, 0};
-      bool scrubbed = false;
-      v = scrub(v, scrubbed);
-      if (scrubbed) atomicOr(d.status, 16);
-      st3(d.xr, i, v);
-      part[0] = dot(v, v);
-    }
-    block_sum<1>(part, smem, res);
-    if (threadIdx.x == 0) d.clip_part[ci] = res[0];
-  });
-}
-
-// Clip scale of one body (adjoint.py:207-223): limit / ||v|| above 1e6.
-// Every workgroup recomputes it in the same order; the body's first chunk
-// reports the safeguard.
-__device__ __forceinline__ double clip_scale(const DeviceState& d, const double* parts, int b, bool reporter) {
-  const double len = sqrt(chunk_sum(parts, 1, 0, d.vcb[b], d.vcb[b + 1]));
+// Clip scale (adjoint.py:207-223, _clip_apply) of a vector with squared norm
+// `sq`: limit / ||v|| above the 1e6 limit, else 1.
+__device__ __forceinline__ double clip_factor(double sq) {
+  const double len = sqrt(sq);
   const double limit = 1.0e6;
-  if (isfinite(len) && len > limit) {
-    if (reporter && threadIdx.x == 0) atomicOr(d.status, 16);
-    return limit / len;
-  }
-  return 1.0;
+  return isfinite(len) && len > limit ? limit / len : 1.0;
 }
 
+// The raw map output after the S^T product's clip pass (Dirichlet rows and
+// non-finite components already zero): scaled to the clip limit (the body's
+// column-group norms summed in group order, every workgroup the same; the
+// body's first chunk reports the safeguard), then its step from mu and the
+// Anderson push with this vertex's Gram terms.
 __global__ __launch_bounds__(kBlock) void k_adj_push(DeviceState d) {
   __shared__ double smem[(kGramEntries + 1) * kBlock / kWave];
   __shared__ double res[kGramEntries + 1];
   for_active(d.pref_av, d.vcb, d.B, [&](int ci, int b) {
-    const double sc = clip_scale(d, d.clip_part, b, ci == d.vcb[b]);
+    const double sc = clip_factor(chunk_sum(d.t_norm, 1, 0, d.tcb[b], d.tcb[b + 1]));
+    if (sc != 1.0 && ci == d.vcb[b] && threadIdx.x == 0) atomicOr(d.status, 16);
     const Chunk c = d.vchunks[ci];
     const int i = c.begin + threadIdx.x;
     double acc[kGramEntries + 1];
@@ -1466,9 +1487,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_push(DeviceState d) {
     for (int k = 0; k < kGramEntries + 1; ++k) acc[k] = 0.0;
     if (i < c.end) {
       V3 raw = ld3(d.xr, i);
-      if (sc != 1.0) raw = sc * raw;
-      if (!d.free[i]) raw = V3{0, 0, 0};
-      st3(d.xr, i, raw);
+      if (sc != 1.0) {
+        raw = sc * raw;
+        st3(d.xr, i, raw);
+      }
       const V3 prev = ld3(d.sol, i);
       const V3 step = raw - prev;
       acc[kGramEntries] = dot(step, step);
@@ -1517,33 +1539,21 @@ __global__ __launch_bounds__(kBlock) void k_adj_apply(DeviceState d) {
   });
 }
 
-__global__ __launch_bounds__(kBlock) void k_adj_finish(DeviceState d) {
-  __shared__ double smem[kBlock / kWave];
-  __shared__ double res[1];
-  for_active(d.pref_av, d.vcb, d.B, [&](int ci, int b) {
-    const double sc = clip_scale(d, d.clip_part2, b, ci == d.vcb[b]);
-    const Chunk c = d.vchunks[ci];
-    const int i = c.begin + threadIdx.x;
-    double part[1] = {0.0};
-    if (i < c.end) {
-      V3 v = ld3(d.xr, i);
-      if (sc != 1.0) v = sc * v;
-      st3(d.sol, i, v);
-      part[0] = dot(v, v);
-    }
-    block_sum<1>(part, smem, res);
-    if (threadIdx.x == 0) d.conv_part[4 * ci + 1] = res[0];
-  });
-}
-
 // Every body's adjoint convergence test (one thread per body, one
-// workgroup), then the work-list prefix of the bodies still active.
+// workgroup), then the work-list prefix of the bodies still active. The
+// mixed iterate's safe clip (adjoint.py:207-223) is decided here: mu =
+// fscale * xr (stored as mu by the next iteration's first kernel or after
+// the loop), and its squared norm for the gate is fscale^2 times the
+// iterate's (exactly the iterate's when no clip applies).
 __global__ __launch_bounds__(kBlock) void k_adj_check(DeviceState d, StepScalars s) {
   for (int b = threadIdx.x; b < d.B; b += kBlock) {
     if (!d.aactive[b]) continue;
-    double t[2];
-    chunk_sums<2>(d.conv_part, 4, d.vcb[b], d.vcb[b + 1], t);
-    const double t0 = t[0], t1 = t[1];
+    const double t0 = chunk_sum(d.conv_part, 4, 0, d.vcb[b], d.vcb[b + 1]);
+    const double sq = chunk_sum(d.clip_part2, 1, 0, d.vcb[b], d.vcb[b + 1]);
+    const double sc = clip_factor(sq);
+    if (sc != 1.0) atomicOr(d.status, 16);
+    d.fscale[b] = sc;
+    const double t1 = sc == 1.0 ? sq : (sc * sc) * sq;
     const int k = d.aiter[b];
     d.aiter[b] = k + 1;
     if (k > 0 && sqrt(t0) < s.rel_tol * sqrt(t1) + s.abs_tol) {
@@ -1555,6 +1565,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_check(DeviceState d, StepScalars
     }
   }
   write_prefix(d.aactive, d.vcb, d.B, d.pref_av);
+}
+
+// mu = fscale * xr of every vertex, after an adjoint loop (the bodies that
+// converged in its last iteration had theirs only in this form).
+__global__ __launch_bounds__(kBlock) void k_adj_finalize(DeviceState d) {
+  const Chunk c = d.vchunks[blockIdx.x];
+  const int i = c.begin + threadIdx.x;
+  if (i < c.end) st3(d.sol, i, d.fscale[c.body] * ld3(d.xr, i));
 }
 
 // Routing guard per body (adjoint.py:226-261): one block loops over the body.
@@ -1859,7 +1877,8 @@ size_t prefix_lds(const DeviceState& d) { return sizeof(int) * static_cast<size_
 // the S product writes the 4-wide postorder y4, the S^T product the FP64
 // [N*3] out in original order.
 template <typename Real, typename Acc, typename In>
-void spmv_pair(const DeviceState& d, double* out, const double* base, const int* active, hipStream_t stream) {
+void spmv_pair(const DeviceState& d, double* out, const double* base, const int* active, AdjointEpilogue adj,
+               hipStream_t stream) {
   const Real* val = static_cast<const Real*>(d.fval);
   In* y4 = static_cast<In*>(d.y4);
   if (d.s_nitems)
@@ -1867,15 +1886,16 @@ void spmv_pair(const DeviceState& d, double* out, const double* base, const int*
                        d.s_list, val, static_cast<const In*>(d.rhs4), y4, active);
   if (d.t_nitems)
     hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_t), dim3(kColWaves * kWave), 0, stream, d.t_items, d.t_woff,
-                       d.t_list, d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base, active);
+                       d.t_list, d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base, active, adj);
 }
 
 // Dispatch on the stored precision.
-void spmv_dispatch(const DeviceState& d, double* out, const double* base, const int* active, hipStream_t stream) {
+void spmv_dispatch(const DeviceState& d, double* out, const double* base, const int* active, hipStream_t stream,
+                   AdjointEpilogue adj = AdjointEpilogue{}) {
   if (d.f32)
-    spmv_pair<float, float, float>(d, out, base, active, stream);
+    spmv_pair<float, float, float>(d, out, base, active, adj, stream);
   else
-    spmv_pair<double, double, double>(d, out, base, active, stream);
+    spmv_pair<double, double, double>(d, out, base, active, adj, stream);
 }
 
 // Iteration products: x_raw = S^T (S rhs) (+ base, the defect-correction form).
@@ -1959,12 +1979,14 @@ void launch_adjoint_setup(const DeviceState& d, const StepScalars& s, const doub
 void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStream_t stream) {
   const size_t lds = prefix_lds(d);
   hipLaunchKernelGGL(k_adj_rhs, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
-  spmv_iteration(d, d.defect ? d.sol : nullptr, d.aactive, stream);
-  hipLaunchKernelGGL(k_adj_clip_partial, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
+  spmv_dispatch(d, d.xr, d.defect ? d.sol : nullptr, d.aactive, stream, AdjointEpilogue{d.free, d.t_norm, d.status});
   hipLaunchKernelGGL(k_adj_push, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   hipLaunchKernelGGL(k_adj_apply, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
-  hipLaunchKernelGGL(k_adj_finish, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   hipLaunchKernelGGL(k_adj_check, dim3(1), dim3(kBlock), lds, stream, d, s);
+}
+
+void launch_adjoint_finalize(const DeviceState& d, hipStream_t stream) {
+  hipLaunchKernelGGL(k_adj_finalize, dim3(d.nvc), dim3(kBlock), 0, stream, d);
 }
 
 void launch_adjoint_route(const DeviceState& d, hipStream_t stream) {

@@ -164,6 +164,8 @@ struct dphd_solver {
   int key_iter[3] = {-1, -1, -1};
   double key_abs[3] = {-1, -1, -1}, key_rel[3] = {-1, -1, -1};
   int last_iters[3] = {0, 0, 0};
+  // Per loop mode since the last build: iteration chunks launched, host waits.
+  int64_t loop_stats[6] = {0, 0, 0, 0, 0, 0};
 
   ~dphd_solver() { release(); }
 
@@ -385,7 +387,9 @@ void build_solver(dphd_solver* s, double dt) {
   std::vector<RowItem> s_items;
   std::vector<ColGroup> t_items;
   std::vector<ColEntry> t_entries;
+  std::vector<int> tcb(B + 1, 0);  // column groups of each body
   for (int b = 0; b < B; ++b) {
+    tcb[b] = static_cast<int>(t_items.size());
     const int v0 = s->voff[b], n = s->voff[b + 1] - v0, c0 = slot0[b];
     FactorLayout& L = layouts[b];
     for (int o = 0; o < n; ++o) {
@@ -437,6 +441,7 @@ void build_solver(dphd_solver* s, double dt) {
     }
     L = FactorLayout{};
   }
+  tcb[B] = static_cast<int>(t_items.size());
   std::vector<int> a_off, a_col;
   std::vector<double> a_val;
   const bool defect = s->cfg.factor_precision == 32;
@@ -667,8 +672,9 @@ void build_solver(dphd_solver* s, double dt) {
   d.g_elem = grid_for(d.nec);
   d.gram_part = dzeros<double>(o, static_cast<size_t>(d.nvc) * kGramEntries);
   d.conv_part = dzeros<double>(o, static_cast<size_t>(d.nvc) * 4);
-  d.clip_part = dzeros<double>(o, d.nvc);
   d.clip_part2 = dzeros<double>(o, d.nvc);
+  d.t_norm = dzeros<double>(o, std::max<size_t>(t_items.size(), 1));
+  d.tcb = dupload(o, tcb);
   d.iter = dzeros<int>(o, B);
   d.active = dzeros<int>(o, B);
   d.status = dzeros<int>(o, 1);
@@ -679,7 +685,7 @@ void build_solver(dphd_solver* s, double dt) {
   d.pref_fv = dzeros<int>(o, B + 1);
   d.pref_fe = dzeros<int>(o, B + 1);
   d.pref_av = dzeros<int>(o, B + 1);
-  d.ascale = dzeros<double>(o, 2 * static_cast<size_t>(B));
+  d.fscale = dzeros<double>(o, static_cast<size_t>(B));
   d.route = dzeros<double>(o, 2 * static_cast<size_t>(B));
   d.gmat_tmp = dzeros<double>(o, 2 * static_cast<size_t>(E));
   d.gact_tmp = dzeros<double>(o, static_cast<size_t>(E));
@@ -738,6 +744,7 @@ void build_solver(dphd_solver* s, double dt) {
   s->info.factor_values = factor_values;
   s->info.factor_slots = slots;
   s->info.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::fill(std::begin(s->loop_stats), std::end(s->loop_stats), 0);
 }
 
 // Device arrays of the mesh contact (MeshContact.__init__, mesh_contact.py:180-243).
@@ -972,9 +979,12 @@ int run_loop(dphd_solver* s, int mode) {
     const int burst = launched == 0 ? first : 1;
     for (int k = 0; k < burst && launched < max_chunks; ++k, ++launched) HIP_CHECK(hipGraphLaunch(exec, s->stream));
     HIP_CHECK(hipStreamSynchronize(s->stream));
+    ++s->loop_stats[2 * mode + 1];
     if (!*reinterpret_cast<volatile int*>(s->any_host)) break;
   }
+  s->loop_stats[2 * mode] += launched;
   s->last_iters[mode] = launched * C;
+  if (mode != kForward) launch_adjoint_finalize(s->d, s->stream);
   return launched;
 }
 
@@ -992,8 +1002,11 @@ void run_adjoint_direct(dphd_solver* s, const DeviceState& d) {
     launch_any_active(d, d.aactive, s->stream);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->loop_stats[2 * kAdjoint] += 1;
+    s->loop_stats[2 * kAdjoint + 1] += 1;
     if (!*reinterpret_cast<volatile int*>(s->any_host)) break;
   }
+  launch_adjoint_finalize(d, s->stream);
 }
 
 // Woodbury projection of the adjoint for the parameter gradients; returns z
@@ -1015,7 +1028,7 @@ const double* woodbury(dphd_solver* s) {
     s->wb_capacity = count;
   }
   for (int k = 0; k < count; ++k) {
-    launch_woodbury_seed(w, k, d.arhs, d.sol, d.aiter, d.aactive, d.aa, st);
+    launch_woodbury_seed(w, k, d.arhs, d.sol, d.xr, d.fscale, d.aiter, d.aactive, d.aa, st);
     launch_active_prefix(d, true, st);
     run_loop(s, kWoodbury);
     HIP_CHECK(hipMemcpyAsync(w.columns + static_cast<size_t>(k) * 3 * s->N, d.sol, vb, hipMemcpyDeviceToDevice, st));
@@ -1417,6 +1430,15 @@ int dphd_get_info(const dphd_solver* s, dphd_info* info) {
     return DPHD_EINVAL;
   }
   *info = s->info;
+  return DPHD_OK;
+}
+
+int dphd_loop_stats(const dphd_solver* s, int64_t* stats) {
+  if (!s || !stats) {
+    set_error("dphd_loop_stats: null argument");
+    return DPHD_EINVAL;
+  }
+  std::copy(std::begin(s->loop_stats), std::end(s->loop_stats), stats);
   return DPHD_OK;
 }
 

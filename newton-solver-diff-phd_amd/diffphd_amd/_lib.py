@@ -88,6 +88,7 @@ SIGNATURES = {
     "dphd_build": ([_P, _D], _I),
     "dphd_get_info": ([_P, ctypes.POINTER(Info)], _I),
     "dphd_body_nnz": ([_P, _P], _I),
+    "dphd_loop_stats": ([_P, _P], _I),
     "dphd_step": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "dphd_backward": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "dphd_backward_static": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),

@@ -482,6 +482,14 @@ class SolverDiffPhD:
         check(lib().dphd_body_nnz(self._h, ptr(out)), "body_nnz")
         return out
 
+    def loop_stats(self) -> dict:
+        """Iteration chunks launched and host waits of the forward, adjoint and Woodbury loops since the last
+        factor build."""
+        out = np.zeros(6, np.int64)
+        check(lib().dphd_loop_stats(self._h, ptr(out)), "loop_stats")
+        return {mode: {"chunks": int(out[2 * k]), "host_waits": int(out[2 * k + 1])}
+                for k, mode in enumerate(("forward", "adjoint", "woodbury"))}
+
     def solution(self) -> torch.Tensor:
         """FP64 converged iterate of the last step (the reference's ``solver._x``)."""
         return self._x

@@ -57,8 +57,13 @@ def test_fixture_outputs_with_the_fp32_factor(cuda, name):
     diff = np.abs(np.asarray(port["iterations"]) - np.asarray(ref["iterations"]))
     print(f"[result] fixture {name} fp32 factor: largest iteration difference {int(diff.max())} "
           f"(oracle max {int(np.max(ref['iterations']))})", flush=True)
-    # Every status bit, the adjoint's (8 not converged, 16 safeguard) included, as the oracle's.
-    np.testing.assert_array_equal(port["status_bits"], ref["status_bits"])
+    # Every status bit as the oracle's, with one exception: where the adjoint diverges and is held at the
+    # safe-clip limit (bit 16 on both sides), the FP32 products' rounding is relative to that 1e6 norm
+    # (~1e-7 of it), so the fixture's 1e-11 gate cannot close and the FP32 mode also reports bit 8.
+    port_bits, ref_bits = np.asarray(port["status_bits"]), np.asarray(ref["status_bits"])
+    extra = port_bits & ~ref_bits
+    assert np.all((ref_bits & ~port_bits) == 0), (port_bits, ref_bits)
+    assert np.all((extra == 0) | ((extra == 8) & ((ref_bits & 16) != 0))), (port_bits, ref_bits)
     assert abs(port["loss"] - ref["loss"]) <= 1e-6 * max(1.0, abs(ref["loss"]))
 
 
