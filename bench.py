@@ -307,6 +307,193 @@ def timed_batched(model, dev, dt, idx, tgt, args, rel_tol, abs_tol):
                 qd0=qd0, fwd=fwd, adj=adj, loops=loops)
 
 
+# ---------------------------------------------------------------------------
+# The other benchmark configurations, one JSON line each (--workload):
+# steps/s, iterations, microseconds per iteration, loop chunks per step and
+# algorithmic GB/s. Setup (factor) is excluded from the timed region.
+
+
+def bytes_iteration(n, e, z, s, corners, samples, adjoint, window=5):
+    """The per-iteration byte model for one body: 2 Z (s+4) + 8 (n+1) + E (4K + G_e + 3s) + c_f n V forward
+    (G_e = 12 s for Tet4, 0 for Hex8's shared table), 2 Z (s+4) + 8 (n+1) + E (4K + 81 Q s_J) + c_a n V adjoint."""
+    if adjoint:
+        return 2 * z * (s + 4) + 8 * (n + 1) + e * (4 * corners + 81 * samples * 8) + 34 * n * 24
+    g_e = 12 * s if corners == 4 else 0
+    return 2 * z * (s + 4) + 8 * (n + 1) + e * (4 * corners + g_e + 3 * s) + (6 + 3 * window + 5 + 4) * n * 24
+
+
+def config_scene(name):
+    """Synthetic inputs of the benchmark configurations (seeds: 1 materials, 2 targets)."""
+    from diffphd_amd import meshes
+
+    if name in ("config1", "config3"):
+        nx, ny, nz, cell = (20, 5, 5, 0.01) if name == "config1" else (64, 16, 16, 0.005)
+        v, hexes = meshes.hex_grid(nx, ny, nz, cell)
+        xcell = np.arange(len(hexes)) // (ny * nz)
+        stiff = (xcell > nx // 3) & (xcell <= (nx // 3) * 2)
+        mu, lam = meshes.lame(np.where(stiff, 1.0e7, 1.0e6), 0.45)
+        free = v[:, 0] != 0.0
+        tip = np.flatnonzero(v[:, 0] == v[:, 0].max())
+        q0 = v if name == "config1" else meshes.twisted_positions(v, np.pi / 6.0)
+        return dict(kind="hex8", v=v, hexes=hexes, mats=np.column_stack([mu, lam]), free=free, tip=tip,
+                    q0=q0.astype(np.float32), gravity=(0.0, 0.0, 0.0), density=1000.0,
+                    dt=0.01 if name == "config1" else 1e-3, steps=25 if name == "config1" else 100,
+                    rel_tol=1e-4 if name == "config1" else 1e-6, adjoint=name == "config3")
+    if name == "config2":
+        v, t, p = meshes.kuhn_tet_grid(20, 20, 20, 0.005)
+        young = np.exp(np.random.default_rng(1).uniform(np.log(1e5), np.log(1e7), len(t)))
+        mu, lam = meshes.lame(young, 0.4)
+        top = np.flatnonzero(v[:, 2] == v[:, 2].max())
+        target = v[top] + np.random.default_rng(2).normal(0.0, 1.0e-3, (len(top), 3))
+        return dict(kind="tet4", v=v, t=t, p=p, mats=np.column_stack([mu, lam, np.zeros_like(mu)]),
+                    free=v[:, 2] > v[:, 2].min(), top=top, target=target, q0=v.astype(np.float32),
+                    gravity=(0.0, 0.0, -9.81), density=1000.0, dt=1e-3, steps=50, rel_tol=1e-6, adjoint=True)
+    if name == "config5":
+        v, t, p = meshes.kuhn_tet_grid(16, 16, 8, 0.0025)
+        mu, lam = meshes.lame(np.full(len(t), 5.0e5), 0.45)
+        top_z = v[:, 2].max()
+        surface = np.flatnonzero(v[:, 2] == top_z).astype(np.int32)
+        target = v[surface] + np.random.default_rng(2).normal(0.0, 1.0e-3, (len(surface), 3))
+        center0 = np.array([0.02, 0.02, top_z + 0.005])
+
+        def sphere(step, total=200):
+            f = min(step, total) / total
+            c = center0 + np.array([0.003 * f, 0.0, -0.006 * f])
+            return (float(c[0]), float(c[1]), float(c[2]), 0.005)
+
+        return dict(kind="tet4", v=v, t=t, p=p, mats=np.column_stack([mu, lam, np.zeros_like(mu)]),
+                    free=v[:, 2] > v[:, 2].min(), top=surface, target=target, q0=v.astype(np.float32),
+                    gravity=(0.0, 0.0, -9.81), density=1100.0, dt=1e-3, steps=40, rel_tol=1e-6, adjoint=True,
+                    sphere=sphere, friction=0.5)
+    raise ValueError(name)
+
+
+def tip_rotation_torch(q, rest, tip):
+    """Signed rotation of the tip cross-section about x (differentiable)."""
+    import torch
+
+    r0 = torch.as_tensor(rest[tip, 1:] - rest[tip, 1:].mean(axis=0), dtype=torch.float64, device=q.device)
+    qt = q[torch.as_tensor(tip, device=q.device)][:, 1:].to(torch.float64)
+    r = qt - qt.mean(dim=0)
+    return torch.atan2((r0[:, 0] * r[:, 1] - r0[:, 1] * r[:, 0]).sum(), (r0 * r).sum())
+
+
+def run_workload(name, args):
+    import torch
+
+    from diffphd_amd import Mesh, SolverDiffPhD
+    from diffphd_amd.model import Model
+
+    dev = torch.device("cuda:0")
+    sc = config_scene(name)
+    precision = 64 if name == "config1" else args.precision
+    steps = sc["steps"] if args.workload_steps <= 0 else args.workload_steps
+    n = len(sc["v"])
+    kw = dict(density=sc["density"], rel_tol=sc["rel_tol"], abs_tol=1e-9, max_iterations=10000,
+              factor_precision=precision, jacobian_mode="element")
+    if sc["kind"] == "hex8":
+        model = Model(particle_q=sc["q0"], particle_qd=np.zeros_like(sc["q0"]),
+                      particle_inv_mass=sc["free"].astype(np.float32), particle_world=np.zeros(n, np.int32),
+                      gravity=np.zeros((1, 3)), tet_indices=None, tet_poses=None, tet_materials=None, device=dev)
+        solver = SolverDiffPhD(model, hex_mesh=Mesh("beam", sc["v"], sc["hexes"]), hex_materials=sc["mats"], **kw)
+        elements, corners, samples = len(sc["hexes"]), 8, 8
+        materials = solver.hex_materials
+    else:
+        model = Model(particle_q=sc["q0"], particle_qd=np.zeros_like(sc["q0"]),
+                      particle_inv_mass=sc["free"].astype(np.float32), particle_world=np.zeros(n, np.int32),
+                      gravity=np.asarray(sc["gravity"])[None], tet_indices=sc["t"], tet_poses=sc["p"],
+                      tet_materials=sc["mats"], device=dev, requires_grad=sc["adjoint"])
+        extra = {}
+        if "sphere" in sc:
+            extra = dict(frictional_boundary=("spherical", sc["sphere"](0)), frictional_vertices=sc["top"],
+                         friction=sc["friction"])
+        solver = SolverDiffPhD(model, **kw, **extra)
+        elements, corners, samples = len(sc["t"]), 4, 1
+        materials = model.tet_materials
+    t0 = time.time()
+    solver.prepare(sc["dt"])
+    setup = time.time() - t0
+    if sc["adjoint"] and sc["kind"] == "hex8":
+        materials.requires_grad_(True)
+
+    def forces(t):
+        if name != "config1":
+            return None
+        f = torch.zeros((n, 3), dtype=torch.float32, device=dev)
+        f[torch.as_tensor(sc["tip"], device=dev), 0] = 80.0 * t / 25
+        return f
+
+    def one_rollout(count):
+        state = model.state(requires_grad=sc["adjoint"])
+        q0, qd0 = state.particle_q, state.particle_qd
+        with torch.set_grad_enabled(sc["adjoint"]):
+            for t in range(count):
+                if "sphere" in sc:
+                    solver.boundary_parameters = sc["sphere"](t)
+                f = forces(t)
+                if f is not None:
+                    state.particle_f = f
+                out = model.state()
+                solver.step(state, out, None, None, sc["dt"])
+                state = out
+            if sc["adjoint"]:
+                if name == "config3":
+                    loss = tip_rotation_torch(state.particle_q, sc["v"], sc["tip"])
+                else:
+                    idx = torch.as_tensor(sc["top"], device=dev)
+                    tgt = torch.as_tensor(sc["target"], dtype=torch.float64, device=dev)
+                    loss = ((state.particle_q[idx].to(torch.float64) - tgt) ** 2).sum()
+                loss.backward()
+        return q0, qd0
+
+    one_rollout(1)
+    torch.cuda.synchronize()
+    solver.read_status()
+    before = solver.loop_stats()
+    start = time.perf_counter()
+    one_rollout(steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    after = solver.loop_stats()
+    status = solver.read_status()
+    solver.iteration_log = []
+    one_rollout(steps)
+    log_ = solver.iteration_log
+    solver.iteration_log = None
+    fwd = np.array([int(it[0]) for kind, it in log_ if kind == "forward"])
+    adj = np.array([int(it[0]) for kind, it in log_ if kind == "adjoint"])
+    z = float(solver.body_factor_nnz()[0])
+    s_bytes = {64: 8, 32: 4}[precision]
+    total_bytes = fwd.sum() * bytes_iteration(n, elements, z, s_bytes, corners, samples, False)
+    if sc["adjoint"]:
+        total_bytes += adj.sum() * bytes_iteration(n, elements, z, s_bytes, corners, samples, True)
+    iterations = int(fwd.sum() + adj.sum())
+    loops = {m: {k: (after[m][k] - before[m][k]) / steps for k in after[m]} for m in after}
+    result = {
+        "workload": name,
+        "what": "forward" if not sc["adjoint"] else "forward+adjoint",
+        "steps": steps,
+        "config_steps": sc["steps"] if name != "config5" else 200,
+        "steps_per_s": steps / elapsed,
+        "ms_per_step": 1e3 * elapsed / steps,
+        "setup_s": setup,
+        "factor_precision": precision,
+        "vertices": n,
+        "elements": elements,
+        "factor_nnz": int(z),
+        "iterations": {"forward_mean": float(fwd.mean()), "forward_max": int(fwd.max()),
+                       "adjoint_mean": float(adj.mean()) if len(adj) else None,
+                       "adjoint_max": int(adj.max()) if len(adj) else None},
+        "us_per_iteration": 1e6 * elapsed / max(iterations, 1),
+        "element_iterations_per_s": elements * iterations / elapsed,
+        "algorithmic_gbps": total_bytes / elapsed / 1e9,
+        "frac_of_hbm_peak": total_bytes / elapsed / 1e9 / HBM_PEAK_GBPS,
+        "loops_per_step": loops,
+        "status": status,
+    }
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,7 +511,12 @@ def main():
     ap.add_argument("--no-gradient-check", action="store_true")
     ap.add_argument("--accurate-rel", type=float, default=1e-10,
                     help="relative tolerance (absolute 0) of the gradient-accurate setting; 0 skips it")
+    ap.add_argument("--dump-iterations", default=None,
+                    help="write the per-step, per-body iteration counts and factor sizes to this JSON file")
     ap.add_argument("--probe-reps", type=int, default=20)
+    ap.add_argument("--workload", default="config4", choices=("config1", "config2", "config3", "config4", "config5"),
+                    help="config4 (default) is the headline batched workload; the others print their own line")
+    ap.add_argument("--workload-steps", type=int, default=0, help="steps of a --workload run (0 = the config's)")
     ap.add_argument("--probe-only", action="store_true",
                     help="only the factor-product probe and the copy calibration (for counter runs)")
     args = ap.parse_args()
@@ -336,6 +528,10 @@ def main():
             file=sys.stderr,
         )
         sys.exit(2)
+
+    if args.workload != "config4":
+        run_workload(args.workload, args)
+        return
 
     import torch
 
@@ -378,6 +574,10 @@ def main():
     elem_iters = float((fwd_iters * e_b).sum() + (adj_iters * e_b).sum())
     rollout_gbps = total_bytes / elapsed / 1e9
 
+    if args.dump_iterations:
+        Path(args.dump_iterations).write_text(json.dumps({
+            "forward": fwd_iters.tolist(), "adjoint": adj_iters.tolist(), "nnz": z_b.tolist(),
+            "vertices": n_b.tolist(), "elements": e_b.tolist(), "factor_values": int(solver.info.factor_values)}))
     probe = factor_probe(solver, dev, n_total, z_total, s_bytes, args)
     copy_gbps = copy_ceiling(dev)
 

@@ -68,15 +68,36 @@ struct ColEntry {
 // S^T product work item (one workgroup): the 256 columns [c0, c0 + 256) of
 // body `body` (columns from `end` on belong to the next body), over the rows
 // entries[e0 .. e1) of the blocks that cover them (block by block, rows
-// ascending).
+// ascending). `gid` is the column group (the index of its whole-group item).
+// A whole group, or a group's only slice, writes its columns (part < 0); a
+// slice of a group cut into several writes its sums to partial slot `part`,
+// and the group's combine item adds the slices in order.
 struct ColGroup {
   int c0;
   int e0;
   int e1;
   int end;
   int body;
-  int pad[3];
+  int part;
+  int gid;
+  int pad;
 };
+
+// Combine item of a group cut into slices: columns [c0, c0 + 256) of body
+// `body` (up to `end`), partial slots part0 .. part0 + parts - 1 in row order.
+struct ColCombine {
+  int c0;
+  int end;
+  int body;
+  int part0;
+  int parts;
+  int gid;
+  int pad[2];
+};
+
+// Rows of a slice of a column group cut for the sliced S^T list (entries);
+// the default of the build.
+constexpr int kSliceRows = 256;
 
 struct Chunk {
   int body;
@@ -125,6 +146,24 @@ struct DeviceState {
   int *t_woff = nullptr, *t_list = nullptr;
   ColEntry* t_entries = nullptr;  // block rows of the column groups
   int g_s = 0, g_t = 0;       // grids of the two products
+  // The sliced S^T list (the same columns and rows, groups cut into slices of
+  // at most `slice_rows` entries; items ts_items, per-workgroup lists
+  // ts_woff/ts_list over a grid of g_ts), the combine items of the cut groups
+  // (tc_items, g_tc workgroups of four) and their partial sums (t_part,
+  // [slots][12][64] of the accumulation type).
+  ColGroup* ts_items = nullptr;
+  int ts_nitems = 0;
+  int *ts_woff = nullptr, *ts_list = nullptr;
+  ColCombine* tc_items = nullptr;
+  int tc_nitems = 0;
+  void* t_part = nullptr;
+  int g_ts = 0, g_tc = 0;
+  int slice_rows = kSliceRows;
+  // Which S^T list runs: t_mode[0] is written with each active-body prefix
+  // (1 = sliced, when the active bodies hold fewer than t_switch column
+  // groups); t_mode[1] is the constant choice for every body at once.
+  int* t_mode = nullptr;
+  int t_switch = 0;
   bool f32 = false;           // FP32 factor values
   // constant matrix for the defect correction (per coordinate)
   int *a_off = nullptr, *a_col = nullptr;

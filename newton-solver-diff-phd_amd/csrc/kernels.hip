@@ -286,10 +286,22 @@ __device__ void write_prefix(const int* active, const int* base, int B, int* out
   for (int b = threadIdx.x; b <= B; b += blockDim.x) out[b] = pref[b];
 }
 
+// The S^T list for the next products (t_mode[0]): sliced while the active
+// bodies hold fewer than `threshold` column groups (tcb: each body's groups).
+// Every thread of the workgroup calls it.
+__device__ void write_mode(const int* active, const int* tcb, int B, int threshold, int* t_mode) {
+  extern __shared__ int pref[];
+  __syncthreads();
+  const int groups = active_prefix(active, tcb, B, pref);
+  if (threadIdx.x == 0) t_mode[0] = groups < threshold ? 1 : 0;
+}
+
 __global__ __launch_bounds__(kBlock) void k_active_prefix(const int* active, const int* base0, int* out0,
-                                                          const int* base1, int* out1, int B) {
+                                                          const int* base1, int* out1, int B, const int* tcb,
+                                                          int threshold, int* t_mode) {
   write_prefix(active, base0, B, out0);
   if (base1) write_prefix(active, base1, B, out1);
+  write_mode(active, tcb, B, threshold, t_mode);
 }
 
 // Loop `fn(item, body)` over the active items listed by the prefix `pref_g`
@@ -731,12 +743,68 @@ struct AdjointEpilogue {
 // woff[g + 1]) whose body is active (lane l of every wavefront holds the
 // l-th group of a window of 64); its kColWaves wavefronts split a group's
 // rows.
+// The lane's four columns c .. c + 3 of a column group (body columns end
+// at gend) from their sums acc: plus the base rows, the adjoint's clip pass,
+// stored at their vertices; with the clip pass, the group's squared norm to
+// adj.norm[gid]. The calling wavefront's 64 lanes all take part.
+template <typename Acc>
+__device__ __forceinline__ void cols_store(const Acc (&acc)[4][3], int c, int gend, const int* __restrict__ gorig,
+                                           const double* base, double* __restrict__ out, const AdjointEpilogue& adj,
+                                           int gid, int lane) {
+  // A body's slots are a multiple of 4, so the lane's four columns are all
+  // the body's or none; their vertices in one load, then the base rows of
+  // all four together.
+  double sq = 0.0;
+  bool scrubbed = false;
+  if (c < gend) {
+    const int4 o4 = *reinterpret_cast<const int4*>(gorig + c);
+    const int o[4] = {o4.x, o4.y, o4.z, o4.w};
+    V3 add[4] = {};
+    if (base) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) add[u] = ld3(base, o[u] >= 0 ? o[u] : 0);
+    }
+    uint8_t keep[4] = {1, 1, 1, 1};
+    if (adj.free) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) keep[u] = o[u] >= 0 ? adj.free[o[u]] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (o[u] >= 0) {
+        V3 v = V3{static_cast<double>(acc[u][0]), static_cast<double>(acc[u][1]), static_cast<double>(acc[u][2])} +
+               add[u];
+        if (adj.free) {
+          if (!keep[u]) v = V3{0.0, 0.0, 0.0};
+          if (!isfinite(v.x)) { v.x = 0.0; scrubbed = true; }
+          if (!isfinite(v.y)) { v.y = 0.0; scrubbed = true; }
+          if (!isfinite(v.z)) { v.z = 0.0; scrubbed = true; }
+          sq += dot(v, v);
+        }
+        st3(out, o[u], v);
+      }
+  }
+  if (adj.free) {
+    const double total = wave_sum(sq);
+    const bool any_scrubbed = __ballot(scrubbed) != 0;
+    if (lane == 0) {
+      adj.norm[gid] = total;
+      if (any_scrubbed) atomicOr(adj.status, 16);
+    }
+  }
+}
+
+// S^T over a column-group list (whole groups, or the sliced list); returns
+// at once unless *mode selects this list (`want`). Sliced items with a
+// partial slot store their sums to part_buf[slot][12][64] (coalesced) for the
+// combine kernel.
 template <typename Real, typename Acc, typename In>
 __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
     const ColGroup* __restrict__ items, const int* __restrict__ woff, const int* __restrict__ list,
     const ColEntry* __restrict__ entries, const Real* __restrict__ val, const In* __restrict__ y,
     double* __restrict__ out, const int* __restrict__ gorig, const double* base, const int* __restrict__ active,
-    AdjointEpilogue adj) {
+    AdjointEpilogue adj, const int* __restrict__ mode, int want, Acc* __restrict__ part_buf) {
+  if (*mode != want) return;
   // Rows per batch: 16 with FP32 values, 8 with FP64 (the same bytes).
   constexpr int kWaves = kColWaves, kBatch = sizeof(Real) == 4 ? 16 : 8;
   static_assert(kBatch <= kWave, "a batch's entries are held one per lane");
@@ -760,7 +828,7 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
       const int j = __builtin_amdgcn_readfirstlane(static_cast<int>(__builtin_ctzll(pending)));
       pending &= pending - 1;
       const int gc0 = lane_int(mine.c0, j), ge0 = lane_int(mine.e0, j), ge1 = lane_int(mine.e1, j);
-      const int gend = lane_int(mine.end, j);
+      const int gend = lane_int(mine.end, j), gpart = lane_int(mine.part, j), gid = lane_int(mine.gid, j);
       const int c = gc0 + 4 * lane;
       Acc acc[4][3];
 #pragma unroll
@@ -822,51 +890,43 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
         for (int w = 0; w < kWaves - 1; ++w)
 #pragma unroll
           for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] += part[w][t][lane];
-        // A body's slots are a multiple of 4, so the lane's four columns
-        // are all the body's or none; their vertices in one load, then the
-        // base rows of all four together.
-        double sq = 0.0;
-        bool scrubbed = false;
-        if (c < gend) {
-          const int4 o4 = *reinterpret_cast<const int4*>(gorig + c);
-          const int o[4] = {o4.x, o4.y, o4.z, o4.w};
-          V3 add[4] = {};
-          if (base) {
+        if (gpart >= 0) {
+          Acc* dst = part_buf + static_cast<size_t>(gpart) * 12 * kWave + lane;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) add[u] = ld3(base, o[u] >= 0 ? o[u] : 0);
-          }
-          uint8_t keep[4] = {1, 1, 1, 1};
-          if (adj.free) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) keep[u] = o[u] >= 0 ? adj.free[o[u]] : 0;
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            if (o[u] >= 0) {
-              V3 v = V3{static_cast<double>(acc[u][0]), static_cast<double>(acc[u][1]),
-                        static_cast<double>(acc[u][2])} + add[u];
-              if (adj.free) {
-                if (!keep[u]) v = V3{0.0, 0.0, 0.0};
-                if (!isfinite(v.x)) { v.x = 0.0; scrubbed = true; }
-                if (!isfinite(v.y)) { v.y = 0.0; scrubbed = true; }
-                if (!isfinite(v.z)) { v.z = 0.0; scrubbed = true; }
-                sq += dot(v, v);
-              }
-              st3(out, o[u], v);
-            }
-        }
-        if (adj.free) {
-          const double total = wave_sum(sq);
-          const bool any_scrubbed = __ballot(scrubbed) != 0;
-          if (lane == 0) {
-            adj.norm[lane_int(mine_id, j)] = total;
-            if (any_scrubbed) atomicOr(adj.status, 16);
-          }
+          for (int t = 0; t < 12; ++t) dst[t * kWave] = acc[t / 3][t % 3];
+        } else {
+          cols_store<Acc>(acc, c, gend, gorig, base, out, adj, gid, lane);
         }
       }
       __syncthreads();
     }
   }
+}
+
+// The groups cut into slices: one wavefront per group adds its slices'
+// partial sums in row order, then stores its columns as k_factor_cols does.
+template <typename Acc>
+__global__ __launch_bounds__(kBlock) void k_factor_combine(const ColCombine* __restrict__ items, int count,
+                                                           const Acc* __restrict__ part_buf, double* __restrict__ out,
+                                                           const int* __restrict__ gorig, const double* base,
+                                                           const int* __restrict__ active, AdjointEpilogue adj,
+                                                           const int* __restrict__ mode) {
+  if (*mode != 1) return;
+  const int lane = static_cast<int>(threadIdx.x) % kWave;
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * (kBlock / kWave) +
+                                               static_cast<int>(threadIdx.x) / kWave);
+  if (w >= count) return;
+  const ColCombine it = items[w];
+  if (!active[it.body]) return;
+  Acc acc[4][3];
+#pragma unroll
+  for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] = Acc(0);
+  for (int p = it.part0; p < it.part0 + it.parts; ++p) {
+    const Acc* src = part_buf + static_cast<size_t>(p) * 12 * kWave + lane;
+#pragma unroll
+    for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] += src[t * kWave];
+  }
+  cols_store<Acc>(acc, it.c0 + 4 * lane, it.end, gorig, base, out, adj, it.gid, lane);
 }
 
 // r = rhs - A x with the FP64 constant matrix (FP32-factor mode), one thread
@@ -1106,6 +1166,7 @@ __global__ __launch_bounds__(kBlock) void k_conv_check(DeviceState d, StepScalar
   }
   write_prefix(d.active, d.vcb, d.B, d.pref_fv);
   write_prefix(d.active, d.ecb, d.B, d.pref_fe);
+  write_mode(d.active, d.tcb, d.B, d.t_switch, d.t_mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -1565,6 +1626,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_check(DeviceState d, StepScalars
     }
   }
   write_prefix(d.aactive, d.vcb, d.B, d.pref_av);
+  write_mode(d.aactive, d.tcb, d.B, d.t_switch, d.t_mode);
 }
 
 // mu = fscale * xr of every vertex, after an adjoint loop (the bodies that
@@ -1876,26 +1938,41 @@ size_t prefix_lds(const DeviceState& d) { return sizeof(int) * static_cast<size_
 // The pair x = S^T (S rhs4) (+ base) over the items of the active bodies:
 // the S product writes the 4-wide postorder y4, the S^T product the FP64
 // [N*3] out in original order.
+// The S product, then S^T from whichever list *mode selects: the whole
+// groups, or the sliced list and its combine (the other launches return at
+// once).
 template <typename Real, typename Acc, typename In>
 void spmv_pair(const DeviceState& d, double* out, const double* base, const int* active, AdjointEpilogue adj,
-               hipStream_t stream) {
+               const int* mode, hipStream_t stream) {
   const Real* val = static_cast<const Real*>(d.fval);
   In* y4 = static_cast<In*>(d.y4);
+  Acc* part_buf = static_cast<Acc*>(d.t_part);
   if (d.s_nitems)
     hipLaunchKernelGGL((k_factor_rows<Real, Acc, In>), dim3(d.g_s), dim3(kBlock), 0, stream, d.s_items, d.s_woff,
                        d.s_list, val, static_cast<const In*>(d.rhs4), y4, active);
   if (d.t_nitems)
-    hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_t), dim3(kColWaves * kWave), 0, stream, d.t_items, d.t_woff,
-                       d.t_list, d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base, active, adj);
+    hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_t), dim3(kColWaves * kWave), 0, stream, d.t_items,
+                       d.t_woff, d.t_list, d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base, active,
+                       adj, mode, 0, part_buf);
+  if (d.ts_nitems) {
+    hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_ts), dim3(kColWaves * kWave), 0, stream, d.ts_items,
+                       d.ts_woff, d.ts_list, d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base,
+                       active, adj, mode, 1, part_buf);
+    if (d.tc_nitems)
+      hipLaunchKernelGGL(k_factor_combine<Acc>, dim3(d.g_tc), dim3(kBlock), 0, stream, d.tc_items, d.tc_nitems,
+                         static_cast<const Acc*>(part_buf), out, d.gorig, base, active, adj, mode);
+  }
 }
 
-// Dispatch on the stored precision.
+// Dispatch on the stored precision; `all` selects the list chosen for every
+// body at once instead of the active bodies' (the loops' prefix kernels set it).
 void spmv_dispatch(const DeviceState& d, double* out, const double* base, const int* active, hipStream_t stream,
-                   AdjointEpilogue adj = AdjointEpilogue{}) {
+                   AdjointEpilogue adj = AdjointEpilogue{}, bool all = false) {
+  const int* mode = d.t_mode + (all ? 1 : 0);
   if (d.f32)
-    spmv_pair<float, float, float>(d, out, base, active, adj, stream);
+    spmv_pair<float, float, float>(d, out, base, active, adj, mode, stream);
   else
-    spmv_pair<double, double, double>(d, out, base, active, adj, stream);
+    spmv_pair<double, double, double>(d, out, base, active, adj, mode, stream);
 }
 
 // Iteration products: x_raw = S^T (S rhs) (+ base, the defect-correction form).
@@ -1934,10 +2011,10 @@ void launch_active_prefix(const DeviceState& d, bool adjoint, hipStream_t stream
   const size_t lds = prefix_lds(d);
   if (adjoint)
     hipLaunchKernelGGL(k_active_prefix, dim3(1), dim3(kBlock), lds, stream, d.aactive, d.vcb, d.pref_av, nullptr,
-                       nullptr, d.B);
+                       nullptr, d.B, d.tcb, d.t_switch, d.t_mode);
   else
     hipLaunchKernelGGL(k_active_prefix, dim3(1), dim3(kBlock), lds, stream, d.active, d.vcb, d.pref_fv, d.ecb,
-                       d.pref_fe, d.B);
+                       d.pref_fe, d.B, d.tcb, d.t_switch, d.t_mode);
 }
 
 void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const double* targets,
@@ -2026,7 +2103,7 @@ void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, 
   else
     hipLaunchKernelGGL(k_pack4<double>, dim3(grid), dim3(kBlock), 0, stream, d.N, d.gpost, rhs,
                        static_cast<double*>(d.rhs4));
-  spmv_dispatch(d, out, nullptr, d.ones, stream);
+  spmv_dispatch(d, out, nullptr, d.ones, stream, AdjointEpilogue{}, true);
 }
 
 void launch_refined_inverse(const DeviceState& d, const double* rhs, double* out, double* r, double* corr, int sweeps,

@@ -437,7 +437,8 @@ void build_solver(dphd_solver* s, double dt) {
         }
       }
       if (t_entries.size() > static_cast<size_t>(INT32_MAX)) throw std::invalid_argument("too many factor rows");
-      t_items.push_back(ColGroup{c0 + gc0, e0, static_cast<int>(t_entries.size()), c0 + L.slots, b, {0, 0, 0}});
+      t_items.push_back(ColGroup{c0 + gc0, e0, static_cast<int>(t_entries.size()), c0 + L.slots, b, -1,
+                                 static_cast<int>(t_items.size()), 0});
     }
     L = FactorLayout{};
   }
@@ -604,6 +605,36 @@ void build_solver(dphd_solver* s, double dt) {
   d.t_items = dupload(o, t_items);
   d.t_nitems = static_cast<int>(t_items.size());
   d.t_entries = dupload(o, t_entries);
+  // The sliced S^T list: each group's rows cut into near-equal slices of at
+  // most slice_rows entries; a group of one slice writes its columns, the
+  // others write partial sums that one combine item per group adds in order.
+  std::vector<ColGroup> ts_items;
+  std::vector<ColCombine> tc_items;
+  int parts = 0;
+  for (int g = 0; g < static_cast<int>(t_items.size()); ++g) {
+    const ColGroup& G = t_items[g];
+    const int count = G.e1 - G.e0;
+    const int pieces = std::max(1, (count + d.slice_rows - 1) / d.slice_rows);
+    if (pieces == 1) {
+      ts_items.push_back(G);
+      continue;
+    }
+    tc_items.push_back(ColCombine{G.c0, G.end, G.body, parts, pieces, g, {0, 0}});
+    for (int k = 0; k < pieces; ++k) {
+      ColGroup it = G;
+      it.e0 = G.e0 + static_cast<int>(static_cast<long long>(count) * k / pieces);
+      it.e1 = G.e0 + static_cast<int>(static_cast<long long>(count) * (k + 1) / pieces);
+      it.part = parts + k;
+      ts_items.push_back(it);
+    }
+    parts += pieces;
+  }
+  d.ts_items = dupload(o, ts_items);
+  d.ts_nitems = static_cast<int>(ts_items.size());
+  d.tc_items = dupload(o, tc_items);
+  d.tc_nitems = static_cast<int>(tc_items.size());
+  d.t_part = dalloc<unsigned char>(o, static_cast<size_t>(std::max(parts, 1)) * 12 * kWave *
+                                          (defect ? sizeof(float) : sizeof(double)));
   {
     // Grids of the products: no more workgroups than the device holds at
     // once, so that every wavefront's list runs concurrently.
@@ -611,7 +642,13 @@ void build_solver(dphd_solver* s, double dt) {
     HIP_CHECK(product_capacity(defect, &rows_cap, &cols_cap));
     d.g_s = std::min(grid_for((d.s_nitems + 3) / 4), rows_cap >= 8 ? rows_cap & ~7 : rows_cap);
     d.g_t = std::min(grid_for(d.t_nitems), cols_cap >= 8 ? cols_cap & ~7 : cols_cap);
+    d.g_ts = std::min(grid_for(d.ts_nitems), cols_cap >= 8 ? cols_cap & ~7 : cols_cap);
+    d.g_tc = std::max(1, (d.tc_nitems + kBlock / kWave - 1) / (kBlock / kWave));
+    // The sliced list while the active bodies' groups cannot fill the device.
+    d.t_switch = cols_cap;
   }
+  const std::vector<int> mode_init = {0, static_cast<int>(t_items.size()) < d.t_switch ? 1 : 0};
+  d.t_mode = dupload(o, mode_init);
   {
     // Cost of an item in memory round trips of its wavefront: an S item
     // streams its rows in passes of 64 quads, a wavefront of an S^T group
@@ -636,6 +673,16 @@ void build_solver(dphd_solver* s, double dt) {
     balanced_lists(cost, body, B, d.g_t, 1, off, list);
     d.t_woff = dupload(o, off);
     d.t_list = dupload(o, list);
+    cost.clear();
+    body.clear();
+    for (const ColGroup& g : ts_items) {
+      const int batch = kColWaves * (defect ? 16 : 8);
+      cost.push_back(1.0 + (g.e1 - g.e0 + batch - 1) / batch);
+      body.push_back(g.body);
+    }
+    balanced_lists(cost, body, B, d.g_ts, 1, off, list);
+    d.ts_woff = dupload(o, off);
+    d.ts_list = dupload(o, list);
   }
   if (defect) {
     d.a_off = dupload(o, a_off);

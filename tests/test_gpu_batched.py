@@ -11,7 +11,7 @@
   (tests/test_setup_bitexact.py); the pure-Python factor is too slow at
   k = 16.
 * all 256 bodies of the benchmark workload batched, against single-body
-  solvers of the same bodies (same kernels, so equal to rounding).
+  solvers of the same bodies.
 * Dirichlet gradients (explicit targets and integrated pinned velocities)
   against the oracle.
 """
@@ -125,36 +125,43 @@ def test_batched_rollout_gradients_match_per_body_oracle(cuda, precision, rel_to
             assert err <= tol, (i, b["k"], key, err)
 
 
-def test_full_batch_gradients_equal_single_body_solvers(cuda):
-    """All 256 bodies of the benchmark workload in one batched solver (FP32 factor, rel 1e-6) against
-    single-body solvers of a sample of them: equal iteration counts and gradients to rounding."""
+@pytest.mark.parametrize("precision,tol,slack", [(64, 1e-6, 0), (32, 1e-5, 1)], ids=["fp64", "fp32"])
+def test_full_batch_gradients_equal_single_body_solvers(cuda, precision, tol, slack):
+    """All 256 bodies of the benchmark workload in one batched solver (rel 1e-6) against single-body solvers
+    of a sample of them. The batched solver's S^T runs over whole column groups while most bodies are
+    active and over row slices of them after, a single body's always over slices, so the products add in
+    different orders. With FP64 products the iteration counts are equal and the gradients differ only by
+    that rounding as the Anderson solves carry it (~3e-8, as against the oracle); FP32 products (the FP32
+    factor's defect correction) move the iterate the gate stops at by their own rounding."""
     sizes = np.random.default_rng(3).integers(4, 17, 256)
     bodies = config4_bodies(sizes, seed=1)
     steps = 2
-    print("[case] 256 batched bodies vs single-body solvers", flush=True)
+    print(f"[case] 256 batched bodies vs single-body solvers, fp{precision}", flush=True)
     model, voff, eoff = packed_bodies_model(bodies, cuda)
-    solver = _solver(model, rel_tol=1e-6, independent_bodies=True, factor_precision=32)
+    solver = _solver(model, rel_tol=1e-6, independent_bodies=True, factor_precision=precision)
     got = _gpu_rollout_grad(solver, model, voff, bodies, steps)
     assert got["status"] == 0
     order = np.argsort(sizes, kind="stable")
     sample = sorted(set(order[np.linspace(0, 255, 12).astype(int)]) | {int(np.argmax(sizes)), 0, 255})
+    worst = 0.0
     for i in sample:
         b = bodies[i]
         m1, v1, _ = packed_bodies_model([b], cuda)
-        s1 = _solver(m1, rel_tol=1e-6, factor_precision=32)
+        s1 = _solver(m1, rel_tol=1e-6, factor_precision=precision)
         one = _gpu_rollout_grad(s1, m1, v1, [b], steps)
         assert one["status"] == 0
-        np.testing.assert_array_equal(got["forward"][:, i], one["forward"][:, 0], err_msg=f"body {i}")
-        np.testing.assert_array_equal(got["adjoint"][:, i], one["adjoint"][:, 0], err_msg=f"body {i}")
+        assert np.all(np.abs(got["forward"][:, i] - one["forward"][:, 0]) <= slack), (i, got["forward"][:, i], one["forward"])
+        assert np.all(np.abs(got["adjoint"][:, i] - one["adjoint"][:, 0]) <= slack), (i, got["adjoint"][:, i], one["adjoint"])
         v = slice(voff[i], voff[i + 1])
         e = slice(eoff[i], eoff[i + 1])
         for key, mine, theirs in (("materials", got["materials"][e], one["materials"]), ("q", got["q"][v], one["q"]),
                                   ("qd", got["qd"][v], one["qd"])):
             scale = np.abs(theirs).max()
             err = np.abs(mine - theirs).max() / scale
-            assert err <= 1e-12, (i, b["k"], key, err)
-    print(f"[result] 256 bodies: {len(sample)} sampled bodies equal to their single-body solves, "
-          f"adjoint iterations max {got['adjoint'].max()}", flush=True)
+            worst = max(worst, err)
+            assert err <= tol, (i, b["k"], key, err)
+    print(f"[result] 256 bodies fp{precision}: {len(sample)} sampled bodies against their single-body solves, "
+          f"largest gradient difference {worst:.2e}, adjoint iterations max {got['adjoint'].max()}", flush=True)
 
 
 def _grid(cell=0.1):

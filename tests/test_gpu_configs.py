@@ -167,22 +167,95 @@ def test_config3_reduced_rollout_gradient_matches_oracle(cuda):
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
 
 
-def test_config3_full_size_fp32_factor_error(cuda):
-    print("[case] config 3 Hex8 twist 64x16x16, 100 steps, fp64 and fp32 factor", flush=True)
+def _tip_rotation_seed(scene, q):
+    """dL/dq of the tip rotation at positions q (central differences per tip coordinate)."""
+    rest = scene["vertices"]
+    x = np.asarray(q, np.float64)
+    gq = np.zeros_like(x)
+    for i in scene["tip"]:
+        for d in (1, 2):
+            xp, xm = x.copy(), x.copy()
+            xp[i, d] += 1e-7
+            xm[i, d] -= 1e-7
+            gq[i, d] = (tip_rotation(xp, rest, scene["tip"]) - tip_rotation(xm, rest, scene["tip"])) / 2e-7
+    return gq
+
+
+def test_config3_reduced_at_config_stiffness_matches_oracle(cuda):
+    """The twist release at the config's own stiffness (E 1e6, middle third 1e7) on 16x4x4 cells, 10 steps at
+    rel 1e-10: ~3000 iterations per forward step and up to ~3200 per adjoint. Forward and adjoint iteration
+    counts against the oracle's, positions, and the tip-rotation gradient w.r.t. log E (the reference records
+    no Hex8 adjoint; the oracle's Hex8 adjoint is pinned by central differences)."""
+    print("[case] config 3 Hex8 twist 16x4x4 at the config stiffness, 10 steps, rel 1e-10", flush=True)
+    scene = {**config3(16, 4, 4), "rel_tol": 1e-10, "abs_tol": 0.0}
+    steps = 10
+    body = hex_oracle(scene, max_iterations=20000, jacobian_mode="element")
+    refs = oracle_rollout(body, scene, steps, scene["q0"], np.zeros_like(scene["q0"]))
+    gq = _tip_rotation_seed(scene, refs[-1]["q"])
+    grad = oracle_rollout_grad(body, scene, refs, gq)
+    assert grad["status"] == 0 and all(r["status"] == 0 for r in refs)
+    model, solver = _hex_solver(cuda, scene, max_iterations=20000, jacobian_mode="element")
+    solver.hex_materials.requires_grad_(True)
+    solver.iteration_log = []
+    first, last, iterations = gpu_rollout(solver, model, scene["dt"], steps, scene["q0"],
+                                          np.zeros_like(scene["q0"]), track=True)
+    (last.particle_q.to(torch.float64) * torch.as_tensor(gq, device=cuda)).sum().backward()
+    adjoint = [int(it[0]) for kind, it in solver.iteration_log if kind == "adjoint"]
+    assert solver.read_status() == 0
+    fwd_ref = [r["iterations"] for r in refs]
+    dfwd = max(abs(a - b) for a, b in zip(iterations, fwd_ref))
+    dadj = max(abs(a - b) for a, b in zip(adjoint, grad["iterations"]))
+    got = log_young_gradient(solver.hex_materials.grad.cpu().numpy(), scene["materials"])
+    ref = log_young_gradient(grad["materials"], scene["materials"])
+    gerr = np.abs(got - ref).max() / np.abs(ref).max()
+    perr = np.abs(last.particle_q.detach().cpu().numpy() - refs[-1]["q"]).max()
+    print(f"[result] config 3 reduced at config stiffness: forward iterations {iterations} (oracle {fwd_ref}), "
+          f"adjoint {adjoint} (oracle {grad['iterations']}); largest differences {dfwd} / {dadj}; positions "
+          f"{perr:.2e} m; log E gradient {gerr:.2e} of its largest entry", flush=True)
+    assert dfwd <= 1 and dadj <= 1
+    assert perr <= 1e-8
+    assert gerr <= 1e-5
+
+
+def test_config3_full_size_fwd_adjoint_fp32_factor_error(cuda):
+    """64x16x16 cells, 100 steps forward and the tip-rotation adjoint w.r.t. per-cell log E, with the FP64
+    and the FP32 factor at the config's gate (rel 1e-6) and the FP64 factor at rel 1e-10 (abs 0) as the
+    yardstick of what the gate itself leaves in the gradient. Per-element Jacobian weights: with the
+    reference's per-family mean weights on this two-stiffness beam the adjoint operator is not the
+    derivative and its cascade guard fires (status 16) on the twist loss."""
+    print("[case] config 3 Hex8 twist 64x16x16, 100 steps fwd+adjoint, fp64 / fp32 / tight fp64", flush=True)
     scene = config3()
-    finals, its = {}, {}
-    for precision in (64, 32):
-        model, solver = _hex_solver(cuda, scene, precision)
-        _, last, iterations = gpu_rollout(solver, model, scene["dt"], scene["steps"], scene["q0"],
-                                          np.zeros_like(scene["q0"]))
-        assert solver.read_status() == 0, precision
-        finals[precision] = last.particle_q.cpu().numpy().astype(np.float64)
-        its[precision] = iterations
-    err = np.abs(finals[32] - finals[64]).max()
-    dit = max(abs(a - b) for a, b in zip(its[32], its[64]))
-    rot = np.degrees(tip_rotation(finals[64], scene["vertices"], scene["tip"]))
-    print(f"[result] config 3: fp32-factor vs fp64 final position error {err:.3e} m, iteration difference "
-          f"{dit}, max iterations {max(its[64])}, final tip rotation {rot:.3f} deg", flush=True)
+    results = {}
+    for precision, rel in ((64, None), (32, None), (64, 1e-10)):
+        s = scene if rel is None else {**scene, "rel_tol": rel, "abs_tol": 0.0}
+        model, solver = _hex_solver(cuda, s, precision, max_iterations=20000, jacobian_mode="element")
+        solver.hex_materials.requires_grad_(True)
+        solver.iteration_log = []
+        _, last, iterations = gpu_rollout(solver, model, s["dt"], s["steps"], s["q0"], np.zeros_like(s["q0"]),
+                                          track=True)
+        q = last.particle_q.detach().cpu().numpy().astype(np.float64)
+        gq = _tip_rotation_seed(s, q)
+        (last.particle_q.to(torch.float64) * torch.as_tensor(gq, device=cuda)).sum().backward()
+        bits = solver.read_status()
+        adjoint = [int(it[0]) for kind, it in solver.iteration_log if kind == "adjoint"]
+        g = log_young_gradient(solver.hex_materials.grad.cpu().numpy(), s["materials"])
+        results[(precision, rel)] = (q, g, iterations, adjoint, bits)
+        print(f"[result] config 3 full size fp{precision} rel {s['rel_tol']:g}: status {bits}, forward iterations "
+              f"max {max(iterations)} mean {np.mean(iterations):.1f}, adjoint max {max(adjoint)} mean "
+              f"{np.mean(adjoint):.1f}", flush=True)
+        assert bits == 0, (precision, rel, bits)
+    q64, g64, it64, _, _ = results[(64, None)]
+    q32, g32, it32, _, _ = results[(32, None)]
+    gtight = results[(64, 1e-10)][1]
+    scale = np.abs(gtight).max()
+    err = np.abs(q32 - q64).max()
+    dit = max(abs(a - b) for a, b in zip(it32, it64))
+    gerr = np.abs(g32 - g64).max() / scale
+    gate = np.abs(g64 - gtight).max() / scale
+    rot = np.degrees(tip_rotation(q64, scene["vertices"], scene["tip"]))
+    print(f"[result] config 3: fp32-factor vs fp64 final positions {err:.3e} m, forward iteration difference "
+          f"{dit}, final tip rotation {rot:.3f} deg; log E gradient fp32 vs fp64 {gerr:.3e}, fp64 at rel 1e-6 "
+          f"vs rel 1e-10 {gate:.3e} (relative to the largest entry)", flush=True)
     assert err <= 1e-5 * np.abs(scene["vertices"]).max() and dit <= 1
 
 
