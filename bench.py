@@ -489,6 +489,8 @@ def run_workload(name, args):
         "algorithmic_gbps": total_bytes / elapsed / 1e9,
         "frac_of_hbm_peak": total_bytes / elapsed / 1e9 / HBM_PEAK_GBPS,
         "loops_per_step": loops,
+        "product_tuning": {"slice_rows": solver.info.slice_rows, "slice_switch_groups": solver.info.slice_switch,
+                           "tune_seconds": solver.info.tune_seconds},
         "status": status,
     }
     print(json.dumps(result))
@@ -697,6 +699,8 @@ def main():
             "adjoint_max": int(adj_iters.max()),
         },
         "loops_per_step": run["loops"],
+        "product_tuning": {"slice_rows": solver.info.slice_rows, "slice_switch_groups": solver.info.slice_switch,
+                           "tune_seconds": solver.info.tune_seconds},
         "rollout_hbm_gbps": rollout_gbps,
         "status": run["status"],
         "gradients_finite": grad_ok,

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPHD_ABI_VERSION 4
+#define DPHD_ABI_VERSION 5
 
 enum {
   DPHD_OK = 0,
@@ -91,6 +91,8 @@ typedef struct {
   int integrate_pinned_velocities;
   int jacobian_mode;     /* 0: reference mean weights in K; 1: per-element weights (exact derivative) */
   int iteration_chunk;   /* iterations per captured graph launch; 0 picks the default */
+  int slice_rows;        /* rows per slice of the sliced S^T product; 0 = tuned at dphd_build on this solver's
+                            factor (the choice is in dphd_info), > 0 fixes it (bitwise repeatable builds) */
 } dphd_config;
 
 typedef struct {
@@ -101,6 +103,9 @@ typedef struct {
   double setup_seconds;     /* host time of the last dphd_build */
   int64_t factor_values;    /* values stored on the device for S (dense supernode blocks, zeros included) */
   int64_t factor_slots;     /* postorder slots of the device vectors (vertices plus alignment gaps) */
+  int slice_rows;           /* rows per slice of the sliced S^T product (tuned or fixed) */
+  int slice_switch;         /* the sliced S^T runs while the active bodies hold fewer column groups */
+  double tune_seconds;      /* time of the product tuning within setup_seconds (0 when fixed) */
 } dphd_info;
 
 int dphd_create(const dphd_config* config, int device, dphd_solver** out);

@@ -293,6 +293,10 @@ void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const do
                             float* gq_in, float* gqd_in, float* gf_in, double* gmat, double* gact, float* gdir,
                             const double* z, bool any_pinned, hipStream_t stream);
 void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, hipStream_t stream);
+// The S^T product alone (of y4 into xr) over the bodies flagged in `active`,
+// from the list *mode selects (0 whole groups, 1 slices and combine): the
+// launches the product tuning times.
+void launch_cols(const DeviceState& d, const int* active, const int* mode, hipStream_t stream);
 // A^{-1} rhs to FP64 accuracy in the FP32-factor mode: S^T S rhs, then
 // `sweeps` refinements out += S^T S (rhs - A out) with the FP64 A (r, corr
 // are [N*3] work vectors); the plain products in the FP64 mode.

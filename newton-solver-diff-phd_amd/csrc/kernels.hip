@@ -22,6 +22,8 @@
 // flags built in LDS at kernel start. Every reduction is a fixed-shape
 // butterfly or a chunk-ordered sum: results are bitwise repeatable run to
 // run, and no float atomics are used.
+#include <stdexcept>
+
 #include "contact.h"
 #include "kernels.h"
 #include "pd_math.h"
@@ -1942,14 +1944,11 @@ size_t prefix_lds(const DeviceState& d) { return sizeof(int) * static_cast<size_
 // groups, or the sliced list and its combine (the other launches return at
 // once).
 template <typename Real, typename Acc, typename In>
-void spmv_pair(const DeviceState& d, double* out, const double* base, const int* active, AdjointEpilogue adj,
-               const int* mode, hipStream_t stream) {
+void cols_launch(const DeviceState& d, double* out, const double* base, const int* active, AdjointEpilogue adj,
+                 const int* mode, hipStream_t stream) {
   const Real* val = static_cast<const Real*>(d.fval);
   In* y4 = static_cast<In*>(d.y4);
   Acc* part_buf = static_cast<Acc*>(d.t_part);
-  if (d.s_nitems)
-    hipLaunchKernelGGL((k_factor_rows<Real, Acc, In>), dim3(d.g_s), dim3(kBlock), 0, stream, d.s_items, d.s_woff,
-                       d.s_list, val, static_cast<const In*>(d.rhs4), y4, active);
   if (d.t_nitems)
     hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_t), dim3(kColWaves * kWave), 0, stream, d.t_items,
                        d.t_woff, d.t_list, d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base, active,
@@ -1964,10 +1963,22 @@ void spmv_pair(const DeviceState& d, double* out, const double* base, const int*
   }
 }
 
+template <typename Real, typename Acc, typename In>
+void spmv_pair(const DeviceState& d, double* out, const double* base, const int* active, AdjointEpilogue adj,
+               const int* mode, hipStream_t stream) {
+  const Real* val = static_cast<const Real*>(d.fval);
+  In* y4 = static_cast<In*>(d.y4);
+  if (d.s_nitems)
+    hipLaunchKernelGGL((k_factor_rows<Real, Acc, In>), dim3(d.g_s), dim3(kBlock), 0, stream, d.s_items, d.s_woff,
+                       d.s_list, val, static_cast<const In*>(d.rhs4), y4, active);
+  cols_launch<Real, Acc, In>(d, out, base, active, adj, mode, stream);
+}
+
 // Dispatch on the stored precision; `all` selects the list chosen for every
 // body at once instead of the active bodies' (the loops' prefix kernels set it).
 void spmv_dispatch(const DeviceState& d, double* out, const double* base, const int* active, hipStream_t stream,
                    AdjointEpilogue adj = AdjointEpilogue{}, bool all = false) {
+  if (!d.t_mode) throw std::logic_error("factor products launched before their S^T list choice was allocated");
   const int* mode = d.t_mode + (all ? 1 : 0);
   if (d.f32)
     spmv_pair<float, float, float>(d, out, base, active, adj, mode, stream);
@@ -2093,6 +2104,14 @@ void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const do
     hipLaunchKernelGGL(k_adj_parameter_add, dim3((d.E + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, d, gmat,
                        gact);
   }
+}
+
+void launch_cols(const DeviceState& d, const int* active, const int* mode, hipStream_t stream) {
+  if (!mode) throw std::logic_error("S^T launched without a list choice");
+  if (d.f32)
+    cols_launch<float, float, float>(d, d.xr, nullptr, active, AdjointEpilogue{}, mode, stream);
+  else
+    cols_launch<double, double, double>(d, d.xr, nullptr, active, AdjointEpilogue{}, mode, stream);
 }
 
 void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, hipStream_t stream) {

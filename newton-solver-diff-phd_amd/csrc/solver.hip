@@ -191,6 +191,130 @@ int chunk_size(const dphd_solver* s) { return s->cfg.iteration_chunk > 0 ? s->cf
 void build_mesh_contact(dphd_solver* s);
 void build_friction(dphd_solver* s);
 
+// The sliced S^T list of `t_items` with slices of at most `rows` entries into
+// d (items, per-workgroup lists, combine items, partial buffer, grids);
+// device buffers go to `owned`.
+void build_sliced(DeviceState& d, const std::vector<ColGroup>& t_items, int rows, int bodies, int cols_cap,
+                  bool defect, std::vector<void*>& owned) {
+  std::vector<ColGroup> ts_items;
+  std::vector<ColCombine> tc_items;
+  int parts = 0;
+  for (int g = 0; g < static_cast<int>(t_items.size()); ++g) {
+    const ColGroup& G = t_items[g];
+    const int count = G.e1 - G.e0;
+    const int pieces = std::max(1, (count + rows - 1) / rows);
+    if (pieces == 1) {
+      ts_items.push_back(G);
+      continue;
+    }
+    tc_items.push_back(ColCombine{G.c0, G.end, G.body, parts, pieces, g, {0, 0}});
+    for (int k = 0; k < pieces; ++k) {
+      ColGroup it = G;
+      it.e0 = G.e0 + static_cast<int>(static_cast<long long>(count) * k / pieces);
+      it.e1 = G.e0 + static_cast<int>(static_cast<long long>(count) * (k + 1) / pieces);
+      it.part = parts + k;
+      ts_items.push_back(it);
+    }
+    parts += pieces;
+  }
+  d.slice_rows = rows;
+  d.ts_items = dupload(owned, ts_items);
+  d.ts_nitems = static_cast<int>(ts_items.size());
+  d.tc_items = dupload(owned, tc_items);
+  d.tc_nitems = static_cast<int>(tc_items.size());
+  d.t_part = dalloc<unsigned char>(owned, static_cast<size_t>(std::max(parts, 1)) * 12 * kWave *
+                                              (defect ? sizeof(float) : sizeof(double)));
+  d.g_ts = std::min(grid_for(d.ts_nitems), cols_cap >= 8 ? cols_cap & ~7 : cols_cap);
+  d.g_tc = std::max(1, (d.tc_nitems + kBlock / kWave - 1) / (kBlock / kWave));
+  std::vector<double> cost;
+  std::vector<int> body, off, list;
+  for (const ColGroup& g : ts_items) {
+    const int batch = kColWaves * (defect ? 16 : 8);
+    cost.push_back(1.0 + (g.e1 - g.e0 + batch - 1) / batch);
+    body.push_back(g.body);
+  }
+  balanced_lists(cost, body, bodies, d.g_ts, 1, off, list);
+  d.ts_woff = dupload(owned, off);
+  d.ts_list = dupload(owned, list);
+}
+
+// The S^T shapes tuned on this solver's own factor: every candidate slice
+// length is timed (HIP events, warm launches) over prefixes of the bodies
+// holding 1, 1/2, ..., 1/16 of the column groups, with the whole groups at
+// the same levels; the slice length with the least time summed over the
+// levels is kept, and the sliced list runs below the largest active-group
+// count at which it beat the whole groups. A candidate must win by 5% to
+// replace the default slice length, so timing noise rarely changes the
+// choice (and with it the products' summation order).
+void tune_products(dphd_solver* s, const std::vector<ColGroup>& t_items, int cols_cap, bool defect) {
+  DeviceState& d = s->d;
+  const int B = s->B;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<int> tcb(B + 1, 0);
+  for (const ColGroup& g : t_items) ++tcb[g.body + 1];
+  for (int b = 0; b < B; ++b) tcb[b + 1] += tcb[b];
+  const int total = tcb[B];
+  std::vector<void*> tmp;
+  const int levels = B == 1 ? 1 : 5;
+  std::vector<const int*> act(levels);
+  std::vector<int> act_groups(levels);
+  for (int lv = 0; lv < levels; ++lv) {
+    const int target = std::max(1, total >> lv);
+    std::vector<int> a(B, 0);
+    int groups = 0;
+    for (int b = 0; b < B && groups < target; ++b) {
+      a[b] = 1;
+      groups += tcb[b + 1] - tcb[b];
+    }
+    act[lv] = dupload(tmp, a);
+    act_groups[lv] = groups;
+  }
+  const int* modes = dupload(tmp, std::vector<int>{0, 1});
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  hipStream_t st = s->stream;
+  auto timed = [&](const int* active, int list) {
+    launch_cols(d, active, modes + list, st);
+    HIP_CHECK(hipEventRecord(e0, st));
+    for (int r = 0; r < 3; ++r) launch_cols(d, active, modes + list, st);
+    HIP_CHECK(hipEventRecord(e1, st));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    return static_cast<double>(ms) / 3.0;
+  };
+  std::vector<double> whole(levels);
+  for (int lv = 0; lv < levels; ++lv) whole[lv] = timed(act[lv], 0);
+  const int candidates[] = {kSliceRows, 64, 128, 512};
+  int best = kSliceRows;
+  double best_sum = 0.0;
+  std::vector<double> best_sliced;
+  for (const int rows : candidates) {
+    build_sliced(d, t_items, rows, B, cols_cap, defect, tmp);
+    std::vector<double> sl(levels);
+    double sum = 0.0;
+    for (int lv = 0; lv < levels; ++lv) sum += (sl[lv] = timed(act[lv], 1));
+    if (rows == kSliceRows || sum < 0.95 * best_sum) {
+      best = rows;
+      best_sum = sum;
+      best_sliced = sl;
+    }
+  }
+  int threshold = 0;
+  for (int lv = 0; lv < levels; ++lv)
+    if (best_sliced[lv] < whole[lv]) threshold = std::max(threshold, lv == 0 ? INT_MAX : act_groups[lv] + 1);
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  HIP_CHECK(hipStreamSynchronize(st));
+  for (void* p : tmp) (void)hipFree(p);
+  build_sliced(d, t_items, best, B, cols_cap, defect, s->owned);
+  d.t_switch = threshold;
+  s->info.slice_rows = best;
+  s->info.slice_switch = threshold;
+  s->info.tune_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 void build_solver(dphd_solver* s, double dt) {
   const auto t0 = std::chrono::steady_clock::now();
   HIP_CHECK(hipSetDevice(s->device));
@@ -605,36 +729,7 @@ void build_solver(dphd_solver* s, double dt) {
   d.t_items = dupload(o, t_items);
   d.t_nitems = static_cast<int>(t_items.size());
   d.t_entries = dupload(o, t_entries);
-  // The sliced S^T list: each group's rows cut into near-equal slices of at
-  // most slice_rows entries; a group of one slice writes its columns, the
-  // others write partial sums that one combine item per group adds in order.
-  std::vector<ColGroup> ts_items;
-  std::vector<ColCombine> tc_items;
-  int parts = 0;
-  for (int g = 0; g < static_cast<int>(t_items.size()); ++g) {
-    const ColGroup& G = t_items[g];
-    const int count = G.e1 - G.e0;
-    const int pieces = std::max(1, (count + d.slice_rows - 1) / d.slice_rows);
-    if (pieces == 1) {
-      ts_items.push_back(G);
-      continue;
-    }
-    tc_items.push_back(ColCombine{G.c0, G.end, G.body, parts, pieces, g, {0, 0}});
-    for (int k = 0; k < pieces; ++k) {
-      ColGroup it = G;
-      it.e0 = G.e0 + static_cast<int>(static_cast<long long>(count) * k / pieces);
-      it.e1 = G.e0 + static_cast<int>(static_cast<long long>(count) * (k + 1) / pieces);
-      it.part = parts + k;
-      ts_items.push_back(it);
-    }
-    parts += pieces;
-  }
-  d.ts_items = dupload(o, ts_items);
-  d.ts_nitems = static_cast<int>(ts_items.size());
-  d.tc_items = dupload(o, tc_items);
-  d.tc_nitems = static_cast<int>(tc_items.size());
-  d.t_part = dalloc<unsigned char>(o, static_cast<size_t>(std::max(parts, 1)) * 12 * kWave *
-                                          (defect ? sizeof(float) : sizeof(double)));
+  int cols_capacity = 0;
   {
     // Grids of the products: no more workgroups than the device holds at
     // once, so that every wavefront's list runs concurrently.
@@ -642,13 +737,16 @@ void build_solver(dphd_solver* s, double dt) {
     HIP_CHECK(product_capacity(defect, &rows_cap, &cols_cap));
     d.g_s = std::min(grid_for((d.s_nitems + 3) / 4), rows_cap >= 8 ? rows_cap & ~7 : rows_cap);
     d.g_t = std::min(grid_for(d.t_nitems), cols_cap >= 8 ? cols_cap & ~7 : cols_cap);
-    d.g_ts = std::min(grid_for(d.ts_nitems), cols_cap >= 8 ? cols_cap & ~7 : cols_cap);
-    d.g_tc = std::max(1, (d.tc_nitems + kBlock / kWave - 1) / (kBlock / kWave));
-    // The sliced list while the active bodies' groups cannot fill the device.
+    // The sliced list while the active bodies' groups cannot fill the device
+    // (replaced by the tuned shapes unless slice_rows is fixed).
+    cols_capacity = cols_cap;
     d.t_switch = cols_cap;
+    build_sliced(d, t_items, s->cfg.slice_rows > 0 ? s->cfg.slice_rows : kSliceRows, B, cols_cap, defect, o);
   }
-  const std::vector<int> mode_init = {0, static_cast<int>(t_items.size()) < d.t_switch ? 1 : 0};
-  d.t_mode = dupload(o, mode_init);
+  // The list choice every S^T launch reads: allocated here, before any
+  // product runs (the frictional boundary's setup applies the inverse);
+  // the tuning at the end of the build rewrites the all-bodies entry.
+  d.t_mode = dupload(o, std::vector<int>{0, static_cast<int>(t_items.size()) < d.t_switch ? 1 : 0});
   {
     // Cost of an item in memory round trips of its wavefront: an S item
     // streams its rows in passes of 64 quads, a wavefront of an S^T group
@@ -673,16 +771,6 @@ void build_solver(dphd_solver* s, double dt) {
     balanced_lists(cost, body, B, d.g_t, 1, off, list);
     d.t_woff = dupload(o, off);
     d.t_list = dupload(o, list);
-    cost.clear();
-    body.clear();
-    for (const ColGroup& g : ts_items) {
-      const int batch = kColWaves * (defect ? 16 : 8);
-      cost.push_back(1.0 + (g.e1 - g.e0 + batch - 1) / batch);
-      body.push_back(g.body);
-    }
-    balanced_lists(cost, body, B, d.g_ts, 1, off, list);
-    d.ts_woff = dupload(o, off);
-    d.ts_list = dupload(o, list);
   }
   if (defect) {
     d.a_off = dupload(o, a_off);
@@ -790,6 +878,14 @@ void build_solver(dphd_solver* s, double dt) {
   s->info.jacobian_bytes = jac_bytes;
   s->info.factor_values = factor_values;
   s->info.factor_slots = slots;
+  s->info.slice_rows = d.slice_rows;
+  s->info.slice_switch = d.t_switch;
+  s->info.tune_seconds = 0.0;
+  if (s->cfg.slice_rows == 0 && d.t_nitems > 0) {
+    tune_products(s, t_items, cols_capacity, defect);
+    const int all_sliced = static_cast<int>(t_items.size()) < d.t_switch ? 1 : 0;
+    HIP_CHECK(hipMemcpy(d.t_mode + 1, &all_sliced, sizeof(int), hipMemcpyHostToDevice));
+  }
   s->info.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::fill(std::begin(s->loop_stats), std::end(s->loop_stats), 0);
 }
@@ -1135,6 +1231,10 @@ int dphd_create(const dphd_config* config, int device, dphd_solver** out) {
   }
   if (!std::isfinite(c.abs_tol) || c.abs_tol < 0 || !std::isfinite(c.rel_tol) || c.rel_tol < 0) {
     set_error("abs_tol and rel_tol must be finite and nonnegative");
+    return DPHD_EINVAL;
+  }
+  if (c.slice_rows < 0) {
+    set_error("slice_rows must be 0 (tuned) or positive");
     return DPHD_EINVAL;
   }
   if (c.factor_precision != 64 && c.factor_precision != 32) {

@@ -33,6 +33,7 @@ class Config(ctypes.Structure):
         ("integrate_pinned_velocities", ctypes.c_int),
         ("jacobian_mode", ctypes.c_int),
         ("iteration_chunk", ctypes.c_int),
+        ("slice_rows", ctypes.c_int),
     ]
 
 
@@ -49,6 +50,9 @@ class Info(ctypes.Structure):
         ("setup_seconds", ctypes.c_double),
         ("factor_values", ctypes.c_int64),
         ("factor_slots", ctypes.c_int64),
+        ("slice_rows", ctypes.c_int),
+        ("slice_switch", ctypes.c_int),
+        ("tune_seconds", ctypes.c_double),
     ]
 
 
@@ -104,7 +108,7 @@ SIGNATURES = {
 _lib = None
 
 
-ABI_VERSION = 4  # DPHD_ABI_VERSION of include/diffphd.h
+ABI_VERSION = 5  # DPHD_ABI_VERSION of include/diffphd.h
 
 
 def lib():

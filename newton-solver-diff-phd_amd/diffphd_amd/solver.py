@@ -83,6 +83,7 @@ class SolverDiffPhD:
         factor_precision: int = 64,
         jacobian_mode: str = "reference",
         iteration_chunk: int = 0,
+        slice_rows: int = 0,
     ):
         if material not in MATERIALS:
             raise ValueError("material must be 'pd' or 'neo_hookean'")
@@ -225,6 +226,7 @@ class SolverDiffPhD:
             integrate_pinned_velocities=int(self.integrate_pinned_velocities),
             jacobian_mode=0 if jacobian_mode == "reference" else 1,
             iteration_chunk=int(iteration_chunk),
+            slice_rows=int(slice_rows),
         )
         handle = ctypes.c_void_p()
         check(lib().dphd_create(ctypes.byref(cfg), self.device.index or 0, ctypes.byref(handle)), "create")
