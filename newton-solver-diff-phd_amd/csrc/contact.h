@@ -160,11 +160,14 @@ struct WoodburyDev {
 // Rows, values and the selection (one workgroup); `route` holds the routing
 // guard's (scale, diverged) of body 0.
 void launch_woodbury_rows(const WoodburyDev& w, const double* mu, const double* route, hipStream_t stream);
-// Seed the adjoint solve of the k-th selected row's column: rhs = J_r^T,
-// solution (sol, and the mixed iterate xr with its clip scale 1) zero,
-// iteration state of body 0 reset.
-void launch_woodbury_seed(const WoodburyDev& w, int k, double* rhs, double* sol, double* xr, double* fscale,
-                          int* aiter, int* aactive, int* aa, hipStream_t stream);
+// Seed the batch of columns first .. first + C - 1 on the C bodies (n
+// vertices each) of the batch solver: body k's right-hand side is J_r^T of
+// selected row first + k, or off (inactive) past the selected count; its
+// solution, mixed iterate and iteration state reset.
+void launch_woodbury_batch_seed(const WoodburyDev& w, int first, int bodies, int n, double* rhs, double* sol,
+                                double* xr, double* fscale, int* aiter, int* aactive, int* aa, hipStream_t stream);
+// to |= from, then from = 0 (one status word into another).
+void launch_status_merge(int* from, int* to, hipStream_t stream);
 // G alpha = J mu and z = mu - sum_k alpha_k column_k, then the safe clip of z.
 void launch_woodbury_project(const WoodburyDev& w, const double* mu, const uint8_t* free, double* z, int* status,
                              hipStream_t stream);

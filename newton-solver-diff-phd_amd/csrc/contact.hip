@@ -991,21 +991,36 @@ __global__ __launch_bounds__(256) void k_wb_rows(WoodburyDev w, const double* mu
   if (tid == 0) *w.count = s_base;
 }
 
-__global__ __launch_bounds__(256) void k_wb_seed(WoodburyDev w, int k, double* rhs, double* sol, double* xr,
-                                                 double* fscale, int* aiter, int* aactive, int* aa) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    *aiter = 0;
-    *aactive = 1;
-    aa[0] = aa[1] = 0;
-    *fscale = 1.0;
+__global__ __launch_bounds__(256) void k_wb_batch_seed(WoodburyDev w, int first, int bodies, int n, double* rhs,
+                                                       double* sol, double* xr, double* fscale, int* aiter,
+                                                       int* aactive, int* aa) {
+  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= static_cast<long long>(bodies) * n) return;
+  const int k = static_cast<int>(t / n), i = static_cast<int>(t % n);
+  const bool on = first + k < *w.count;
+  V3 v{0, 0, 0};
+  if (on) {
+    const int r = w.order[first + k];
+    if (i == w.row_node[r]) v = ld3(w.row_vec, r);
   }
-  if (i >= w.n) return;
-  const int r = w.order[k];
-  const V3 v = i == w.row_node[r] ? ld3(w.row_vec, r) : V3{0, 0, 0};
-  st3(rhs, i, v);
-  st3(sol, i, V3{0, 0, 0});
-  st3(xr, i, V3{0, 0, 0});
+  const int j = static_cast<int>(t);
+  st3(rhs, j, v);
+  st3(sol, j, V3{0, 0, 0});
+  st3(xr, j, V3{0, 0, 0});
+  if (i == 0) {
+    aiter[k] = 0;
+    aactive[k] = on ? 1 : 0;
+    aa[2 * k] = aa[2 * k + 1] = 0;
+    fscale[k] = 1.0;
+  }
+}
+
+__global__ void k_status_merge(int* from, int* to) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int bits = *from;
+    if (bits) atomicOr(to, bits);
+    *from = 0;
+  }
 }
 
 // _gram (adjoint.py:404-425) and the Eigen LDLT of G over the selected rows.
@@ -1145,10 +1160,15 @@ void launch_woodbury_rows(const WoodburyDev& w, const double* mu, const double* 
   hipLaunchKernelGGL(k_wb_rows, dim3(1), dim3(256), 0, stream, w, mu, route);
 }
 
-void launch_woodbury_seed(const WoodburyDev& w, int k, double* rhs, double* sol, double* xr, double* fscale,
-                          int* aiter, int* aactive, int* aa, hipStream_t stream) {
-  hipLaunchKernelGGL(k_wb_seed, dim3(blocks_for(w.n)), dim3(256), 0, stream, w, k, rhs, sol, xr, fscale, aiter,
-                     aactive, aa);
+void launch_woodbury_batch_seed(const WoodburyDev& w, int first, int bodies, int n, double* rhs, double* sol,
+                                double* xr, double* fscale, int* aiter, int* aactive, int* aa, hipStream_t stream) {
+  const long long total = static_cast<long long>(bodies) * n;
+  hipLaunchKernelGGL(k_wb_batch_seed, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, stream, w,
+                     first, bodies, n, rhs, sol, xr, fscale, aiter, aactive, aa);
+}
+
+void launch_status_merge(int* from, int* to, hipStream_t stream) {
+  hipLaunchKernelGGL(k_status_merge, dim3(1), dim3(64), 0, stream, from, to);
 }
 
 void launch_woodbury_project(const WoodburyDev& w, const double* mu, const uint8_t* free, double* z, int* status,

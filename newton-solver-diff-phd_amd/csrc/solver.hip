@@ -143,7 +143,7 @@ struct dphd_solver {
   // buffers of the projection
   dphd::WoodburyDev wb;
   double *rec_mult = nullptr, *rec_omega = nullptr, *rec_normal = nullptr;
-  double *wb_mu = nullptr, *wb_rhs = nullptr, *wb_z = nullptr;
+  double *wb_mu = nullptr, *wb_z = nullptr;
   int wb_capacity = 0;
   bool have_topology = false, have_particles = false, have_materials = false;
   // build products
@@ -166,10 +166,18 @@ struct dphd_solver {
   int last_iters[3] = {0, 0, 0};
   // Per loop mode since the last build: iteration chunks launched, host waits.
   int64_t loop_stats[6] = {0, 0, 0, 0, 0, 0};
+  // Every body a copy of body 0 (the factor is computed once).
+  bool identical_bodies = false;
+  // The Woodbury columns' solver: copies of this (single) body, one column
+  // per copy, solved together; its checkpoint and fibre targets replicated.
+  dphd_solver* wb_batch = nullptr;
+  double* wb_ckpt = nullptr;
+  double* wb_targets = nullptr;
 
   ~dphd_solver() { release(); }
 
   void release() {
+    drop_woodbury_batch();
     for (auto& e : exec) {
       if (e) (void)hipGraphExecDestroy(e);
       e = nullptr;
@@ -181,6 +189,8 @@ struct dphd_solver {
     any_host = nullptr;
     built = false;
   }
+
+  void drop_woodbury_batch();
 };
 
 namespace dphd {
@@ -450,6 +460,7 @@ void build_solver(dphd_solver* s, double dt) {
     for (int t = 0; t < nthreads; ++t)
       pool.emplace_back([&]() {
         for (int b = next++; b < B; b = next++) {
+          if (s->identical_bodies && b > 0) continue;  // copies of body 0, filled in below
           try {
             const int v0 = s->voff[b], v1 = s->voff[b + 1], e0 = s->eoff[b], e1 = s->eoff[b + 1];
             std::vector<int> local(static_cast<size_t>(e1 - e0) * K);
@@ -492,6 +503,12 @@ void build_solver(dphd_solver* s, double dt) {
   }
   for (int b = 0; b < B; ++b)
     if (!errors[b].empty()) throw std::invalid_argument(errors[b]);
+  if (s->identical_bodies)
+    for (int b = 1; b < B; ++b) {
+      amats[b] = amats[0];
+      factor_entries[b] = factor_entries[0];
+      layouts[b] = layouts[0];
+    }
   // Global factor blocks: body b's postorder slots are offset by slot0[b]
   // (a multiple of 4), its values by the values before it.
   int64_t znnz = 0, annz = 0;
@@ -1022,7 +1039,6 @@ void build_friction(dphd_solver* s) {
   s->rec_omega = dzeros<double>(o, k1);
   s->rec_normal = dzeros<double>(o, 3 * k1);
   s->wb_mu = dzeros<double>(o, 3 * static_cast<size_t>(n));
-  s->wb_rhs = dzeros<double>(o, 3 * static_cast<size_t>(n));
   s->wb_z = dzeros<double>(o, 3 * static_cast<size_t>(n));
   WoodburyDev& w = s->wb;
   w = WoodburyDev{};
@@ -1152,16 +1168,63 @@ void run_adjoint_direct(dphd_solver* s, const DeviceState& d) {
   launch_adjoint_finalize(d, s->stream);
 }
 
+// The solver of the Woodbury columns: up to 64 copies of this single body
+// (the reference solves its columns 64 at a time, adjoint.py:662-663), built
+// on the first backward that selects rows and kept until this solver is
+// rebuilt. It shares this solver's stream; its factor is computed once.
+dphd_solver* woodbury_batch(dphd_solver* s) {
+  if (s->wb_batch) return s->wb_batch;
+  const int C = std::max(1, std::min(64, s->wb.R));
+  const int N = s->N, E = s->E, K = s->K;
+  auto* b = new dphd_solver;
+  b->cfg = s->cfg;
+  b->device = s->device;
+  b->K = s->K;
+  b->Q = s->Q;
+  b->B = C;
+  b->N = C * N;
+  b->E = C * E;
+  for (int c = 0; c <= C; ++c) {
+    b->voff.push_back(c * N);
+    b->eoff.push_back(c * E);
+  }
+  for (int c = 0; c < C; ++c) {
+    for (const int v : s->indices) b->indices.push_back(v + c * N);
+    b->poses.insert(b->poses.end(), s->poses.begin(), s->poses.end());
+    b->inv_mass.insert(b->inv_mass.end(), s->inv_mass.begin(), s->inv_mass.end());
+    b->gravity.insert(b->gravity.end(), s->gravity.begin(), s->gravity.begin() + 3);
+    b->density.push_back(s->density[0]);
+    b->materials.insert(b->materials.end(), s->materials.begin(), s->materials.end());
+    b->mdir_in.insert(b->mdir_in.end(), s->mdir_in.begin(), s->mdir_in.end());
+    b->mstiff_in.insert(b->mstiff_in.end(), s->mstiff_in.begin(), s->mstiff_in.end());
+  }
+  b->have_topology = b->have_particles = b->have_materials = true;
+  b->identical_bodies = true;
+  b->stream = s->stream;
+  try {
+    build_solver(b, s->dt);
+  } catch (...) {
+    b->stream = nullptr;
+    delete b;
+    throw;
+  }
+  s->wb_ckpt = dalloc<double>(s->owned, static_cast<size_t>(C) * 3 * N);
+  s->wb_targets = s->d.has_muscles ? dalloc<double>(s->owned, static_cast<size_t>(C) * E) : nullptr;
+  s->wb_batch = b;
+  return b;
+}
+
 // Woodbury projection of the adjoint for the parameter gradients; returns z
-// (Adjoint._woodbury, adjoint.py:1272-1354). d.sol holds mu and d.arhs the
-// seed on entry and again on return.
-const double* woodbury(dphd_solver* s) {
+// (Adjoint._woodbury, adjoint.py:1272-1354). The selected rows' columns
+// W_T = (A - K)^{-1} J^T are solved in batches, one column per body of the
+// batch solver, each stopping on its own gate after at most
+// min(max_iterations, 120) iterations (adjoint.py:662-806, 966-1052).
+const double* woodbury(dphd_solver* s, const double* checkpoint, const double* targets) {
   DeviceState& d = s->d;
   WoodburyDev& w = s->wb;
   hipStream_t st = s->stream;
   const size_t vb = sizeof(double) * 3 * static_cast<size_t>(s->N);
   HIP_CHECK(hipMemcpyAsync(s->wb_mu, d.sol, vb, hipMemcpyDeviceToDevice, st));
-  HIP_CHECK(hipMemcpyAsync(s->wb_rhs, d.arhs, vb, hipMemcpyDeviceToDevice, st));
   launch_woodbury_rows(w, s->wb_mu, d.route, st);
   int count = 0;
   HIP_CHECK(hipMemcpyAsync(&count, w.count, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1170,15 +1233,50 @@ const double* woodbury(dphd_solver* s) {
     w.columns = dalloc<double>(s->owned, static_cast<size_t>(count) * 3 * s->N);
     s->wb_capacity = count;
   }
-  for (int k = 0; k < count; ++k) {
-    launch_woodbury_seed(w, k, d.arhs, d.sol, d.xr, d.fscale, d.aiter, d.aactive, d.aa, st);
-    launch_active_prefix(d, true, st);
-    run_loop(s, kWoodbury);
-    HIP_CHECK(hipMemcpyAsync(w.columns + static_cast<size_t>(k) * 3 * s->N, d.sol, vb, hipMemcpyDeviceToDevice, st));
+  if (count > 0) {
+    dphd_solver* b = woodbury_batch(s);
+    DeviceState& bd = b->d;
+    const int C = b->B;
+    // The tolerances may have changed between steps (dphd_set_tolerances).
+    b->cfg.max_iterations = s->cfg.max_iterations;
+    b->cfg.abs_tol = s->cfg.abs_tol;
+    b->cfg.rel_tol = s->cfg.rel_tol;
+    if (!bd.jac) {
+      bd.jac = dalloc<double>(b->owned, static_cast<size_t>(b->E) * b->Q * 81);
+      bd.ke = dalloc<double>(b->owned, static_cast<size_t>(b->E) * b->K * b->K * 9);
+      bd.kval = dalloc<double>(b->owned, static_cast<size_t>(bd.kblocks) * 9);
+      for (int m = kAdjoint; m <= kWoodbury; ++m) {
+        if (b->exec[m]) HIP_CHECK(hipGraphExecDestroy(b->exec[m]));
+        b->exec[m] = nullptr;
+      }
+    }
+    // The adjoint operator at the recorded step, once for every copy.
+    for (int c = 0; c < C; ++c) {
+      HIP_CHECK(hipMemcpyAsync(s->wb_ckpt + static_cast<size_t>(c) * 3 * s->N, checkpoint, vb,
+                               hipMemcpyDeviceToDevice, st));
+      if (s->wb_targets && targets)
+        HIP_CHECK(hipMemcpyAsync(s->wb_targets + static_cast<size_t>(c) * s->E, targets, sizeof(double) * s->E,
+                                 hipMemcpyDeviceToDevice, st));
+    }
+    StepScalars sc = scalars(b);
+    launch_adjoint_setup(bd, sc, s->wb_ckpt, s->wb_targets && targets ? s->wb_targets : nullptr, nullptr, nullptr, st);
+    for (int first = 0; first < count; first += C) {
+      launch_woodbury_batch_seed(w, first, C, s->N, bd.arhs, bd.sol, bd.xr, bd.fscale, bd.aiter, bd.aactive, bd.aa,
+                                 st);
+      launch_active_prefix(bd, true, st);
+      const int64_t chunks0 = b->loop_stats[2 * kWoodbury], waits0 = b->loop_stats[2 * kWoodbury + 1];
+      run_loop(b, kWoodbury);
+      s->loop_stats[2 * kWoodbury] += b->loop_stats[2 * kWoodbury] - chunks0;
+      s->loop_stats[2 * kWoodbury + 1] += b->loop_stats[2 * kWoodbury + 1] - waits0;
+      const int here = std::min(C, count - first);
+      HIP_CHECK(hipMemcpyAsync(w.columns + static_cast<size_t>(first) * 3 * s->N, bd.sol,
+                               vb * static_cast<size_t>(here), hipMemcpyDeviceToDevice, st));
+    }
+    // The columns' safe clips report in this solver's status (as the
+    // reference's batched columns share the adjoint's status word).
+    launch_status_merge(bd.status, d.status, st);
   }
   launch_woodbury_project(w, s->wb_mu, d.free, s->wb_z, d.status, st);
-  HIP_CHECK(hipMemcpyAsync(d.sol, s->wb_mu, vb, hipMemcpyDeviceToDevice, st));
-  HIP_CHECK(hipMemcpyAsync(d.arhs, s->wb_rhs, vb, hipMemcpyDeviceToDevice, st));
   return s->wb_z;
 }
 
@@ -1251,6 +1349,20 @@ int dphd_create(const dphd_config* config, int device, dphd_solver** out) {
     s->Q = c.element_kind == DPHD_TET4 ? 1 : 8;
     *out = s;
   })
+}
+
+void dphd_solver::drop_woodbury_batch() {
+  if (!wb_batch) return;
+  dphd_solver* b = wb_batch;
+  wb_batch = nullptr;
+  wb_ckpt = wb_targets = nullptr;  // owned by this solver, freed with its buffers
+  (void)hipSetDevice(device);
+  if (stream) (void)hipStreamSynchronize(stream);
+  b->stream = nullptr;  // this solver's stream
+  b->release();
+  if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+  if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+  delete b;
 }
 
 void dphd_destroy(dphd_solver* s) {
@@ -1688,7 +1800,8 @@ int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle
     dphd::launch_adjoint_route(s->d, s->stream);
     // Only parameter gradients use z; state gradients use mu (adjoint.py:1145-1148).
     const double* z = nullptr;
-    if ((gmaterials || gactivations) && s->has_friction && s->fc.K > 0) z = dphd::woodbury(s);
+    if ((gmaterials || gactivations) && s->has_friction && s->fc.K > 0)
+      z = dphd::woodbury(s, checkpoint, s->d.has_muscles ? s->targets : nullptr);
     dphd::launch_adjoint_outputs(s->d, sc, checkpoint, gqd_out, gq_in, gqd_in, gf_in, gmaterials, gactivations,
                                  gdirichlet, z, s->any_pinned, s->stream);
     HIP_CHECK(hipGetLastError());
