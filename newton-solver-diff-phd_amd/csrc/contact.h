@@ -7,15 +7,17 @@
 // q~ += h^2/m f; it has no gradient.
 //
 // Layout: the vertices of each face set (sorted, unique) and the faces are
-// device arrays; the pairs within the contact radius are found by a hit
-// bitmask per direction, row-major [vertex][face word] (one 64-bit word per
-// 64 faces, built by wavefront ballots) and its transpose [face][vertex word].
-// Broad phase: each face word's bounding box (widened by the radius) is
-// built per evaluation, and a vertex outside it skips the word's 64 tests.
-// A particle's force then sums the forces of its own hit pairs in face order
-// and subtracts the reactions of the hit pairs on the faces it is a corner
-// of in vertex order; hit pairs are recomputed where they are summed, so no
-// per-pair force matrix is stored and no float atomics are used.
+// device arrays. Broad phase: per direction, a uniform grid over the other
+// side's faces, rebuilt per evaluation, whose cells are at least as large as
+// every face's box widened by the contact radius (so a face overlaps at most
+// 2 x 2 x 2 cells); each cell lists the faces whose widened box overlaps it
+// and the vertices inside it, both ascending. A vertex within the radius of
+// a face lies in the face's widened box, hence in one of its cells: a
+// particle's own pairs are tested over its cell's faces in face order, the
+// reactions on a face over its cells' vertices merged in vertex order, so
+// every sum keeps the reference's order and nothing of size V x F is stored
+// (pair forces are recomputed where they are summed; no float atomics).
+// A non-finite position leaves the grid off and every pair is tested.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,17 +26,23 @@
 
 namespace dphd {
 
+constexpr int kGridDim = 32;                               // cells per axis at most
+constexpr int kGridCells = kGridDim * kGridDim * kGridDim;
+
 struct MeshContactSide {
   int nv = 0;                 // vertices of this side's faces
   int nf = 0;                 // faces of the other side
   const int* verts = nullptr; // [nv] particle ids (sorted)
   const int* faces = nullptr; // [nf*3] the other side's faces
   double kn = 0.0;            // normal stiffness of this side's vertices
-  uint64_t* rows = nullptr;   // [nv * row_words] hits by vertex
-  uint64_t* cols = nullptr;   // [nf * col_words] hits by face
-  int row_words = 0;          // ceil(nf / 64)
-  int col_words = 0;          // ceil(nv / 64)
-  double* boxes = nullptr;    // [row_words * 6] per 64-face word: lower, upper corner (widened)
+  double* grid = nullptr;     // [11]: origin (3), cell size, cells per axis (3), 1 = on, upper bounds (3)
+  int* fcell_off = nullptr;   // [kGridCells + 1] faces per cell (CSR over cells)
+  int* fcell = nullptr;       // [8 * nf]
+  int* vcell_off = nullptr;   // [kGridCells + 1] vertices per cell
+  int* vcell = nullptr;       // [nv] vertex slots
+  int* cursor = nullptr;      // [2 * kGridCells] fill positions (faces, vertices)
+  int* unsorted = nullptr;    // [8 * nf + nv] cell entries in arrival order (faces, then vertices)
+  int* vcell_of = nullptr;    // [nv] cell of each vertex, -1 outside every widened face box
 };
 
 struct MeshContactDev {

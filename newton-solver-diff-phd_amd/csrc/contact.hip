@@ -1,19 +1,25 @@
 // gfx950 kernels of the mesh contact state force (reference:
 // src/newton_solver_diff_phd/_src/solvers/diff_phd/mesh_contact.py:34-179).
 //
-//   k_mc_rows       one wavefront per (vertex, 64 faces): the pair test of
-//                   _pair_forces (:90-137) per lane, hits as a ballot word
-//   k_mc_transpose  the hit words by face
-//   k_mc_gather     _gather_forces (:140-170) + add_state_force (:173-177):
-//                   a particle's own hit pairs in face order, then the
-//                   reactions on the faces it is a corner of in vertex order
+//   k_mc_grid      per direction, the grid over the other side's faces (their
+//                  boxes widened by the contact radius) and whether every
+//                  position is finite
+//   k_mc_bin / k_mc_scan / k_mc_rank
+//                  faces and vertices binned into the cells (counting sort),
+//                  each cell's lists ascending
+//   k_mc_gather    _gather_forces (:140-170) + add_state_force (:173-177):
+//                  a particle's own pairs (_pair_forces, :90-137) over its
+//                  cell's faces in face order, then the reactions on the faces
+//                  it is a corner of over their cells' vertices in vertex order
 //
-// Pair forces are recomputed where they are summed from the same inputs with
-// floating-point contraction off, so the hit test and the force agree
-// bitwise between the kernels and every sum has the reference's order
-// (pairs outside the radius contribute exact zeros there and are skipped).
+// Pair forces are computed with floating-point contraction off, so the hit
+// test and the force are those of the reference's arithmetic and every sum
+// has the reference's order (pairs outside the radius contribute exact zeros
+// there and are skipped).
 #include "contact.h"
 #include "pd_math.h"
+
+#include <climits>
 
 namespace dphd {
 namespace {
@@ -141,121 +147,296 @@ __global__ void k_mc_zero(int* pairs) {
   pairs[1] = 0;
 }
 
-// Broad phase: the bounds of each word of 64 faces (faces in index order),
-// widened by the contact radius times 1 + 1e-6. A vertex outside a word's
-// box is farther than the radius from every face of the word in exact
-// arithmetic, and the margin covers the rounding of the closest-point
-// distance, so skipping the word drops no pair the exact test would hit.
-__global__ __launch_bounds__(256) void k_mc_boxes(MeshContactSide side, Vec3Src pos, double reach) {
-  const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
-  const int lane = threadIdx.x & 63;
-  if (wave >= side.row_words) return;  // uniform per wavefront
-  const int f = static_cast<int>(wave) * 64 + lane;
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  bool finite = true;
-  if (f < side.nf) {
+__device__ __forceinline__ bool finite3(V3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+
+// Cell range of one axis for the interval [lo, hi] (clamped to the grid).
+__device__ __forceinline__ int cell_of(double x, double origin, double h, int dims) {
+  const double t = floor((x - origin) / h);
+  return t < 0.0 ? 0 : (t >= dims ? dims - 1 : static_cast<int>(t));
+}
+
+// The grid of one direction (one workgroup): the bounds of the other side's
+// face boxes widened by `reach`, the largest widened box edge, and whether
+// every face corner and vertex is finite. Cell size: the largest box edge,
+// or the bounds' largest edge over kGridDim when that is larger.
+__global__ __launch_bounds__(256) void k_mc_grid(MeshContactSide side, Vec3Src pos, double reach) {
+  __shared__ double red[7][256];
+  __shared__ int bad[256];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, edge = 0.0;
+  int nonfinite = 0;
+  for (int f = threadIdx.x; f < side.nf; f += blockDim.x) {
+    double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int k = 0; k < 3; ++k) {
       const V3 v = pos(side.faces[3 * f + k]);
+      nonfinite |= !finite3(v);
       const double c[3] = {v.x, v.y, v.z};
       for (int d = 0; d < 3; ++d) {
-        finite = finite && isfinite(c[d]);
-        lo[d] = fmin(lo[d], c[d]);
-        hi[d] = fmax(hi[d], c[d]);
+        blo[d] = fmin(blo[d], c[d]);
+        bhi[d] = fmax(bhi[d], c[d]);
       }
     }
-  }
-  for (int o = 32; o > 0; o >>= 1)
     for (int d = 0; d < 3; ++d) {
-      lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
-      hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+      lo[d] = fmin(lo[d], blo[d] - reach);
+      hi[d] = fmax(hi[d], bhi[d] + reach);
+      edge = fmax(edge, (bhi[d] + reach) - (blo[d] - reach));
     }
-  // A non-finite corner leaves the word unbounded (never skipped), so the
-  // exact test sees it as before.
-  const bool all_finite = __ballot(!finite) == 0;
-  if (lane < 3) side.boxes[6 * wave + lane] = all_finite ? lo[lane] - reach : -INFINITY;
-  else if (lane < 6) side.boxes[6 * wave + lane] = all_finite ? hi[lane - 3] + reach : INFINITY;
+  }
+  for (int s = threadIdx.x; s < side.nv; s += blockDim.x) nonfinite |= !finite3(pos(side.verts[s]));
+  for (int d = 0; d < 3; ++d) {
+    red[d][threadIdx.x] = lo[d];
+    red[3 + d][threadIdx.x] = hi[d];
+  }
+  red[6][threadIdx.x] = edge;
+  bad[threadIdx.x] = nonfinite;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (static_cast<int>(threadIdx.x) < off) {
+      for (int d = 0; d < 3; ++d) {
+        red[d][threadIdx.x] = fmin(red[d][threadIdx.x], red[d][threadIdx.x + off]);
+        red[3 + d][threadIdx.x] = fmax(red[3 + d][threadIdx.x], red[3 + d][threadIdx.x + off]);
+      }
+      red[6][threadIdx.x] = fmax(red[6][threadIdx.x], red[6][threadIdx.x + off]);
+      bad[threadIdx.x] |= bad[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double* g = side.grid;
+    const bool on = !bad[0] && side.nf > 0;
+    double h = red[6][0];
+    for (int d = 0; d < 3; ++d) h = fmax(h, (red[3 + d][0] - red[d][0]) / kGridDim);
+    // The margin keeps a widened box within two cells per axis whatever the
+    // rounding of the cell coordinates (they are at most kGridDim + 1).
+    h *= 1.0 + 1.0e-9;
+    if (!(h > 0.0)) h = 1.0;
+    for (int d = 0; d < 3; ++d) {
+      g[d] = on ? red[d][0] : 0.0;
+      g[8 + d] = on ? red[3 + d][0] : 0.0;
+      const double cells = on ? floor((red[3 + d][0] - red[d][0]) / h) + 1.0 : 1.0;
+      g[4 + d] = fmin(cells, static_cast<double>(kGridDim));
+    }
+    g[3] = h;
+    g[7] = on ? 1.0 : 0.0;
+  }
 }
 
-__global__ __launch_bounds__(256) void k_mc_rows(MeshContactSide side, Vec3Src pos, Vec3Src vel, double radius,
-                                                 double kf, double mu, int* pairs) {
-  const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
-  const int lane = threadIdx.x & 63;
-  if (wave >= static_cast<long long>(side.nv) * side.row_words) return;  // uniform per wavefront
-  const int s = static_cast<int>(wave / side.row_words), word = static_cast<int>(wave % side.row_words);
-  const size_t at = static_cast<size_t>(s) * side.row_words + word;
-  const V3 p = pos(side.verts[s]);
-  const double* box = side.boxes + 6 * word;
-  if (p.x < box[0] || p.y < box[1] || p.z < box[2] || p.x > box[3] || p.y > box[4] || p.z > box[5]) {
-    if (lane == 0) side.rows[at] = 0;
-    return;
-  }
-  const int f = word * 64 + lane;
-  bool hit = false;
-  if (f < side.nf) hit = pair_force(pos, vel, side.verts[s], side.faces + 3 * f, radius, side.kn, kf, mu).hit;
-  const uint64_t mask = __ballot(hit);
-  if (lane == 0) {
-    side.rows[at] = mask;
-    if (mask) atomicAdd(pairs, __popcll(mask));
+__global__ __launch_bounds__(256) void k_mc_clear(MeshContactSide side) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= kGridCells; c += gridDim.x * blockDim.x) {
+    side.fcell_off[c] = 0;
+    side.vcell_off[c] = 0;
   }
 }
 
-__global__ __launch_bounds__(256) void k_mc_transpose(MeshContactSide side) {
-  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= static_cast<long long>(side.nf) * side.col_words) return;
-  const int f = static_cast<int>(t / side.col_words), cw = static_cast<int>(t % side.col_words);
-  const int fw = f >> 6, fb = f & 63;
-  uint64_t word = 0;
-  for (int b = 0; b < 64; ++b) {
-    const int s = cw * 64 + b;
-    if (s >= side.nv) break;
-    word |= ((side.rows[static_cast<size_t>(s) * side.row_words + fw] >> fb) & 1ull) << b;
+struct CellBox {
+  int lo[3], hi[3];
+};
+
+// The cells a face's widened box overlaps.
+__device__ __forceinline__ CellBox face_cells(const MeshContactSide& side, const Vec3Src& pos, int f, double reach) {
+  const double* g = side.grid;
+  CellBox cb;
+  for (int d = 0; d < 3; ++d) {
+    double blo = INFINITY, bhi = -INFINITY;
+    for (int k = 0; k < 3; ++k) {
+      const V3 v = pos(side.faces[3 * f + k]);
+      const double c = d == 0 ? v.x : (d == 1 ? v.y : v.z);
+      blo = fmin(blo, c);
+      bhi = fmax(bhi, c);
+    }
+    const int dims = static_cast<int>(g[4 + d]);
+    cb.lo[d] = cell_of(blo - reach, g[d], g[3], dims);
+    cb.hi[d] = min(cell_of(bhi + reach, g[d], g[3], dims), cb.lo[d] + 1);
   }
-  side.cols[static_cast<size_t>(f) * side.col_words + cw] = word;
+  return cb;
 }
 
-// The particle's own pairs (as a vertex of `side`) in face order.
+__device__ __forceinline__ int cell_index(const double* g, int x, int y, int z) {
+  return (x * static_cast<int>(g[5]) + y) * static_cast<int>(g[6]) + z;
+}
+
+// The cell of a vertex, or -1 when it lies outside the widened boxes' bounds
+// (then no face is within the radius).
+__device__ __forceinline__ int vertex_cell(const MeshContactSide& side, V3 p) {
+  const double* g = side.grid;
+  const double c[3] = {p.x, p.y, p.z};
+  int idx[3];
+  for (int d = 0; d < 3; ++d) {
+    if (c[d] < g[d] || c[d] > g[8 + d]) return -1;
+    idx[d] = cell_of(c[d], g[d], g[3], static_cast<int>(g[4 + d]));
+  }
+  return cell_index(g, idx[0], idx[1], idx[2]);
+}
+
+// Counts per cell (into *_off[cell + 1]); pass 1 writes the entries at the
+// fill positions of the arrival-order lists instead.
+__global__ __launch_bounds__(256) void k_mc_bin(MeshContactSide side, Vec3Src pos, double reach, int pass) {
+  if (side.grid[7] == 0.0) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < side.nf) {
+    const CellBox cb = face_cells(side, pos, t, reach);
+    for (int x = cb.lo[0]; x <= cb.hi[0]; ++x)
+      for (int y = cb.lo[1]; y <= cb.hi[1]; ++y)
+        for (int z = cb.lo[2]; z <= cb.hi[2]; ++z) {
+          const int c = cell_index(side.grid, x, y, z);
+          if (pass == 0)
+            atomicAdd(side.fcell_off + c + 1, 1);
+          else
+            side.unsorted[atomicAdd(side.cursor + c, 1)] = t;
+        }
+  } else if (t < side.nf + side.nv) {
+    const int s = t - side.nf;
+    const int c = vertex_cell(side, pos(side.verts[s]));
+    if (pass == 0) {
+      side.vcell_of[s] = c;
+      if (c >= 0) atomicAdd(side.vcell_off + c + 1, 1);
+    } else if (c >= 0) {
+      side.unsorted[8 * side.nf + atomicAdd(side.cursor + kGridCells + c, 1)] = s;
+    }
+  }
+}
+
+// Prefix sums of both count arrays (one workgroup; each thread a contiguous
+// run of cells, then a scan of the run totals) and the fill positions.
+__global__ __launch_bounds__(256) void k_mc_scan(MeshContactSide side) {
+  if (side.grid[7] == 0.0) return;
+  __shared__ int run[2][256];
+  constexpr int per = kGridCells / 256;
+  const int c0 = threadIdx.x * per;
+  for (int a = 0; a < 2; ++a) {
+    int* off = a == 0 ? side.fcell_off : side.vcell_off;
+    int total = 0;
+    for (int c = c0; c < c0 + per; ++c) total += off[c + 1];
+    run[a][threadIdx.x] = total;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    int acc = 0;
+    for (int t = 0; t < 256; ++t) {
+      const int v = run[threadIdx.x][t];
+      run[threadIdx.x][t] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  for (int a = 0; a < 2; ++a) {
+    int* off = a == 0 ? side.fcell_off : side.vcell_off;
+    int acc = run[a][threadIdx.x];
+    for (int c = c0; c < c0 + per; ++c) {
+      acc += off[c + 1];
+      off[c + 1] = acc;
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < kGridCells; c += blockDim.x) {
+    side.cursor[c] = side.fcell_off[c];
+    side.cursor[kGridCells + c] = side.vcell_off[c];
+  }
+}
+
+// Each cell's lists ascending: an entry's place is the number of smaller
+// entries of its cell (entries of a cell are distinct), one thread per entry.
+__global__ __launch_bounds__(256) void k_mc_rank(MeshContactSide side) {
+  if (side.grid[7] == 0.0) return;
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool face = j < 8 * side.nf;
+  if (!face) j -= 8 * side.nf;
+  const int* off = face ? side.fcell_off : side.vcell_off;
+  if (j >= off[kGridCells]) return;
+  const int* in = side.unsorted + (face ? 0 : 8 * side.nf);
+  int lo = 0, hi = kGridCells;  // the cell c with off[c] <= j < off[c + 1]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (off[mid] <= j)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int v = in[j];
+  int rank = 0;
+  for (int i = off[lo]; i < off[lo + 1]; ++i) rank += in[i] < v;
+  (face ? side.fcell : side.vcell)[off[lo] + rank] = v;
+}
+
+// The particle's own pairs (as vertex s of `side`) in face order; counts the hits.
 __device__ __forceinline__ V3 own_pairs(const MeshContactSide& side, int s, const Vec3Src& pos, const Vec3Src& vel,
-                                        double radius, double kf, double mu, V3 total) {
-  for (int word = 0; word < side.row_words; ++word) {
-    uint64_t m = side.rows[static_cast<size_t>(s) * side.row_words + word];
-    while (m) {
-      const int f = word * 64 + __builtin_ctzll(m);
-      m &= m - 1;
-      total = total + pair_force(pos, vel, side.verts[s], side.faces + 3 * f, radius, side.kn, kf, mu).force;
+                                        double radius, double kf, double mu, V3 total, int& hits) {
+  const bool on = side.grid[7] != 0.0;
+  int b = 0, e = side.nf;
+  if (on) {
+    const int c = side.vcell_of[s];
+    if (c < 0) return total;
+    b = side.fcell_off[c];
+    e = side.fcell_off[c + 1];
+  }
+  for (int k = b; k < e; ++k) {
+    const int f = on ? side.fcell[k] : k;
+    const PairForce h = pair_force(pos, vel, side.verts[s], side.faces + 3 * f, radius, side.kn, kf, mu);
+    if (h.hit) {
+      total = total + h.force;
+      ++hits;
     }
   }
   return total;
 }
 
 // Reactions on the particle as corner `corner` of face f of `side`'s faces,
-// over the side's vertices in order.
+// over the side's vertices in order: the vertex lists of the face's cells
+// (disjoint, each ascending) merged.
 __device__ __forceinline__ V3 face_reactions(const MeshContactSide& side, int f, int corner, const Vec3Src& pos,
-                                             const Vec3Src& vel, double radius, double kf, double mu, V3 total) {
-  for (int cw = 0; cw < side.col_words; ++cw) {
-    uint64_t m = side.cols[static_cast<size_t>(f) * side.col_words + cw];
-    while (m) {
-      const int s = cw * 64 + __builtin_ctzll(m);
-      m &= m - 1;
+                                             const Vec3Src& vel, double radius, double kf, double mu, double reach,
+                                             V3 total) {
+  if (side.grid[7] == 0.0) {
+    for (int s = 0; s < side.nv; ++s) {
       const PairForce h = pair_force(pos, vel, side.verts[s], side.faces + 3 * f, radius, side.kn, kf, mu);
-      total = total - vget(h.bary, corner) * h.force;
+      if (h.hit) total = total - vget(h.bary, corner) * h.force;
     }
+    return total;
+  }
+  const CellBox cb = face_cells(side, pos, f, reach);
+  int at[8], end[8], lists = 0;
+  for (int x = cb.lo[0]; x <= cb.hi[0]; ++x)
+    for (int y = cb.lo[1]; y <= cb.hi[1]; ++y)
+      for (int z = cb.lo[2]; z <= cb.hi[2]; ++z) {
+        const int c = cell_index(side.grid, x, y, z);
+        if (lists < 8 && side.vcell_off[c] < side.vcell_off[c + 1]) {
+          at[lists] = side.vcell_off[c];
+          end[lists] = side.vcell_off[c + 1];
+          ++lists;
+        }
+      }
+  while (true) {
+    int pick = -1, best = INT_MAX;
+    for (int l = 0; l < lists; ++l)
+      if (at[l] < end[l] && side.vcell[at[l]] < best) {
+        best = side.vcell[at[l]];
+        pick = l;
+      }
+    if (pick < 0) break;
+    ++at[pick];
+    const PairForce h = pair_force(pos, vel, side.verts[best], side.faces + 3 * f, radius, side.kn, kf, mu);
+    if (h.hit) total = total - vget(h.bary, corner) * h.force;
   }
   return total;
 }
 
 __global__ __launch_bounds__(256) void k_mc_gather(MeshContactDev mc, Vec3Src pos, Vec3Src vel, double* target,
-                                                   const int* vbody, const double* mass, double dt) {
+                                                   const int* vbody, const double* mass, double dt, double reach) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= mc.n) return;
   V3 total{0, 0, 0};
-  if (mc.slot_a[i] >= 0) total = own_pairs(mc.ab, mc.slot_a[i], pos, vel, mc.radius, mc.kf, mc.mu, total);
-  if (mc.slot_b[i] >= 0) total = own_pairs(mc.ba, mc.slot_b[i], pos, vel, mc.radius, mc.kf, mc.mu, total);
+  int hits_ab = 0, hits_ba = 0;
+  if (mc.slot_a[i] >= 0) total = own_pairs(mc.ab, mc.slot_a[i], pos, vel, mc.radius, mc.kf, mc.mu, total, hits_ab);
+  if (mc.slot_b[i] >= 0) total = own_pairs(mc.ba, mc.slot_b[i], pos, vel, mc.radius, mc.kf, mc.mu, total, hits_ba);
+  if (hits_ab) atomicAdd(mc.pairs, hits_ab);
+  if (hits_ba) atomicAdd(mc.pairs + 1, hits_ba);
   // As a corner of a face of B pushed by A's vertices, then of A by B's.
   for (int k = mc.inc_b_off[i]; k < mc.inc_b_off[i + 1]; ++k)
-    total = face_reactions(mc.ab, mc.inc_b[2 * k], mc.inc_b[2 * k + 1], pos, vel, mc.radius, mc.kf, mc.mu, total);
+    total = face_reactions(mc.ab, mc.inc_b[2 * k], mc.inc_b[2 * k + 1], pos, vel, mc.radius, mc.kf, mc.mu, reach,
+                           total);
   for (int k = mc.inc_a_off[i]; k < mc.inc_a_off[i + 1]; ++k)
-    total = face_reactions(mc.ba, mc.inc_a[2 * k], mc.inc_a[2 * k + 1], pos, vel, mc.radius, mc.kf, mc.mu, total);
+    total = face_reactions(mc.ba, mc.inc_a[2 * k], mc.inc_a[2 * k + 1], pos, vel, mc.radius, mc.kf, mc.mu, reach,
+                           total);
   mc.force[3 * i] = total.x;
   mc.force[3 * i + 1] = total.y;
   mc.force[3 * i + 2] = total.z;
@@ -1089,20 +1270,23 @@ void launch_mesh_contact(const MeshContactDev& mc, const float* pos32, const flo
                          const double* vel64, double* target, const int* vbody, const double* mass, double dt,
                          hipStream_t stream) {
   const Vec3Src pos{pos32, pos64}, vel{vel32, vel64};
+  // Widened by 1 + 1e-6: the margin covers the rounding of the closest-point
+  // distance, so no pair the exact test hits lies outside a face's box.
+  const double reach = mc.radius * (1.0 + 1.0e-6);
   hipLaunchKernelGGL(k_mc_zero, dim3(1), dim3(1), 0, stream, mc.pairs);
   for (int d = 0; d < 2; ++d) {
-    const MeshContactSide* side = d == 0 ? &mc.ab : &mc.ba;
-    const long long waves = static_cast<long long>(side->nv) * side->row_words;
-    if (waves == 0) continue;
-    hipLaunchKernelGGL(k_mc_boxes, dim3(blocks_for(side->row_words * 64LL)), dim3(256), 0, stream, *side, pos,
-                       mc.radius * (1.0 + 1.0e-6));
-    hipLaunchKernelGGL(k_mc_rows, dim3(blocks_for(waves * 64)), dim3(256), 0, stream, *side, pos, vel, mc.radius,
-                       mc.kf, mc.mu, mc.pairs + d);
-    hipLaunchKernelGGL(k_mc_transpose, dim3(blocks_for(static_cast<long long>(side->nf) * side->col_words)),
-                       dim3(256), 0, stream, *side);
+    const MeshContactSide& side = d == 0 ? mc.ab : mc.ba;
+    if (side.nv == 0 || side.nf == 0) continue;
+    hipLaunchKernelGGL(k_mc_grid, dim3(1), dim3(256), 0, stream, side, pos, reach);
+    hipLaunchKernelGGL(k_mc_clear, dim3(blocks_for(kGridCells + 1)), dim3(256), 0, stream, side);
+    const long long items = static_cast<long long>(side.nf) + side.nv;
+    hipLaunchKernelGGL(k_mc_bin, dim3(blocks_for(items)), dim3(256), 0, stream, side, pos, reach, 0);
+    hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(256), 0, stream, side);
+    hipLaunchKernelGGL(k_mc_bin, dim3(blocks_for(items)), dim3(256), 0, stream, side, pos, reach, 1);
+    hipLaunchKernelGGL(k_mc_rank, dim3(blocks_for(8LL * side.nf + side.nv)), dim3(256), 0, stream, side);
   }
   hipLaunchKernelGGL(k_mc_gather, dim3(blocks_for(mc.n)), dim3(256), 0, stream, mc, pos, vel, target, vbody, mass,
-                     dt);
+                     dt, reach);
 }
 
 }  // namespace dphd

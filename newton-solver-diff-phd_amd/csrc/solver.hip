@@ -955,11 +955,16 @@ void build_mesh_contact(dphd_solver* s) {
     sd.verts = verts;
     sd.faces = faces;
     sd.kn = kn;
-    sd.row_words = (nf + 63) / 64;
-    sd.col_words = (nv + 63) / 64;
-    sd.rows = dzeros<uint64_t>(o, static_cast<size_t>(nv) * sd.row_words);
-    sd.cols = dzeros<uint64_t>(o, static_cast<size_t>(nf) * sd.col_words);
-    sd.boxes = dzeros<double>(o, static_cast<size_t>(sd.row_words) * 6);
+    // A face's widened box spans at most two cells per axis (the cell edge is
+    // at least the largest box edge), so it is binned into at most 8 cells.
+    sd.grid = dzeros<double>(o, 11);
+    sd.fcell_off = dzeros<int>(o, kGridCells + 1);
+    sd.fcell = dzeros<int>(o, 8 * static_cast<size_t>(std::max(nf, 1)));
+    sd.vcell_off = dzeros<int>(o, kGridCells + 1);
+    sd.vcell = dzeros<int>(o, static_cast<size_t>(std::max(nv, 1)));
+    sd.cursor = dzeros<int>(o, 2 * static_cast<size_t>(kGridCells));
+    sd.vcell_of = dzeros<int>(o, static_cast<size_t>(std::max(nv, 1)));
+    sd.unsorted = dzeros<int>(o, 8 * static_cast<size_t>(nf) + nv + 1);
     return sd;
   };
   mc.ab = side(dva, static_cast<int>(va.size()), dfb, fb, s->mc_params[1]);

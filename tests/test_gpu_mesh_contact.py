@@ -97,9 +97,9 @@ def test_invalid_mesh_contact_is_rejected(cuda):
 
 
 def test_broad_phase_keeps_every_pair_of_a_wide_scene(cuda):
-    """A 12 x 12 slab under a 4 x 4 plate: most of the slab's face words lie out of the plate's reach
-    (and the other way round), so the face-word boxes skip them; forces and pair counts must equal
-    the oracle's brute-force evaluation on perturbed states."""
+    """A 12 x 12 slab under a 4 x 4 plate: most of the slab's vertices lie outside the plate's
+    widened face boxes and most grid cells hold no pair, so the broad phase skips them; forces and
+    pair counts must equal the oracle's brute-force evaluation on perturbed states."""
     from diffphd_amd import meshes, parity
 
     print("[case] mesh contact broad phase", flush=True)
