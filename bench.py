@@ -413,6 +413,20 @@ def run_workload(name, args):
     t0 = time.time()
     solver.prepare(sc["dt"])
     setup = time.time() - t0
+    s_bytes = {64: 8, 32: 4}[precision]
+    z = float(solver.body_factor_nnz()[0])
+    probe = factor_probe(solver, dev, n, z, s_bytes, args)
+    products = {
+        "launch_us": probe["launch_s"] * 1e6,
+        "algorithmic_gbps": probe["launch_bytes"] / probe["launch_s"] / 1e9,
+        "frac": probe["launch_bytes"] / probe["launch_s"] / 1e9 / HBM_PEAK_GBPS,
+        "stored_gbps": probe["stored_bytes"] / probe["launch_s"] / 1e9,
+        "bytes_per_launch": probe["launch_bytes"],
+        "stored_bytes_per_launch": probe["stored_bytes"],
+    }
+    if args.probe_only:
+        print(json.dumps({"workload": name, "products": products, "copy_ceiling_gbps": copy_ceiling(dev)}))
+        return
     if sc["adjoint"] and sc["kind"] == "hex8":
         materials.requires_grad_(True)
 
@@ -462,8 +476,6 @@ def run_workload(name, args):
     solver.iteration_log = None
     fwd = np.array([int(it[0]) for kind, it in log_ if kind == "forward"])
     adj = np.array([int(it[0]) for kind, it in log_ if kind == "adjoint"])
-    z = float(solver.body_factor_nnz()[0])
-    s_bytes = {64: 8, 32: 4}[precision]
     total_bytes = fwd.sum() * bytes_iteration(n, elements, z, s_bytes, corners, samples, False)
     if sc["adjoint"]:
         total_bytes += adj.sum() * bytes_iteration(n, elements, z, s_bytes, corners, samples, True)
@@ -491,9 +503,79 @@ def run_workload(name, args):
         "loops_per_step": loops,
         "product_tuning": {"slice_rows": solver.info.slice_rows, "slice_switch_groups": solver.info.slice_switch,
                            "tune_seconds": solver.info.tune_seconds},
+        "products": products,
         "status": status,
+        "cpu_baseline": None if args.no_cpu_baseline else workload_cpu_baseline(name, sc, args),
     }
     print(json.dumps(result))
+
+
+# Steps of the bounded CPU sample per configuration (each about 5-60 s of oracle time on 16 cores).
+WORKLOAD_CPU_STEPS = {"config1": 25, "config2": 10, "config3": 3, "config5": 20}
+
+
+def workload_cpu_baseline(name, sc, args):
+    """FP64 C oracle on the host cores: the configuration's first steps from its initial state, then the
+    backward sweep of its loss (adjoint configurations), timed without setup."""
+    import torch
+
+    from diffphd_amd import _lib
+    from oracle import oracle
+
+    steps = args.workload_cpu_steps or WORKLOAD_CPU_STEPS[name]
+    threads = min(16, os.cpu_count() or 1)
+    n = len(sc["v"])
+    kw = dict(materials=sc["mats"], density=sc["density"], inverse_mass=sc["free"].astype(np.float32),
+              gravity=sc["gravity"], dt=sc["dt"], rel_tol=sc["rel_tol"], max_iterations=10000,
+              jacobian_mode="element", threads=threads, factor=lambda csr: _lib.Factor(*csr).as_dict())
+    if sc["kind"] == "hex8":
+        body = oracle.OracleBody("hex8", sc["hexes"], rest=sc["v"], **kw)
+    elif "sphere" in sc:
+        body = oracle.OracleBody("tet4", sc["t"], poses=sc["p"], frictional_boundary=("spherical", sc["sphere"](0)),
+                                 frictional_vertices=sc["top"], friction=sc["friction"], **kw)
+    else:
+        body = oracle.OracleBody("tet4", sc["t"], poses=sc["p"], **kw)
+    q, qd = sc["q0"].astype(np.float32), np.zeros((n, 3), np.float32)
+    records, iterations = [], 0
+    t0 = time.perf_counter()
+    for t in range(steps):
+        if "sphere" in sc:
+            body.contact["params"] = np.asarray(sc["sphere"](t), dtype=np.float64)
+        f = None
+        if name == "config1":
+            f = np.zeros((n, 3), np.float32)
+            f[sc["tip"], 0] = 80.0 * t / 25
+        fw = body.forward(sc["dt"], q, qd, f)
+        iterations += fw["iterations"]
+        records.append(fw)
+        q, qd = fw["q"], fw["qd"]
+    if sc["adjoint"]:
+        if name == "config3":
+            qt = torch.tensor(q, dtype=torch.float64, requires_grad=True)
+            tip_rotation_torch(qt, sc["v"], sc["tip"]).backward()
+            gq = qt.grad.numpy().astype(np.float32)
+        else:
+            gq = np.zeros_like(q)
+            gq[sc["top"]] = 2.0 * (q[sc["top"]] - sc["target"])
+        gqd = np.zeros_like(q)
+        for t in reversed(range(steps)):
+            if "sphere" in sc:
+                body.contact["params"] = np.asarray(sc["sphere"](t), dtype=np.float64)
+            bw = body.backward(sc["dt"], records[t]["x"], gq, gqd, materials=True)
+            iterations += bw["iterations"]
+            gq, gqd = bw["q"], bw["qd"]
+    seconds = time.perf_counter() - t0
+    what = "forward" if not sc["adjoint"] else "forward + the backward sweep"
+    return {
+        "value": steps / seconds,
+        "unit": "steps/s",
+        "cores": threads,
+        "kind": "port",
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(),
+        "sample": f"the configuration's first {steps} steps ({what}, {iterations} iterations) on the FP64 oracle; "
+                  f"{seconds:.1f} s",
+    }
 
 
 def main():
@@ -519,6 +601,8 @@ def main():
     ap.add_argument("--workload", default="config4", choices=("config1", "config2", "config3", "config4", "config5"),
                     help="config4 (default) is the headline batched workload; the others print their own line")
     ap.add_argument("--workload-steps", type=int, default=0, help="steps of a --workload run (0 = the config's)")
+    ap.add_argument("--workload-cpu-steps", type=int, default=0,
+                    help="steps of a --workload run's CPU sample (0 = the configuration's default)")
     ap.add_argument("--probe-only", action="store_true",
                     help="only the factor-product probe and the copy calibration (for counter runs)")
     args = ap.parse_args()
@@ -559,6 +643,8 @@ def main():
 
     run = timed_batched(model, dev, dt, idx, tgt, args, 1e-6, 1e-9)
     solver, elapsed = run["solver"], run["elapsed"]
+    tuning = {"slice_rows": solver.info.slice_rows, "slice_switch_groups": solver.info.slice_switch,
+              "tune_seconds": solver.info.tune_seconds}
     log(f"[bench] batched setup {run['setup']:.1f}s, nnz(S) {solver.info.factor_nnz}; rollout {elapsed:.3f}s for "
         f"{args.steps} fwd+adj steps, loss {run['loss']:.6e}, status {run['status']}, loops/step {run['loops']}")
     fwd_iters, adj_iters = run["fwd"], run["adj"]
@@ -699,8 +785,7 @@ def main():
             "adjoint_max": int(adj_iters.max()),
         },
         "loops_per_step": run["loops"],
-        "product_tuning": {"slice_rows": solver.info.slice_rows, "slice_switch_groups": solver.info.slice_switch,
-                           "tune_seconds": solver.info.tune_seconds},
+        "product_tuning": tuning,
         "rollout_hbm_gbps": rollout_gbps,
         "status": run["status"],
         "gradients_finite": grad_ok,

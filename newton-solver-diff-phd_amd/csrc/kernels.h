@@ -208,7 +208,7 @@ struct DeviceState {
   // adjoint loop. Written by the seed and check kernels.
   int *pref_fv = nullptr, *pref_fe = nullptr, *pref_av = nullptr;
   int* status = nullptr;    // [1]
-  int* any_active = nullptr;  // mapped host int
+  int* any_active = nullptr;  // mapped host ints [2]: any body active, most iterations run
   // adjoint
   double* jac = nullptr;     // [E*Q*81]
   // Assembled adjoint operator K (minus A in the defect-correction mode):
@@ -271,7 +271,7 @@ struct FrictionDev;
 // A^{-1}(b + J^T lambda) before the Anderson step.
 void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const double* targets,
                               const FrictionDev* fc, hipStream_t stream);
-void launch_any_active(const DeviceState& d, const int* active, hipStream_t stream);
+void launch_any_active(const DeviceState& d, const int* active, const int* iters, hipStream_t stream);
 // Work-list prefixes after the active flags were set from outside a check
 // kernel (adjoint: pref_av from aactive, else pref_fv and pref_fe from active).
 void launch_active_prefix(const DeviceState& d, bool adjoint, hipStream_t stream);

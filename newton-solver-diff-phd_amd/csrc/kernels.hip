@@ -425,15 +425,29 @@ __global__ __launch_bounds__(kBlock) void k_write_state(DeviceState d, double dt
   }
 }
 
-__global__ void k_any_active(const int* active, int B, int* out) {
-  __shared__ int flag;
-  if (threadIdx.x == 0) flag = 0;
+// out[0]: whether any body is still active; out[1]: the most iterations any
+// body has run (the host sizes the next solve's first burst of chunks by it).
+__global__ __launch_bounds__(kBlock) void k_any_active(const int* active, const int* iters, int B, int* out) {
+  __shared__ int flag[kBlock], most[kBlock];
+  int mine = 0, top = 0;
+  for (int b = threadIdx.x; b < B; b += blockDim.x) {
+    mine |= active[b];
+    top = max(top, iters[b]);
+  }
+  flag[threadIdx.x] = mine;
+  most[threadIdx.x] = top;
   __syncthreads();
-  int mine = 0;
-  for (int b = threadIdx.x; b < B; b += blockDim.x) mine |= active[b];
-  if (mine) flag = 1;  // benign same-value store
-  __syncthreads();
-  if (threadIdx.x == 0) *out = flag;
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (static_cast<int>(threadIdx.x) < off) {
+      flag[threadIdx.x] |= flag[threadIdx.x + off];
+      most[threadIdx.x] = max(most[threadIdx.x], most[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = flag[0];
+    out[1] = most[0];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2040,8 +2054,8 @@ void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const 
   hipLaunchKernelGGL(k_conv_check, dim3(1), dim3(kBlock), lds, stream, d, s);
 }
 
-void launch_any_active(const DeviceState& d, const int* active, hipStream_t stream) {
-  hipLaunchKernelGGL(k_any_active, dim3(1), dim3(kBlock), 0, stream, active, d.B, d.any_active);
+void launch_any_active(const DeviceState& d, const int* active, const int* iters, hipStream_t stream) {
+  hipLaunchKernelGGL(k_any_active, dim3(1), dim3(kBlock), 0, stream, active, iters, d.B, d.any_active);
 }
 
 void launch_write_state(const DeviceState& d, const StepScalars& s, float* q, float* qd, double* checkpoint,

@@ -866,9 +866,10 @@ void build_solver(dphd_solver* s, double dt) {
   if (s->has_mesh_contact) build_mesh_contact(s);
   if (s->has_friction) build_friction(s);
   void* host = nullptr;
-  HIP_CHECK(hipHostMalloc(&host, sizeof(int), hipHostMallocMapped));
+  HIP_CHECK(hipHostMalloc(&host, 2 * sizeof(int), hipHostMallocMapped));
   s->any_host = static_cast<int*>(host);
-  *s->any_host = 0;
+  s->any_host[0] = 0;
+  s->any_host[1] = 0;
   void* dev = nullptr;
   HIP_CHECK(hipHostGetDevicePointer(&dev, host, 0));
   d.any_active = static_cast<int*>(dev);
@@ -1112,7 +1113,8 @@ hipGraphExec_t capture(dphd_solver* s, int mode) {
     else
       launch_forward_iteration(s->d, sc, s->targets, s->has_friction ? &s->fc : nullptr, s->stream);
   }
-  launch_any_active(s->d, mode != kForward ? s->d.aactive : s->d.active, s->stream);
+  launch_any_active(s->d, mode != kForward ? s->d.aactive : s->d.active, mode != kForward ? s->d.aiter : s->d.iter,
+                    s->stream);
   HIP_CHECK(hipStreamEndCapture(s->stream, &graph));
   hipGraphExec_t exec = nullptr;
   HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1135,9 +1137,11 @@ int run_loop(dphd_solver* s, int mode) {
   const int C = chunk_size(s);
   const int max_chunks = (loop_limit(s, mode) + C - 1) / C;
   const int last = s->last_iters[mode];
-  // One chunk fewer than last time, so a step that converges faster wastes
-  // at most one predicated-off chunk.
-  int first = std::max(1, std::min(max_chunks, last / C - 1));
+  // First burst: the chunks the previous solve needed (its most iterations of
+  // any body, rounded up to whole chunks), so a solve that needs as many
+  // waits once; one that converges sooner runs predicated-off chunks (every
+  // kernel exits at its first flag read) instead of a host round trip.
+  int first = std::max(1, std::min(max_chunks, (last + C - 1) / C));
   int launched = 0;
   while (launched < max_chunks) {
     const int burst = launched == 0 ? first : 1;
@@ -1147,7 +1151,7 @@ int run_loop(dphd_solver* s, int mode) {
     if (!*reinterpret_cast<volatile int*>(s->any_host)) break;
   }
   s->loop_stats[2 * mode] += launched;
-  s->last_iters[mode] = launched * C;
+  s->last_iters[mode] = reinterpret_cast<volatile int*>(s->any_host)[1];
   if (mode != kForward) launch_adjoint_finalize(s->d, s->stream);
   return launched;
 }
@@ -1163,7 +1167,7 @@ void run_adjoint_direct(dphd_solver* s, const DeviceState& d) {
   const int max_chunks = (sc.max_iterations + C - 1) / C;
   for (int launched = 0; launched < max_chunks; ++launched) {
     for (int c = 0; c < C; ++c) launch_adjoint_iteration(d, sc, s->stream);
-    launch_any_active(d, d.aactive, s->stream);
+    launch_any_active(d, d.aactive, d.aiter, s->stream);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s->stream));
     s->loop_stats[2 * kAdjoint] += 1;
