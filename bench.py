@@ -292,6 +292,17 @@ def timed_batched(model, dev, dt, idx, tgt, args, rel_tol, abs_tol):
     elapsed = time.perf_counter() - start
     status = solver.read_status()
     grads_model = model.tet_materials.grad.detach().clone()
+    # The forward alone (no tape): the same K steps from rest.
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    with torch.no_grad():
+        state = model.state()
+        for _ in range(args.steps):
+            out = model.state()
+            solver.step(state, out, None, None, dt)
+            state = out
+    torch.cuda.synchronize()
+    forward_elapsed = time.perf_counter() - start
     before = solver.loop_stats()
     solver.iteration_log = []
     model.tet_materials.grad = None
@@ -304,7 +315,7 @@ def timed_batched(model, dev, dt, idx, tgt, args, rel_tol, abs_tol):
     adj = np.array([it for kind, it in log_ if kind == "adjoint"])
     loops = {m: {k: (after[m][k] - before[m][k]) / args.steps for k in after[m]} for m in after}
     return dict(solver=solver, setup=setup, elapsed=elapsed, status=status, loss=float(loss.detach()), q0=q0,
-                qd0=qd0, fwd=fwd, adj=adj, loops=loops)
+                qd0=qd0, fwd=fwd, adj=adj, loops=loops, forward_elapsed=forward_elapsed)
 
 
 # ---------------------------------------------------------------------------
@@ -470,6 +481,20 @@ def run_workload(name, args):
     elapsed = time.perf_counter() - start
     after = solver.loop_stats()
     status = solver.read_status()
+    forward_rate = None
+    if sc["adjoint"]:
+        with torch.no_grad():
+            torch.cuda.synchronize()
+            start = time.perf_counter()
+            state = model.state()
+            for t in range(steps):
+                if "sphere" in sc:
+                    solver.boundary_parameters = sc["sphere"](t)
+                out = model.state()
+                solver.step(state, out, None, None, sc["dt"])
+                state = out
+            torch.cuda.synchronize()
+            forward_rate = steps / (time.perf_counter() - start)
     solver.iteration_log = []
     one_rollout(steps)
     log_ = solver.iteration_log
@@ -488,6 +513,7 @@ def run_workload(name, args):
         "config_steps": sc["steps"] if name != "config5" else 200,
         "steps_per_s": steps / elapsed,
         "ms_per_step": 1e3 * elapsed / steps,
+        "forward_steps_per_s": forward_rate if sc["adjoint"] else steps / elapsed,
         "setup_s": setup,
         "factor_precision": precision,
         "vertices": n,
@@ -502,6 +528,7 @@ def run_workload(name, args):
         "frac_of_hbm_peak": total_bytes / elapsed / 1e9 / HBM_PEAK_GBPS,
         "loops_per_step": loops,
         "product_tuning": {"slice_rows": solver.info.slice_rows, "slice_switch_groups": solver.info.slice_switch,
+                           "split_quads": solver.info.split_quads, "split_switch_groups": solver.info.split_switch,
                            "tune_seconds": solver.info.tune_seconds},
         "products": products,
         "status": status,
@@ -644,6 +671,7 @@ def main():
     run = timed_batched(model, dev, dt, idx, tgt, args, 1e-6, 1e-9)
     solver, elapsed = run["solver"], run["elapsed"]
     tuning = {"slice_rows": solver.info.slice_rows, "slice_switch_groups": solver.info.slice_switch,
+              "split_quads": solver.info.split_quads, "split_switch_groups": solver.info.split_switch,
               "tune_seconds": solver.info.tune_seconds}
     log(f"[bench] batched setup {run['setup']:.1f}s, nnz(S) {solver.info.factor_nnz}; rollout {elapsed:.3f}s for "
         f"{args.steps} fwd+adj steps, loss {run['loss']:.6e}, status {run['status']}, loops/step {run['loops']}")
@@ -775,6 +803,7 @@ def main():
             "factor_nnz": int(z_total),
             "parallelism": "single GPU, bodies batched per launch",
         },
+        "forward_steps_per_s": args.steps / run["forward_elapsed"],
         "plain_steps_per_s": plain,
         "speedup_vs_plain": None if plain is None else value / plain,
         "element_iterations_per_s": elem_iters / elapsed,

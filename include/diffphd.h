@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPHD_ABI_VERSION 5
+#define DPHD_ABI_VERSION 6
 
 enum {
   DPHD_OK = 0,
@@ -106,6 +106,8 @@ typedef struct {
   int slice_rows;           /* rows per slice of the sliced S^T product (tuned or fixed) */
   int slice_switch;         /* the sliced S^T runs while the active bodies hold fewer column groups */
   double tune_seconds;      /* time of the product tuning within setup_seconds (0 when fixed) */
+  int split_quads;          /* quads per column chunk of the split S product (tuned or the default) */
+  int split_switch;         /* the split S runs while the active bodies hold fewer column groups */
 } dphd_info;
 
 int dphd_create(const dphd_config* config, int device, dphd_solver** out);

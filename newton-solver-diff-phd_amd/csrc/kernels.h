@@ -42,8 +42,8 @@ constexpr int kColWaves = 4;  // wavefronts of an S^T workgroup
 
 // S product work item (one wavefront): rows out_row .. out_row + count - 1
 // (count <= kRowsPerGroup) of one block, whose values start at `offset`
-// with 4*quads values per row from column col0. Self-contained, so a
-// wavefront needs one load per item.
+// with 4*quads values per row from column col0 (rows 4*stride values
+// apart). Self-contained, so a wavefront needs one load per item.
 struct RowItem {
   long long offset;
   int col0;
@@ -51,8 +51,25 @@ struct RowItem {
   int out_row;
   int count;
   int body;
+  int part;    // -1: the item stores its rows; else its partial-sum slot (a chunk of a split item)
+  int stride;  // quads per stored row of the item's block (= quads unless a chunk of a split item)
   int pad;
 };
+
+// Combine item of an S item split into column chunks: rows out_row ..
+// out_row + count - 1 of body `body`, the chunks' partial sums in slots
+// part0 .. part0 + parts - 1 (column order), each [32] = 8 rows x 4 values.
+struct RowCombine {
+  int out_row;
+  int count;
+  int body;
+  int part0;
+  int parts;
+  int pad[3];
+};
+
+// Quads per column chunk of the split S list; the default of the build.
+constexpr int kSplitQuads = 256;
 
 // One row of one block as an S^T column group sees it: the row's values
 // start at `rowoff` with `quads` quads from column col0, and its S-product
@@ -159,11 +176,26 @@ struct DeviceState {
   void* t_part = nullptr;
   int g_ts = 0, g_tc = 0;
   int slice_rows = kSliceRows;
-  // Which S^T list runs: t_mode[0] is written with each active-body prefix
-  // (1 = sliced, when the active bodies hold fewer than t_switch column
-  // groups); t_mode[1] is the constant choice for every body at once.
+  // The split S list: items wider than split_quads quads cut into column
+  // chunks that write partial sums (ss_items, per-wavefront lists
+  // ss_woff/ss_list over g_ss workgroups), the combine items of the split
+  // ones (sc_items, one thread per value) and their partial sums (s_part,
+  // [parts][32] of the accumulation type).
+  RowItem* ss_items = nullptr;
+  int ss_nitems = 0;
+  int *ss_woff = nullptr, *ss_list = nullptr;
+  RowCombine* sc_items = nullptr;
+  int sc_nitems = 0;
+  void* s_part = nullptr;
+  int g_ss = 0;
+  int split_quads = kSplitQuads;
+  // Which lists run: t_mode[0] (S^T) and t_mode[2] (S) are written with
+  // each active-body prefix (1 = sliced / split, when the active bodies hold
+  // fewer than t_switch / s_switch column groups); t_mode[1] and t_mode[3]
+  // are the constant choices for every body at once.
   int* t_mode = nullptr;
   int t_switch = 0;
+  int s_switch = 0;
   bool f32 = false;           // FP32 factor values
   // constant matrix for the defect correction (per coordinate)
   int *a_off = nullptr, *a_col = nullptr;
@@ -297,6 +329,9 @@ void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, 
 // from the list *mode selects (0 whole groups, 1 slices and combine): the
 // launches the product tuning times.
 void launch_cols(const DeviceState& d, const int* active, const int* mode, hipStream_t stream);
+// The S product alone (of rhs4 into y4), from the list *mode selects (0
+// whole items, 1 split items and their combine): timed by the tuning too.
+void launch_rows(const DeviceState& d, const int* active, const int* mode, hipStream_t stream);
 // A^{-1} rhs to FP64 accuracy in the FP32-factor mode: S^T S rhs, then
 // `sweeps` refinements out += S^T S (rhs - A out) with the FP64 A (r, corr
 // are [N*3] work vectors); the plain products in the FP64 mode.

@@ -291,19 +291,22 @@ __device__ void write_prefix(const int* active, const int* base, int B, int* out
 // The S^T list for the next products (t_mode[0]): sliced while the active
 // bodies hold fewer than `threshold` column groups (tcb: each body's groups).
 // Every thread of the workgroup calls it.
-__device__ void write_mode(const int* active, const int* tcb, int B, int threshold, int* t_mode) {
+__device__ void write_mode(const int* active, const int* tcb, int B, int t_switch, int s_switch, int* t_mode) {
   extern __shared__ int pref[];
   __syncthreads();
   const int groups = active_prefix(active, tcb, B, pref);
-  if (threadIdx.x == 0) t_mode[0] = groups < threshold ? 1 : 0;
+  if (threadIdx.x == 0) {
+    t_mode[0] = groups < t_switch ? 1 : 0;
+    t_mode[2] = groups < s_switch ? 1 : 0;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_active_prefix(const int* active, const int* base0, int* out0,
                                                           const int* base1, int* out1, int B, const int* tcb,
-                                                          int threshold, int* t_mode) {
+                                                          int t_switch, int s_switch, int* t_mode) {
   write_prefix(active, base0, B, out0);
   if (base1) write_prefix(active, base1, B, out1);
-  write_mode(active, tcb, B, threshold, t_mode);
+  write_mode(active, tcb, B, t_switch, s_switch, t_mode);
 }
 
 // Loop `fn(item, body)` over the active items listed by the prefix `pref_g`
@@ -668,14 +671,17 @@ __device__ __forceinline__ double lane_real(double v, int lane) {
 
 // One S item on one wavefront: ROWS rows; a lane takes one quad of every
 // row per pass, and all loads of a pass are issued before its products
-// (rows past the item's count re-read its last row and are dropped).
+// (rows past the item's count re-read its last row and are dropped). A
+// chunk of a split item (part >= 0) stores its 32 sums to its partial slot
+// instead of the rows.
 template <int ROWS, typename Real, typename Acc, typename In>
 __device__ __forceinline__ void row_item(const Real* __restrict__ val, const In* __restrict__ in, In* __restrict__ out,
-                                         long long offset, int col0, int quads, int out_row, int count, int lane) {
+                                         long long offset, int col0, int quads, int stride, int out_row, int count,
+                                         int part, Acc* __restrict__ part_buf, int lane) {
   Acc acc[ROWS][3];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) acc[r][0] = acc[r][1] = acc[r][2] = Acc(0);
-  const size_t width = 4 * static_cast<size_t>(quads);
+  const size_t width = 4 * static_cast<size_t>(stride);
   const Real* rows = val + offset;
   for (int qi = lane; qi < quads; qi += kWave) {
     Acc x[4][3];
@@ -710,21 +716,41 @@ __device__ __forceinline__ void row_item(const Real* __restrict__ val, const In*
   }
   const Acc total = wave_sum32(vals, lane);
   const int k = lane >> 1;
-  if ((lane & 1) == 0 && (k >> 2) < count) out[4 * static_cast<size_t>(out_row) + k] = static_cast<In>(total);
+  if ((lane & 1) == 0) {
+    if (part >= 0)
+      part_buf[static_cast<size_t>(part) * 32 + k] = total;
+    else if ((k >> 2) < count)
+      out[4 * static_cast<size_t>(out_row) + k] = static_cast<In>(total);
+  }
 }
 
-// S product: wavefront w streams the items s_list[woff[w] .. woff[w + 1])
-// whose body is active. Lane l holds the descriptor of the l-th item of a
-// window of 64, so the descriptors of a window cost one load round trip.
+// An S or S^T item list: items, per-worker ranges woff into list, and the
+// workgroups it was balanced over.
+struct RowLists {
+  const RowItem* items;
+  const int* woff;
+  const int* list;
+  int grid;
+};
+
+// S product over the list *mode selects (0 whole items, 1 the split list):
+// wavefront w streams the items list[woff[w] .. woff[w + 1]) whose body is
+// active (workgroups beyond the list's grid return). Lane l holds the
+// descriptor of the l-th item of a window of 64, so the descriptors of a
+// window cost one load round trip.
 template <typename Real, typename Acc, typename In>
-__global__ __launch_bounds__(kBlock) void k_factor_rows(const RowItem* __restrict__ items, const int* __restrict__ woff,
-                                                         const int* __restrict__ list, const Real* __restrict__ val,
+__global__ __launch_bounds__(kBlock) void k_factor_rows(RowLists whole, RowLists split, const Real* __restrict__ val,
                                                          const In* __restrict__ in, In* __restrict__ out,
-                                                         const int* __restrict__ active) {
+                                                         const int* __restrict__ active, const int* __restrict__ mode,
+                                                         Acc* __restrict__ part_buf) {
+  const RowLists L = *mode != 0 ? split : whole;
+  if (static_cast<int>(blockIdx.x) >= L.grid) return;
+  const RowItem* __restrict__ items = L.items;
+  const int* __restrict__ list = L.list;
   const int lane = static_cast<int>(threadIdx.x) % kWave;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * (kBlock / kWave) +
                                                   static_cast<int>(threadIdx.x) / kWave);
-  const int l0 = woff[wave], l1 = woff[wave + 1];
+  const int l0 = L.woff[wave], l1 = L.woff[wave + 1];
   for (int w0 = l0; w0 < l1; w0 += kWave) {
     RowItem mine{};
     bool on = false;
@@ -739,9 +765,28 @@ __global__ __launch_bounds__(kBlock) void k_factor_rows(const RowItem* __restric
       const long long offset = lane_i64(mine.offset, j);
       const int col0 = lane_int(mine.col0, j), quads = lane_int(mine.quads, j);
       const int out_row = lane_int(mine.out_row, j), count = lane_int(mine.count, j);
-      row_item<kRowsPerGroup, Real, Acc>(val, in, out, offset, col0, quads, out_row, count, lane);
+      const int part = lane_int(mine.part, j), stride = lane_int(mine.stride, j);
+      row_item<kRowsPerGroup, Real, Acc>(val, in, out, offset, col0, quads, stride, out_row, count, part, part_buf,
+                                         lane);
     }
   }
+}
+
+// The split S items: one thread per (item, value) adds the chunks' partial
+// sums in column order and stores the value (rows within the item's count).
+template <typename Acc, typename In>
+__global__ __launch_bounds__(kBlock) void k_rows_combine(const RowCombine* __restrict__ items, int count,
+                                                         const Acc* __restrict__ part_buf, In* __restrict__ out,
+                                                         const int* __restrict__ active, const int* __restrict__ mode) {
+  if (*mode == 0) return;
+  const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int w = t >> 5, k = t & 31;
+  if (w >= count) return;
+  const RowCombine it = items[w];
+  if (!active[it.body] || (k >> 2) >= it.count) return;
+  Acc acc = Acc(0);
+  for (int p = it.part0; p < it.part0 + it.parts; ++p) acc += part_buf[static_cast<size_t>(p) * 32 + k];
+  out[4 * static_cast<size_t>(it.out_row) + k] = static_cast<In>(acc);
 }
 
 // The adjoint's safe-clip pass over the S^T output (adjoint.py:184-204,
@@ -810,17 +855,27 @@ __device__ __forceinline__ void cols_store(const Acc (&acc)[4][3], int c, int ge
   }
 }
 
-// S^T over a column-group list (whole groups, or the sliced list); returns
-// at once unless *mode selects this list (`want`). Sliced items with a
+struct ColLists {
+  const ColGroup* items;
+  const int* woff;
+  const int* list;
+  int grid;
+};
+
+// S^T over the column-group list *mode selects (0 whole groups, 1 the sliced
+// list; workgroups beyond the list's grid return). Sliced items with a
 // partial slot store their sums to part_buf[slot][12][64] (coalesced) for the
 // combine kernel.
 template <typename Real, typename Acc, typename In>
 __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
-    const ColGroup* __restrict__ items, const int* __restrict__ woff, const int* __restrict__ list,
-    const ColEntry* __restrict__ entries, const Real* __restrict__ val, const In* __restrict__ y,
-    double* __restrict__ out, const int* __restrict__ gorig, const double* base, const int* __restrict__ active,
-    AdjointEpilogue adj, const int* __restrict__ mode, int want, Acc* __restrict__ part_buf) {
-  if (*mode != want) return;
+    ColLists whole, ColLists sliced, const ColEntry* __restrict__ entries, const Real* __restrict__ val,
+    const In* __restrict__ y, double* __restrict__ out, const int* __restrict__ gorig, const double* base,
+    const int* __restrict__ active, AdjointEpilogue adj, const int* __restrict__ mode, Acc* __restrict__ part_buf) {
+  const ColLists L = *mode != 0 ? sliced : whole;
+  if (static_cast<int>(blockIdx.x) >= L.grid) return;
+  const ColGroup* __restrict__ items = L.items;
+  const int* __restrict__ woff = L.woff;
+  const int* __restrict__ list = L.list;
   // Rows per batch: 16 with FP32 values, 8 with FP64 (the same bytes).
   constexpr int kWaves = kColWaves, kBatch = sizeof(Real) == 4 ? 16 : 8;
   static_assert(kBatch <= kWave, "a batch's entries are held one per lane");
@@ -1182,7 +1237,7 @@ __global__ __launch_bounds__(kBlock) void k_conv_check(DeviceState d, StepScalar
   }
   write_prefix(d.active, d.vcb, d.B, d.pref_fv);
   write_prefix(d.active, d.ecb, d.B, d.pref_fe);
-  write_mode(d.active, d.tcb, d.B, d.t_switch, d.t_mode);
+  write_mode(d.active, d.tcb, d.B, d.t_switch, d.s_switch, d.t_mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -1642,7 +1697,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_check(DeviceState d, StepScalars
     }
   }
   write_prefix(d.aactive, d.vcb, d.B, d.pref_av);
-  write_mode(d.aactive, d.tcb, d.B, d.t_switch, d.t_mode);
+  write_mode(d.aactive, d.tcb, d.B, d.t_switch, d.s_switch, d.t_mode);
 }
 
 // mu = fscale * xr of every vertex, after an adjoint loop (the bodies that
@@ -1957,47 +2012,64 @@ size_t prefix_lds(const DeviceState& d) { return sizeof(int) * static_cast<size_
 // The S product, then S^T from whichever list *mode selects: the whole
 // groups, or the sliced list and its combine (the other launches return at
 // once).
+// S^T from whichever list *mode selects: one launch over both lists' grids
+// (the workgroups of the other list, and those beyond the chosen list's
+// grid, return at once), then the sliced list's combine (which returns at
+// once unless the sliced list ran).
 template <typename Real, typename Acc, typename In>
 void cols_launch(const DeviceState& d, double* out, const double* base, const int* active, AdjointEpilogue adj,
                  const int* mode, hipStream_t stream) {
   const Real* val = static_cast<const Real*>(d.fval);
   In* y4 = static_cast<In*>(d.y4);
   Acc* part_buf = static_cast<Acc*>(d.t_part);
-  if (d.t_nitems)
-    hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_t), dim3(kColWaves * kWave), 0, stream, d.t_items,
-                       d.t_woff, d.t_list, d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base, active,
-                       adj, mode, 0, part_buf);
-  if (d.ts_nitems) {
-    hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(d.g_ts), dim3(kColWaves * kWave), 0, stream, d.ts_items,
-                       d.ts_woff, d.ts_list, d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base,
-                       active, adj, mode, 1, part_buf);
-    if (d.tc_nitems)
-      hipLaunchKernelGGL(k_factor_combine<Acc>, dim3(d.g_tc), dim3(kBlock), 0, stream, d.tc_items, d.tc_nitems,
-                         static_cast<const Acc*>(part_buf), out, d.gorig, base, active, adj, mode);
-  }
+  const ColLists whole{d.t_items, d.t_woff, d.t_list, d.t_nitems ? d.g_t : 0};
+  const ColLists sliced{d.ts_items, d.ts_woff, d.ts_list, d.ts_nitems ? d.g_ts : 0};
+  const int grid = std::max(whole.grid, sliced.grid);
+  if (grid)
+    hipLaunchKernelGGL((k_factor_cols<Real, Acc, In>), dim3(grid), dim3(kColWaves * kWave), 0, stream, whole, sliced,
+                       d.t_entries, val, static_cast<const In*>(y4), out, d.gorig, base, active, adj, mode, part_buf);
+  if (d.tc_nitems)
+    hipLaunchKernelGGL(k_factor_combine<Acc>, dim3(d.g_tc), dim3(kBlock), 0, stream, d.tc_items, d.tc_nitems,
+                       static_cast<const Acc*>(part_buf), out, d.gorig, base, active, adj, mode);
+}
+
+// The S product from whichever list *mode selects, the same way, then the
+// split list's combine.
+template <typename Real, typename Acc, typename In>
+void rows_launch(const DeviceState& d, const int* active, const int* mode, hipStream_t stream) {
+  const Real* val = static_cast<const Real*>(d.fval);
+  In* y4 = static_cast<In*>(d.y4);
+  Acc* part_buf = static_cast<Acc*>(d.s_part);
+  const RowLists whole{d.s_items, d.s_woff, d.s_list, d.s_nitems ? d.g_s : 0};
+  const RowLists split{d.ss_items, d.ss_woff, d.ss_list, d.ss_nitems ? d.g_ss : 0};
+  const int grid = std::max(whole.grid, split.grid);
+  if (grid)
+    hipLaunchKernelGGL((k_factor_rows<Real, Acc, In>), dim3(grid), dim3(kBlock), 0, stream, whole, split, val,
+                       static_cast<const In*>(d.rhs4), y4, active, mode, part_buf);
+  if (d.sc_nitems)
+    hipLaunchKernelGGL((k_rows_combine<Acc, In>), dim3((d.sc_nitems * 32 + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       stream, d.sc_items, d.sc_nitems, static_cast<const Acc*>(part_buf), y4, active, mode);
 }
 
 template <typename Real, typename Acc, typename In>
 void spmv_pair(const DeviceState& d, double* out, const double* base, const int* active, AdjointEpilogue adj,
-               const int* mode, hipStream_t stream) {
-  const Real* val = static_cast<const Real*>(d.fval);
-  In* y4 = static_cast<In*>(d.y4);
-  if (d.s_nitems)
-    hipLaunchKernelGGL((k_factor_rows<Real, Acc, In>), dim3(d.g_s), dim3(kBlock), 0, stream, d.s_items, d.s_woff,
-                       d.s_list, val, static_cast<const In*>(d.rhs4), y4, active);
-  cols_launch<Real, Acc, In>(d, out, base, active, adj, mode, stream);
+               const int* t_mode, const int* s_mode, hipStream_t stream) {
+  rows_launch<Real, Acc, In>(d, active, s_mode, stream);
+  cols_launch<Real, Acc, In>(d, out, base, active, adj, t_mode, stream);
 }
 
-// Dispatch on the stored precision; `all` selects the list chosen for every
-// body at once instead of the active bodies' (the loops' prefix kernels set it).
+// Dispatch on the stored precision; `all` selects the lists chosen for every
+// body at once instead of the active bodies' (the loops' prefix kernels set
+// those).
 void spmv_dispatch(const DeviceState& d, double* out, const double* base, const int* active, hipStream_t stream,
                    AdjointEpilogue adj = AdjointEpilogue{}, bool all = false) {
-  if (!d.t_mode) throw std::logic_error("factor products launched before their S^T list choice was allocated");
-  const int* mode = d.t_mode + (all ? 1 : 0);
+  if (!d.t_mode) throw std::logic_error("factor products launched before their list choices were allocated");
+  const int* t_mode = d.t_mode + (all ? 1 : 0);
+  const int* s_mode = d.t_mode + (all ? 3 : 2);
   if (d.f32)
-    spmv_pair<float, float, float>(d, out, base, active, adj, mode, stream);
+    spmv_pair<float, float, float>(d, out, base, active, adj, t_mode, s_mode, stream);
   else
-    spmv_pair<double, double, double>(d, out, base, active, adj, mode, stream);
+    spmv_pair<double, double, double>(d, out, base, active, adj, t_mode, s_mode, stream);
 }
 
 // Iteration products: x_raw = S^T (S rhs) (+ base, the defect-correction form).
@@ -2036,10 +2108,10 @@ void launch_active_prefix(const DeviceState& d, bool adjoint, hipStream_t stream
   const size_t lds = prefix_lds(d);
   if (adjoint)
     hipLaunchKernelGGL(k_active_prefix, dim3(1), dim3(kBlock), lds, stream, d.aactive, d.vcb, d.pref_av, nullptr,
-                       nullptr, d.B, d.tcb, d.t_switch, d.t_mode);
+                       nullptr, d.B, d.tcb, d.t_switch, d.s_switch, d.t_mode);
   else
     hipLaunchKernelGGL(k_active_prefix, dim3(1), dim3(kBlock), lds, stream, d.active, d.vcb, d.pref_fv, d.ecb,
-                       d.pref_fe, d.B, d.tcb, d.t_switch, d.t_mode);
+                       d.pref_fe, d.B, d.tcb, d.t_switch, d.s_switch, d.t_mode);
 }
 
 void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const double* targets,
@@ -2126,6 +2198,14 @@ void launch_cols(const DeviceState& d, const int* active, const int* mode, hipSt
     cols_launch<float, float, float>(d, d.xr, nullptr, active, AdjointEpilogue{}, mode, stream);
   else
     cols_launch<double, double, double>(d, d.xr, nullptr, active, AdjointEpilogue{}, mode, stream);
+}
+
+void launch_rows(const DeviceState& d, const int* active, const int* mode, hipStream_t stream) {
+  if (!mode) throw std::logic_error("S launched without a list choice");
+  if (d.f32)
+    rows_launch<float, float, float>(d, active, mode, stream);
+  else
+    rows_launch<double, double, double>(d, active, mode, stream);
 }
 
 void launch_apply_inverse(const DeviceState& d, const double* rhs, double* out, hipStream_t stream) {

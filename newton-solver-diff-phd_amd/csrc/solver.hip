@@ -248,15 +248,64 @@ void build_sliced(DeviceState& d, const std::vector<ColGroup>& t_items, int rows
   d.ts_list = dupload(owned, list);
 }
 
-// The S^T shapes tuned on this solver's own factor: every candidate slice
-// length is timed (HIP events, warm launches) over prefixes of the bodies
-// holding 1, 1/2, ..., 1/16 of the column groups, with the whole groups at
-// the same levels; the slice length with the least time summed over the
-// levels is kept, and the sliced list runs below the largest active-group
-// count at which it beat the whole groups. A candidate must win by 5% to
-// replace the default slice length, so timing noise rarely changes the
-// choice (and with it the products' summation order).
-void tune_products(dphd_solver* s, const std::vector<ColGroup>& t_items, int cols_cap, bool defect) {
+// The split S list of `s_items`: items wider than `quads` quads cut into
+// column chunks of at most that many quads (partial sums, combined in column
+// order) into d (items, per-wavefront lists, combine items, partial buffer,
+// grid); device buffers go to `owned`.
+void build_split(DeviceState& d, const std::vector<RowItem>& s_items, int quads, int bodies, int rows_cap,
+                 std::vector<void*>& owned) {
+  std::vector<RowItem> ss_items;
+  std::vector<RowCombine> sc_items;
+  int parts = 0;
+  for (const RowItem& it : s_items) {
+    const int pieces = std::max(1, (it.quads + quads - 1) / quads);
+    if (pieces == 1) {
+      ss_items.push_back(it);
+      continue;
+    }
+    sc_items.push_back(RowCombine{it.out_row, it.count, it.body, parts, pieces, {0, 0, 0}});
+    for (int k = 0; k < pieces; ++k) {
+      const int q0 = static_cast<int>(static_cast<long long>(it.quads) * k / pieces);
+      const int q1 = static_cast<int>(static_cast<long long>(it.quads) * (k + 1) / pieces);
+      RowItem c = it;
+      c.offset = it.offset + 4LL * q0;
+      c.col0 = it.col0 + 4 * q0;
+      c.quads = q1 - q0;
+      c.part = parts + k;
+      ss_items.push_back(c);
+    }
+    parts += pieces;
+  }
+  d.split_quads = quads;
+  d.ss_items = dupload(owned, ss_items);
+  d.ss_nitems = static_cast<int>(ss_items.size());
+  d.sc_items = dupload(owned, sc_items);
+  d.sc_nitems = static_cast<int>(sc_items.size());
+  d.s_part = dalloc<unsigned char>(owned, static_cast<size_t>(std::max(parts, 1)) * 32 *
+                                              (d.f32 ? sizeof(float) : sizeof(double)));
+  d.g_ss = std::min(grid_for((d.ss_nitems + 3) / 4), rows_cap >= 8 ? rows_cap & ~7 : rows_cap);
+  std::vector<double> cost;
+  std::vector<int> body, off, list;
+  for (const RowItem& it : ss_items) {
+    cost.push_back(0.25 + (it.quads + kWave - 1) / kWave);
+    body.push_back(it.body);
+  }
+  balanced_lists(cost, body, bodies, d.g_ss * (kBlock / kWave), kBlock / kWave, off, list);
+  d.ss_woff = dupload(owned, off);
+  d.ss_list = dupload(owned, list);
+}
+
+// The product shapes tuned on this solver's own factor. Activity levels:
+// prefixes of the bodies holding all, 1/2, ..., 1/16 of the column groups
+// (one level for a single body). For each product every candidate cut (S^T:
+// rows per slice; S: quads per column chunk) is timed (HIP events, warm
+// launches) at every level; the candidate with the least time summed over
+// the levels is kept, and its list runs below the largest active-group count
+// at which it beat the uncut list. A candidate must win by 5% to replace
+// the default, so timing noise rarely changes the choice (and with it the
+// products' summation order).
+void tune_products(dphd_solver* s, const std::vector<RowItem>& s_items, const std::vector<ColGroup>& t_items,
+                   int rows_cap, int cols_cap, bool defect) {
   DeviceState& d = s->d;
   const int B = s->B;
   const auto t0 = std::chrono::steady_clock::now();
@@ -284,44 +333,60 @@ void tune_products(dphd_solver* s, const std::vector<ColGroup>& t_items, int col
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
   hipStream_t st = s->stream;
-  auto timed = [&](const int* active, int list) {
-    launch_cols(d, active, modes + list, st);
+  auto timed = [&](auto launch) {
+    launch();
     HIP_CHECK(hipEventRecord(e0, st));
-    for (int r = 0; r < 3; ++r) launch_cols(d, active, modes + list, st);
+    for (int r = 0; r < 3; ++r) launch();
     HIP_CHECK(hipEventRecord(e1, st));
     HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0.0f;
     HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
     return static_cast<double>(ms) / 3.0;
   };
-  std::vector<double> whole(levels);
-  for (int lv = 0; lv < levels; ++lv) whole[lv] = timed(act[lv], 0);
-  const int candidates[] = {kSliceRows, 64, 128, 512};
-  int best = kSliceRows;
-  double best_sum = 0.0;
-  std::vector<double> best_sliced;
-  for (const int rows : candidates) {
-    build_sliced(d, t_items, rows, B, cols_cap, defect, tmp);
-    std::vector<double> sl(levels);
-    double sum = 0.0;
-    for (int lv = 0; lv < levels; ++lv) sum += (sl[lv] = timed(act[lv], 1));
-    if (rows == kSliceRows || sum < 0.95 * best_sum) {
-      best = rows;
-      best_sum = sum;
-      best_sliced = sl;
+  // One product: the uncut list against each candidate cut; returns the
+  // chosen cut and its switch point, leaving the lists built for it.
+  auto tune = [&](const int* candidates, int count, auto build, auto launch) {
+    std::vector<double> whole(levels);
+    for (int lv = 0; lv < levels; ++lv) whole[lv] = timed([&] { launch(act[lv], modes); });
+    int best = candidates[0];
+    double best_sum = 0.0;
+    std::vector<double> best_cut;
+    for (int k = 0; k < count; ++k) {
+      build(candidates[k], tmp);
+      std::vector<double> cut(levels);
+      double sum = 0.0;
+      for (int lv = 0; lv < levels; ++lv) sum += (cut[lv] = timed([&] { launch(act[lv], modes + 1); }));
+      if (k == 0 || sum < 0.95 * best_sum) {
+        best = candidates[k];
+        best_sum = sum;
+        best_cut = cut;
+      }
     }
-  }
-  int threshold = 0;
-  for (int lv = 0; lv < levels; ++lv)
-    if (best_sliced[lv] < whole[lv]) threshold = std::max(threshold, lv == 0 ? INT_MAX : act_groups[lv] + 1);
+    int threshold = 0;
+    for (int lv = 0; lv < levels; ++lv)
+      if (best_cut[lv] < whole[lv]) threshold = std::max(threshold, lv == 0 ? INT_MAX : act_groups[lv] + 1);
+    return std::make_pair(best, threshold);
+  };
+  const int slices[] = {kSliceRows, 64, 128, 512};
+  const auto t_choice = tune(
+      slices, 4, [&](int rows, std::vector<void*>& o) { build_sliced(d, t_items, rows, B, cols_cap, defect, o); },
+      [&](const int* active, const int* mode) { launch_cols(d, active, mode, st); });
+  const int chunks[] = {kSplitQuads, 128, 512};
+  const auto s_choice = tune(
+      chunks, 3, [&](int quads, std::vector<void*>& o) { build_split(d, s_items, quads, B, rows_cap, o); },
+      [&](const int* active, const int* mode) { launch_rows(d, active, mode, st); });
   HIP_CHECK(hipEventDestroy(e0));
   HIP_CHECK(hipEventDestroy(e1));
   HIP_CHECK(hipStreamSynchronize(st));
   for (void* p : tmp) (void)hipFree(p);
-  build_sliced(d, t_items, best, B, cols_cap, defect, s->owned);
-  d.t_switch = threshold;
-  s->info.slice_rows = best;
-  s->info.slice_switch = threshold;
+  build_sliced(d, t_items, t_choice.first, B, cols_cap, defect, s->owned);
+  build_split(d, s_items, s_choice.first, B, rows_cap, s->owned);
+  d.t_switch = t_choice.second;
+  d.s_switch = s_choice.second;
+  s->info.slice_rows = t_choice.first;
+  s->info.slice_switch = t_choice.second;
+  s->info.split_quads = s_choice.first;
+  s->info.split_switch = s_choice.second;
   s->info.tune_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -548,6 +613,8 @@ void build_solver(dphd_solver* s, double dt) {
         it.out_row = c0 + fb.row0 + r;
         it.count = std::min(kRowsPerGroup, fb.rows - r);
         it.body = b;
+        it.part = -1;
+        it.stride = fb.quads;
         s_items.push_back(it);
       }
     }
@@ -746,7 +813,7 @@ void build_solver(dphd_solver* s, double dt) {
   d.t_items = dupload(o, t_items);
   d.t_nitems = static_cast<int>(t_items.size());
   d.t_entries = dupload(o, t_entries);
-  int cols_capacity = 0;
+  int cols_capacity = 0, rows_capacity = 0;
   {
     // Grids of the products: no more workgroups than the device holds at
     // once, so that every wavefront's list runs concurrently.
@@ -757,13 +824,19 @@ void build_solver(dphd_solver* s, double dt) {
     // The sliced list while the active bodies' groups cannot fill the device
     // (replaced by the tuned shapes unless slice_rows is fixed).
     cols_capacity = cols_cap;
+    rows_capacity = rows_cap;
     d.t_switch = cols_cap;
+    d.s_switch = cols_cap;
     build_sliced(d, t_items, s->cfg.slice_rows > 0 ? s->cfg.slice_rows : kSliceRows, B, cols_cap, defect, o);
+    build_split(d, s_items, kSplitQuads, B, rows_cap, o);
   }
-  // The list choice every S^T launch reads: allocated here, before any
+  // The list choices every product launch reads: allocated here, before any
   // product runs (the frictional boundary's setup applies the inverse);
-  // the tuning at the end of the build rewrites the all-bodies entry.
-  d.t_mode = dupload(o, std::vector<int>{0, static_cast<int>(t_items.size()) < d.t_switch ? 1 : 0});
+  // the tuning at the end of the build rewrites the all-bodies entries.
+  {
+    const int all_cut = static_cast<int>(t_items.size()) < cols_capacity ? 1 : 0;
+    d.t_mode = dupload(o, std::vector<int>{0, all_cut, 0, all_cut});
+  }
   {
     // Cost of an item in memory round trips of its wavefront: an S item
     // streams its rows in passes of 64 quads, a wavefront of an S^T group
@@ -898,11 +971,15 @@ void build_solver(dphd_solver* s, double dt) {
   s->info.factor_slots = slots;
   s->info.slice_rows = d.slice_rows;
   s->info.slice_switch = d.t_switch;
+  s->info.split_quads = d.split_quads;
+  s->info.split_switch = d.s_switch;
   s->info.tune_seconds = 0.0;
   if (s->cfg.slice_rows == 0 && d.t_nitems > 0) {
-    tune_products(s, t_items, cols_capacity, defect);
-    const int all_sliced = static_cast<int>(t_items.size()) < d.t_switch ? 1 : 0;
-    HIP_CHECK(hipMemcpy(d.t_mode + 1, &all_sliced, sizeof(int), hipMemcpyHostToDevice));
+    tune_products(s, s_items, t_items, rows_capacity, cols_capacity, defect);
+    const int groups = static_cast<int>(t_items.size());
+    const int all_cut[2] = {groups < d.t_switch ? 1 : 0, groups < d.s_switch ? 1 : 0};
+    HIP_CHECK(hipMemcpy(d.t_mode + 1, &all_cut[0], sizeof(int), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(d.t_mode + 3, &all_cut[1], sizeof(int), hipMemcpyHostToDevice));
   }
   s->info.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::fill(std::begin(s->loop_stats), std::end(s->loop_stats), 0);

@@ -53,6 +53,8 @@ class Info(ctypes.Structure):
         ("slice_rows", ctypes.c_int),
         ("slice_switch", ctypes.c_int),
         ("tune_seconds", ctypes.c_double),
+        ("split_quads", ctypes.c_int),
+        ("split_switch", ctypes.c_int),
     ]
 
 
@@ -108,7 +110,7 @@ SIGNATURES = {
 _lib = None
 
 
-ABI_VERSION = 5  # DPHD_ABI_VERSION of include/diffphd.h
+ABI_VERSION = 6  # DPHD_ABI_VERSION of include/diffphd.h
 
 
 def lib():

@@ -192,7 +192,7 @@ def test_library_exports_every_declared_entry_point():
     lib = _lib.lib()
     for name in sorted(declared):
         assert hasattr(lib, name), name
-    assert lib.dphd_version() == 5
+    assert lib.dphd_version() == 6
     assert set(_lib.SIGNATURES) == declared
 
 
