@@ -8,8 +8,8 @@ m = 5, per-body relative tolerance 1e-6; loss = sum over bodies of the
 squared distance of the top-face vertices to targets drawn N(rest, 1 mm).
 One "step" is one forward PD step of every body plus its adjoint step.
 
-Timed: K forward steps (autograd-recorded) followed by the backward sweep
-through them, all bodies in one batched solver (`value`, steps/s), and the
+Timed: K forward steps (autograd-recorded; K = 40 by default, the
+configuration's rollout) followed by the backward sweep through them, all bodies in one batched solver (`value`, steps/s), and the
 same bodies as 256 independent single-body solvers run one after another
 (the plain per-body GPU path). Roofline: the factor products S and S^T
 of all bodies (A^{-1} = S^T S, the dominant kernels), timed live with HIP
@@ -608,8 +608,8 @@ def workload_cpu_baseline(name, sc, args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10, help="timed fwd+adj steps (the full config uses 40)")
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40, help="timed fwd+adj steps (the config's rollout: 40)")
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--bodies", type=int, default=256)
     ap.add_argument("--kmin", type=int, default=4)
     ap.add_argument("--kmax", type=int, default=16)
