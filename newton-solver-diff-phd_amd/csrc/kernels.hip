@@ -739,10 +739,9 @@ struct RowLists {
 // descriptor of the l-th item of a window of 64, so the descriptors of a
 // window cost one load round trip.
 template <typename Real, typename Acc, typename In>
-__global__ __launch_bounds__(kBlock) void k_factor_rows(RowLists whole, RowLists split, const Real* __restrict__ val,
-                                                         const In* __restrict__ in, In* __restrict__ out,
-                                                         const int* __restrict__ active, const int* __restrict__ mode,
-                                                         Acc* __restrict__ part_buf) {
+__global__ __launch_bounds__(kBlock) void k_factor_rows(
+    RowLists whole, RowLists split, const Real* __restrict__ val, const In* __restrict__ in, In* __restrict__ out,
+    const int* __restrict__ active, const int* __restrict__ mode, Acc* __restrict__ part_buf) {
   const RowLists L = *mode != 0 ? split : whole;
   if (static_cast<int>(blockIdx.x) >= L.grid) return;
   const RowItem* __restrict__ items = L.items;
