@@ -99,6 +99,10 @@ struct FrictionDev {
   double *gap = nullptr, *omega = nullptr, *diag = nullptr, *resid = nullptr, *rhs = nullptr;
   double *force = nullptr, *response = nullptr;
   int *order = nullptr, *slot = nullptr, *nactive = nullptr, *perm = nullptr;
+  // The previous iteration's coupled candidates (ascending), kept by the
+  // Newton kernel before it selects this iteration's: in the FP32-factor mode
+  // only these and the current ones carry a nonzero f - f_prev.
+  int *prev_order = nullptr, *nprev = nullptr;
   double *matrix = nullptr, *compact = nullptr, *work = nullptr;
   int* candidate = nullptr;
   // mesh boundary: faces = left then right; vertices below `split` touch the right faces

@@ -914,6 +914,12 @@ __global__ __launch_bounds__(256) void k_fc_newton(FrictionDev fc, const double*
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ int s_wave[4];
   __shared__ int s_base;
+  // The previous iteration's coupled set, for the FP32-factor mode's solution.
+  if (fc.defect) {
+    const int prev = *fc.nactive;
+    for (int k = tid; k < prev; k += 256) fc.prev_order[k] = fc.order[k];
+    if (tid == 0) *fc.nprev = prev;
+  }
   // contact_active(_scan): coupled candidates (omega != 0) in candidate order.
   if (tid == 0) s_base = 0;
   __syncthreads();
@@ -1019,7 +1025,14 @@ __global__ __launch_bounds__(256) void k_fc_free(FrictionDev fc, const double* s
   if (c >= fc.K) return;
   V3 v = ld3(solved, fc.nodes[c]);
   if (fc.defect) {
-    for (int j = 0; j < fc.K; ++j) v = v - fc.blockT[static_cast<size_t>(j) * fc.K + c] * ld3(fc.fprev, j);
+    // f_prev is zero outside the previous iteration's coupled set (still in
+    // `order`: the Newton kernel selects this iteration's after this one), so
+    // the sum over that set, ascending, is the sum over every candidate.
+    const int prev = *fc.nactive;
+    for (int k = 0; k < prev; ++k) {
+      const int j = fc.order[k];
+      v = v - fc.blockT[static_cast<size_t>(j) * fc.K + c] * ld3(fc.fprev, j);
+    }
   }
   st3(fc.freec, c, v);
 }
@@ -1034,7 +1047,16 @@ __global__ __launch_bounds__(256) void k_fc_solution(FrictionDev fc, double* xr,
   if (i >= fc.n) return;
   V3 v = ld3(xr, i);
   if (fc.defect) {
-    for (int j = 0; j < fc.K; ++j) {
+    // f - f_prev can be nonzero only for candidates coupled now or in the
+    // previous iteration: the two ascending sets merged, so the candidates
+    // are visited in the order of a loop over all of them.
+    const int na = *fc.nactive, np = *fc.nprev;
+    int a = 0, b = 0;
+    while (a < na || b < np) {
+      const int ja = a < na ? fc.order[a] : INT_MAX, jb = b < np ? fc.prev_order[b] : INT_MAX;
+      const int j = min(ja, jb);
+      a += ja == j;
+      b += jb == j;
       const V3 f = fc.slot[j] >= 0 ? ld3(fc.force, j) : V3{0, 0, 0};
       const V3 delta = f - ld3(fc.fprev, j);
       if (delta.x != 0.0 || delta.y != 0.0 || delta.z != 0.0)

@@ -70,6 +70,8 @@ struct RowCombine {
 
 // Quads per column chunk of the split S list; the default of the build.
 constexpr int kSplitQuads = 256;
+// Workgroups of the two combine kernels (each strides over its items).
+constexpr int kCombineGrid = 256;
 
 // One row of one block as an S^T column group sees it: the row's values
 // start at `rowoff` with `quads` quads from column col0, and its S-product

@@ -771,21 +771,24 @@ __global__ __launch_bounds__(kBlock) void k_factor_rows(
   }
 }
 
-// The split S items: one thread per (item, value) adds the chunks' partial
-// sums in column order and stores the value (rows within the item's count).
+// The split S items: a thread per (item, value), striding over the grid,
+// adds the chunks' partial sums in column order and stores the value (rows
+// within the item's count). The grid is small, so that while the whole list
+// runs this launch costs little more than its dispatch.
 template <typename Acc, typename In>
 __global__ __launch_bounds__(kBlock) void k_rows_combine(const RowCombine* __restrict__ items, int count,
                                                          const Acc* __restrict__ part_buf, In* __restrict__ out,
                                                          const int* __restrict__ active, const int* __restrict__ mode) {
   if (*mode == 0) return;
-  const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  const int w = t >> 5, k = t & 31;
-  if (w >= count) return;
-  const RowCombine it = items[w];
-  if (!active[it.body] || (k >> 2) >= it.count) return;
-  Acc acc = Acc(0);
-  for (int p = it.part0; p < it.part0 + it.parts; ++p) acc += part_buf[static_cast<size_t>(p) * 32 + k];
-  out[4 * static_cast<size_t>(it.out_row) + k] = static_cast<In>(acc);
+  const int stride = static_cast<int>(gridDim.x * blockDim.x);
+  for (int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); t < 32 * count; t += stride) {
+    const int w = t >> 5, k = t & 31;
+    const RowCombine it = items[w];
+    if (!active[it.body] || (k >> 2) >= it.count) continue;
+    Acc acc = Acc(0);
+    for (int p = it.part0; p < it.part0 + it.parts; ++p) acc += part_buf[static_cast<size_t>(p) * 32 + k];
+    out[4 * static_cast<size_t>(it.out_row) + k] = static_cast<In>(acc);
+  }
 }
 
 // The adjoint's safe-clip pass over the S^T output (adjoint.py:184-204,
@@ -973,8 +976,9 @@ __global__ __launch_bounds__(kColWaves * kWave) void k_factor_cols(
   }
 }
 
-// The groups cut into slices: one wavefront per group adds its slices'
-// partial sums in row order, then stores its columns as k_factor_cols does.
+// The groups cut into slices: a wavefront per group, striding over the
+// grid's wavefronts, adds its slices' partial sums in row order, then stores
+// its columns as k_factor_cols does. A small grid, as for k_rows_combine.
 template <typename Acc>
 __global__ __launch_bounds__(kBlock) void k_factor_combine(const ColCombine* __restrict__ items, int count,
                                                            const Acc* __restrict__ part_buf, double* __restrict__ out,
@@ -983,20 +987,22 @@ __global__ __launch_bounds__(kBlock) void k_factor_combine(const ColCombine* __r
                                                            const int* __restrict__ mode) {
   if (*mode != 1) return;
   const int lane = static_cast<int>(threadIdx.x) % kWave;
-  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * (kBlock / kWave) +
-                                               static_cast<int>(threadIdx.x) / kWave);
-  if (w >= count) return;
-  const ColCombine it = items[w];
-  if (!active[it.body]) return;
-  Acc acc[4][3];
+  const int waves = static_cast<int>(gridDim.x) * (kBlock / kWave);
+  for (int w = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * (kBlock / kWave) +
+                                              static_cast<int>(threadIdx.x) / kWave);
+       w < count; w += waves) {
+    const ColCombine it = items[w];
+    if (!active[it.body]) continue;
+    Acc acc[4][3];
 #pragma unroll
-  for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] = Acc(0);
-  for (int p = it.part0; p < it.part0 + it.parts; ++p) {
-    const Acc* src = part_buf + static_cast<size_t>(p) * 12 * kWave + lane;
+    for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] = Acc(0);
+    for (int p = it.part0; p < it.part0 + it.parts; ++p) {
+      const Acc* src = part_buf + static_cast<size_t>(p) * 12 * kWave + lane;
 #pragma unroll
-    for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] += src[t * kWave];
+      for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] += src[t * kWave];
+    }
+    cols_store<Acc>(acc, it.c0 + 4 * lane, it.end, gorig, base, out, adj, it.gid, lane);
   }
-  cols_store<Acc>(acc, it.c0 + 4 * lane, it.end, gorig, base, out, adj, it.gid, lane);
 }
 
 // r = rhs - A x with the FP64 constant matrix (FP32-factor mode), one thread
@@ -2046,8 +2052,9 @@ void rows_launch(const DeviceState& d, const int* active, const int* mode, hipSt
     hipLaunchKernelGGL((k_factor_rows<Real, Acc, In>), dim3(grid), dim3(kBlock), 0, stream, whole, split, val,
                        static_cast<const In*>(d.rhs4), y4, active, mode, part_buf);
   if (d.sc_nitems)
-    hipLaunchKernelGGL((k_rows_combine<Acc, In>), dim3((d.sc_nitems * 32 + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                       stream, d.sc_items, d.sc_nitems, static_cast<const Acc*>(part_buf), y4, active, mode);
+    hipLaunchKernelGGL((k_rows_combine<Acc, In>), dim3(std::min(kCombineGrid, (d.sc_nitems * 32 + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, stream, d.sc_items, d.sc_nitems, static_cast<const Acc*>(part_buf), y4, active,
+                       mode);
 }
 
 template <typename Real, typename Acc, typename In>

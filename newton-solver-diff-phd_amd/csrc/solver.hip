@@ -235,7 +235,7 @@ void build_sliced(DeviceState& d, const std::vector<ColGroup>& t_items, int rows
   d.t_part = dalloc<unsigned char>(owned, static_cast<size_t>(std::max(parts, 1)) * 12 * kWave *
                                               (defect ? sizeof(float) : sizeof(double)));
   d.g_ts = std::min(grid_for(d.ts_nitems), cols_cap >= 8 ? cols_cap & ~7 : cols_cap);
-  d.g_tc = std::max(1, (d.tc_nitems + kBlock / kWave - 1) / (kBlock / kWave));
+  d.g_tc = std::max(1, std::min(kCombineGrid, (d.tc_nitems + kBlock / kWave - 1) / (kBlock / kWave)));
   std::vector<double> cost;
   std::vector<int> body, off, list;
   for (const ColGroup& g : ts_items) {
@@ -1093,6 +1093,8 @@ void build_friction(dphd_solver* s) {
   fc.order = dzeros<int>(o, k1);
   fc.slot = dzeros<int>(o, k1);
   fc.nactive = dzeros<int>(o, 1);
+  fc.prev_order = dzeros<int>(o, k1);
+  fc.nprev = dzeros<int>(o, 1);
   fc.perm = dzeros<int>(o, k1);
   fc.compact = dzeros<double>(o, k1);
   fc.work = dzeros<double>(o, k1);
