@@ -40,10 +40,16 @@ constexpr int kRowsPerGroup = 8;
 constexpr int kColsPerGroup = 256;
 constexpr int kColWaves = 4;  // wavefronts of an S^T workgroup
 
+// Blocks of at most kPackedQuads quads per row are packed: their items take
+// kPackedPasses x (64 >> s) rows, a row over 2^s lanes (2^s >= quads).
+constexpr int kPackedQuads = 16;
+constexpr int kPackedPasses = 4;
+
 // S product work item (one wavefront): rows out_row .. out_row + count - 1
-// (count <= kRowsPerGroup) of one block, whose values start at `offset`
-// with 4*quads values per row from column col0 (rows 4*stride values
-// apart). Self-contained, so a wavefront needs one load per item.
+// (count <= kRowsPerGroup, or the packed count) of one block, whose values
+// start at `offset` with 4*quads values per row from column col0 (rows
+// 4*stride values apart). Self-contained, so a wavefront needs one load per
+// item.
 struct RowItem {
   long long offset;
   int col0;
@@ -53,7 +59,7 @@ struct RowItem {
   int body;
   int part;    // -1: the item stores its rows; else its partial-sum slot (a chunk of a split item)
   int stride;  // quads per stored row of the item's block (= quads unless a chunk of a split item)
-  int pad;
+  int packed;  // 0: rows of kRowsPerGroup over all 64 lanes; s + 1: a packed narrow item, 2^s lanes per row
 };
 
 // Combine item of an S item split into column chunks: rows out_row ..
@@ -132,6 +138,13 @@ struct DeviceState {
   float* poses = nullptr;      // Tet4 [E*9]
   double* hexg = nullptr;      // Hex8 [B*192]
   uint8_t* pinmask = nullptr;  // [E] bit k: corner k is a Dirichlet vertex
+  // Local-step staging: element chunk c's distinct vertices are
+  // lv_list[lv_off[c] .. lv_off[c + 1]) (ascending); corner k of element e is
+  // entry lidx[e*K + k] of its chunk's list; lv_max is the longest list.
+  int* lv_off = nullptr;       // [nec + 1]
+  int* lv_list = nullptr;
+  uint16_t* lidx = nullptr;    // [E*K]
+  int lv_max = 0;
   int* ebody = nullptr;        // [E]
   int* vbody = nullptr;        // [N]
   uint8_t* free = nullptr;     // [N]

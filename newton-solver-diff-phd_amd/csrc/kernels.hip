@@ -456,19 +456,40 @@ __global__ __launch_bounds__(kBlock) void k_any_active(const int* active, const 
 // ---------------------------------------------------------------------------
 // Local step.
 
-__device__ __forceinline__ void local_tet4_chunk(const DeviceState& d, int ci, int b, const double* targets) {
+// Copies the positions of element chunk ci's distinct vertices into the
+// stage xs (3 doubles each, in the chunk's list order); every thread of the
+// workgroup calls it. The first barrier keeps the previous chunk's readers
+// ahead of the overwrite. Consecutive threads load consecutive components, so
+// runs of consecutive vertex indices are read coalesced.
+__device__ __forceinline__ void stage_vertices(const DeviceState& d, int ci, double* xs) {
+  const int v0 = d.lv_off[ci], nv = d.lv_off[ci + 1] - v0;
+  __syncthreads();
+  for (int t = threadIdx.x; t < 3 * nv; t += blockDim.x) {
+    const int v = t / 3;
+    xs[t] = d.x[3 * d.lv_list[v0 + v] + (t - 3 * v)];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ V3 staged(const double* xs, unsigned j) { return V3{xs[3 * j], xs[3 * j + 1], xs[3 * j + 2]}; }
+
+__device__ __forceinline__ void local_tet4_chunk(const DeviceState& d, int ci, int b, const double* targets,
+                                                 const double* xs) {
   const Chunk c = d.echunks[ci];
   const int e = c.begin + threadIdx.x;
   if (e >= c.end) return;
   V3 g[4];
   tet_grads(d.poses, e, g);
-  const int* nd = d.idx + 4 * e;
+  // The corners' positions in the stage: four 16-bit entries in one load.
+  const uint2 li = reinterpret_cast<const uint2*>(d.lidx)[e];
+  const unsigned l4[4] = {li.x & 0xffffu, li.x >> 16, li.y & 0xffffu, li.y >> 16};
   M3 f = zero3();
 #pragma unroll
-  for (int k = 0; k < 4; ++k) add_outer(f, ld3(d.x, nd[k]), g[k]);
+  for (int k = 0; k < 4; ++k) add_outer(f, staged(xs, l4[k]), g[k]);
   M3 fd = zero3();
   const unsigned mask = d.pinmask[e];
   if (mask) {
+    const int* nd = d.idx + 4 * e;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
       if (mask & (1u << k)) add_outer(fd, ld3(d.xd, nd[k]), g[k]);
@@ -490,7 +511,8 @@ __device__ __forceinline__ void local_tet4_chunk(const DeviceState& d, int ci, i
 }
 
 // 32 cells per chunk, 8 lanes per cell (one per Gauss sample).
-__device__ __forceinline__ void local_hex8_chunk(const DeviceState& d, int ci, int b, const double* targets) {
+__device__ __forceinline__ void local_hex8_chunk(const DeviceState& d, int ci, int b, const double* targets,
+                                                 const double* xs) {
   const Chunk c = d.echunks[ci];
   const int s = threadIdx.x & 7;
   const int e = c.begin + (threadIdx.x >> 3);
@@ -499,12 +521,16 @@ __device__ __forceinline__ void local_hex8_chunk(const DeviceState& d, int ci, i
 #pragma unroll
   for (int j = 0; j < 8; ++j) out[j][0] = out[j][1] = out[j][2] = 0.0;
   if (valid) {
-    const int* nd = d.idx + 8 * e;
+    // The eight corners' positions in the stage (read alike by the cell's 8 lanes).
+    const uint4 li = reinterpret_cast<const uint4*>(d.lidx)[e];
+    const unsigned l8[8] = {li.x & 0xffffu, li.x >> 16, li.y & 0xffffu, li.y >> 16,
+                            li.z & 0xffffu, li.z >> 16, li.w & 0xffffu, li.w >> 16};
     M3 f = zero3();
 #pragma unroll
-    for (int j = 0; j < 8; ++j) add_outer(f, ld3(d.x, nd[j]), hex_grad(d, b, s, j));
+    for (int j = 0; j < 8; ++j) add_outer(f, staged(xs, l8[j]), hex_grad(d, b, s, j));
     M3 fd = zero3();
     const unsigned mask = d.pinmask[e];
+    const int* nd = d.idx + 8 * e;
     if (mask) {
 #pragma unroll
       for (int j = 0; j < 8; ++j)
@@ -534,12 +560,24 @@ __device__ __forceinline__ void local_hex8_chunk(const DeviceState& d, int ci, i
   if (valid) st3(d.contrib, 8 * e + s, sum);
 }
 
+// Dynamic LDS: the work-list prefix (B + 1 ints), then the vertex stage at
+// the next 8-byte boundary.
+__host__ __device__ __forceinline__ size_t stage_offset(int B) {
+  return (sizeof(int) * static_cast<size_t>(B + 1) + 7) / 8 * 8;
+}
+
+// One element chunk per workgroup at a time: its distinct vertices are loaded
+// once into LDS, then every element (Tet4) or Gauss sample (Hex8) reads its
+// corners there.
 __global__ __launch_bounds__(kBlock) void k_local(DeviceState d, const double* targets) {
+  extern __shared__ int lds_base[];
+  double* xs = reinterpret_cast<double*>(reinterpret_cast<char*>(lds_base) + stage_offset(d.B));
   for_active(d.pref_fe, d.ecb, d.B, [&](int ci, int b) {
+    stage_vertices(d, ci, xs);
     if (d.Q == 1)
-      local_tet4_chunk(d, ci, b, targets);
+      local_tet4_chunk(d, ci, b, targets, xs);
     else
-      local_hex8_chunk(d, ci, b, targets);
+      local_hex8_chunk(d, ci, b, targets, xs);
   });
 }
 
@@ -724,6 +762,72 @@ __device__ __forceinline__ void row_item(const Real* __restrict__ val, const In*
   }
 }
 
+// Sum over aligned groups of 2^s lanes (s <= 4), in every lane of the
+// group: the 16- and 8-lane mirrors (l ^ 15, l ^ 7), then quad_perm xor 2
+// and xor 1, the steps of wave_sum32 below its row swaps. A row whose quads
+// sit one per lane from its group's first lane is thereby summed exactly as
+// wave_sum32 sums a row with one quad per lane (the other lanes add zeros),
+// so packing leaves every S-product value bitwise unchanged.
+template <typename T>
+__device__ __forceinline__ T segment_sum(T v, int s) {
+  if (s > 3) v += dpp_move<kDppRowMirror>(v);
+  if (s > 2) v += dpp_move<kDppHalfMirror>(v);
+  if (s > 1) v += dpp_move<kDppXor2>(v);
+  if (s > 0) v += dpp_move<kDppXor1>(v);
+  return v;
+}
+
+__device__ __forceinline__ void store_sums(float* out, size_t row, const float (&t)[3]) {
+  *reinterpret_cast<f32x4*>(out + 4 * row) = f32x4{t[0], t[1], t[2], 0.0f};
+}
+__device__ __forceinline__ void store_sums(double* out, size_t row, const double (&t)[3]) {
+  f64x2* p = reinterpret_cast<f64x2*>(out + 4 * row);
+  p[0] = f64x2{t[0], t[1]};
+  p[1] = f64x2{t[2], 0.0};
+}
+
+// One S item of a narrow block (at most kPackedQuads quads, padded to 2^s
+// lanes per row) on one wavefront: lane l takes quad l & (2^s - 1) of row
+// l >> s in each of kPackedPasses passes (64 >> s rows a pass), so a lane's
+// four input columns are loaded once for the item and every pass's value
+// load is in flight together; a row's quads are then summed over its 2^s
+// lanes and the row's first lane stores it.
+template <typename Real, typename Acc, typename In>
+__device__ __forceinline__ void row_item_packed(const Real* __restrict__ val, const In* __restrict__ in,
+                                                In* __restrict__ out, long long offset, int col0, int quads,
+                                                int stride, int out_row, int count, int s, int lane) {
+  const int q = lane & ((1 << s) - 1), rl = lane >> s, rp = kWave >> s;
+  const int qc = min(q, quads - 1);
+  Acc x[4][3];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) load_vec4(in, static_cast<size_t>(col0 + 4 * qc + u), x[u]);
+  const Real* rows = val + offset + 4 * qc;
+  const size_t width = 4 * static_cast<size_t>(stride);
+  Real v[kPackedPasses][4];
+#pragma unroll
+  for (int p = 0; p < kPackedPasses; ++p) load_quad<false>(rows + width * min(p * rp + rl, count - 1), v[p]);
+  Acc acc[kPackedPasses][3];
+#pragma unroll
+  for (int p = 0; p < kPackedPasses; ++p) {
+    const bool use = q < quads && p * rp + rl < count;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      Acc t = Acc(0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t += static_cast<Acc>(v[p][u]) * x[u][c];
+      acc[p][c] = segment_sum(use ? t : Acc(0), s);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < kPackedPasses; ++p) {
+    const int r = p * rp + rl;
+    if (q == 0 && r < count) {
+      In t[3] = {static_cast<In>(acc[p][0]), static_cast<In>(acc[p][1]), static_cast<In>(acc[p][2])};
+      store_sums(out, static_cast<size_t>(out_row + r), t);
+    }
+  }
+}
+
 // An S or S^T item list: items, per-worker ranges woff into list, and the
 // workgroups it was balanced over.
 struct RowLists {
@@ -765,8 +869,12 @@ __global__ __launch_bounds__(kBlock) void k_factor_rows(
       const int col0 = lane_int(mine.col0, j), quads = lane_int(mine.quads, j);
       const int out_row = lane_int(mine.out_row, j), count = lane_int(mine.count, j);
       const int part = lane_int(mine.part, j), stride = lane_int(mine.stride, j);
-      row_item<kRowsPerGroup, Real, Acc>(val, in, out, offset, col0, quads, stride, out_row, count, part, part_buf,
-                                         lane);
+      const int packed = lane_int(mine.packed, j);
+      if (packed > 0)
+        row_item_packed<Real, Acc>(val, in, out, offset, col0, quads, stride, out_row, count, packed - 1, lane);
+      else
+        row_item<kRowsPerGroup, Real, Acc>(val, in, out, offset, col0, quads, stride, out_row, count, part, part_buf,
+                                           lane);
     }
   }
 }
@@ -2123,7 +2231,8 @@ void launch_active_prefix(const DeviceState& d, bool adjoint, hipStream_t stream
 void launch_forward_iteration(const DeviceState& d, const StepScalars& s, const double* targets,
                               const FrictionDev* fc, hipStream_t stream) {
   const size_t lds = prefix_lds(d);
-  hipLaunchKernelGGL(k_local, dim3(d.g_elem), dim3(kBlock), lds, stream, d, targets);
+  hipLaunchKernelGGL(k_local, dim3(d.g_elem), dim3(kBlock), stage_offset(d.B) + 24 * static_cast<size_t>(d.lv_max),
+                     stream, d, targets);
   hipLaunchKernelGGL(k_gather, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   spmv_iteration(d, d.defect ? d.x : nullptr, d.active, stream);
   if (fc) launch_contact_iteration(*fc, d.x, d.xr, d.q0, d.active, stream);

@@ -191,11 +191,14 @@ void lpt(const std::vector<int>& items, const std::vector<double>& cost, const s
 }
 }  // namespace
 
-void balanced_lists(const std::vector<double>& cost, const std::vector<int>& body, int bodies, int workers,
-                    int per_block, std::vector<int>& off, std::vector<int>& list) {
+namespace {
+// XCD mode: the bodies onto the 8 XCDs by LPT (workgroup g runs on XCD
+// g % 8, so a body's items share one L2), then each XCD's items onto its
+// workers by LPT; with the bodies more than 5% from even over the XCDs,
+// every item onto any worker.
+void xcd_owners(const std::vector<double>& cost, const std::vector<int>& body, int bodies, int workers,
+                int per_block, std::vector<int>& owner) {
   const int n = static_cast<int>(cost.size());
-  if (static_cast<int>(body.size()) != n || workers < 1 || per_block < 1 || workers % per_block != 0)
-    throw std::invalid_argument("balanced_lists: inconsistent sizes");
   const int blocks = workers / per_block;
   int xcds = blocks % 8 == 0 ? 8 : 1;
   // Bodies onto XCDs; when that leaves the XCDs more than 5% apart (few or
@@ -203,7 +206,6 @@ void balanced_lists(const std::vector<double>& cost, const std::vector<int>& bod
   std::vector<double> bcost(bodies, 0.0);
   double total = 0.0;
   for (int i = 0; i < n; ++i) {
-    if (body[i] < 0 || body[i] >= bodies) throw std::invalid_argument("balanced_lists: body out of range");
     bcost[body[i]] += cost[i];
     total += cost[i];
   }
@@ -221,8 +223,54 @@ void balanced_lists(const std::vector<double>& cost, const std::vector<int>& bod
   std::vector<std::vector<int>> xitems(xcds), xworkers(xcds);
   for (int i = 0; i < n; ++i) xitems[bxcd[body[i]]].push_back(i);
   for (int w = 0; w < workers; ++w) xworkers[(w / per_block) % xcds].push_back(w);
-  std::vector<int> owner(n, 0);
   for (int x = 0; x < xcds; ++x) lpt(xitems[x], cost, xworkers[x], owner);
+}
+
+// Spread mode: each body's items, costliest first, each to a distinct worker
+// (rounds of `workers` when a body has more items): the least loaded workers
+// so far, ties in an order that cycles over the workgroups (and with them
+// the XCDs) before it reuses one. Any subset of the bodies then keeps every
+// XCD busy, at the price of a body's inputs being cached in every L2.
+void spread_owners(const std::vector<double>& cost, const std::vector<int>& body, int bodies, int workers,
+                   int per_block, std::vector<int>& owner) {
+  const int blocks = workers / per_block;
+  std::vector<int> rank(workers);
+  for (int p = 0; p < workers; ++p) rank[(p % blocks) * per_block + p / blocks] = p;
+  std::vector<std::vector<int>> items(bodies);
+  for (int i = 0; i < static_cast<int>(body.size()); ++i) items[body[i]].push_back(i);
+  std::vector<double> load(workers, 0.0);
+  std::vector<int> order(workers);
+  for (int b = 0; b < bodies; ++b) {
+    std::vector<int>& it = items[b];
+    std::stable_sort(it.begin(), it.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    for (size_t k0 = 0; k0 < it.size(); k0 += static_cast<size_t>(workers)) {
+      for (int w = 0; w < workers; ++w) order[w] = w;
+      std::sort(order.begin(), order.end(),
+                [&](int x, int y) { return load[x] != load[y] ? load[x] < load[y] : rank[x] < rank[y]; });
+      const size_t k1 = std::min(it.size(), k0 + static_cast<size_t>(workers));
+      for (size_t k = k0; k < k1; ++k) {
+        const int w = order[k - k0];
+        owner[it[k]] = w;
+        load[w] += cost[it[k]];
+      }
+    }
+  }
+}
+}  // namespace
+
+void balanced_lists(const std::vector<double>& cost, const std::vector<int>& body, int bodies, int workers,
+                    int per_block, std::vector<int>& off, std::vector<int>& list, bool spread) {
+  const int n = static_cast<int>(cost.size());
+  if (static_cast<int>(body.size()) != n || workers < 1 || per_block < 1 || workers % per_block != 0)
+    throw std::invalid_argument("balanced_lists: inconsistent sizes");
+  for (int i = 0; i < n; ++i)
+    if (body[i] < 0 || body[i] >= bodies) throw std::invalid_argument("balanced_lists: body out of range");
+  std::vector<int> owner(n, 0);
+  if (spread) {
+    spread_owners(cost, body, bodies, workers, per_block, owner);
+  } else {
+    xcd_owners(cost, body, bodies, workers, per_block, owner);
+  }
   off.assign(workers + 1, 0);
   for (int i = 0; i < n; ++i) ++off[owner[i] + 1];
   for (int w = 0; w < workers; ++w) off[w + 1] += off[w];

@@ -73,10 +73,13 @@ FactorLayout factor_layout(const Csr& a, const Factor& f);
 // first spread over the XCDs (costliest first, each onto the least loaded),
 // and then each XCD's items over that XCD's workers the same way: a body's
 // items then share one L2. If the bodies do not split over the XCDs within
-// 5% of even, all items are spread over all workers instead. Worker w's
+// 5% of even, all items are spread over all workers instead. With `spread`
+// (the lists that run while few bodies are active), each body's items go to
+// distinct workers across all XCDs instead, costliest first onto the least
+// loaded, so that a lone body still streams through every XCD. Worker w's
 // items, ascending, are list[off[w] .. off[w + 1]).
 void balanced_lists(const std::vector<double>& cost, const std::vector<int>& body, int bodies, int workers,
-                    int per_block, std::vector<int>& off, std::vector<int>& list);
+                    int per_block, std::vector<int>& off, std::vector<int>& list, bool spread = false);
 
 // Validates like DP/sparse.py:73-95 (finite, positive diagonal, exactly
 // symmetric) and drops exact zeros. Throws std::runtime_error.

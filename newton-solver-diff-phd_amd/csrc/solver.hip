@@ -243,7 +243,7 @@ void build_sliced(DeviceState& d, const std::vector<ColGroup>& t_items, int rows
     cost.push_back(1.0 + (g.e1 - g.e0 + batch - 1) / batch);
     body.push_back(g.body);
   }
-  balanced_lists(cost, body, bodies, d.g_ts, 1, off, list);
+  balanced_lists(cost, body, bodies, d.g_ts, 1, off, list, true);
   d.ts_woff = dupload(owned, off);
   d.ts_list = dupload(owned, list);
 }
@@ -290,7 +290,7 @@ void build_split(DeviceState& d, const std::vector<RowItem>& s_items, int quads,
     cost.push_back(0.25 + (it.quads + kWave - 1) / kWave);
     body.push_back(it.body);
   }
-  balanced_lists(cost, body, bodies, d.g_ss * (kBlock / kWave), kBlock / kWave, off, list);
+  balanced_lists(cost, body, bodies, d.g_ss * (kBlock / kWave), kBlock / kWave, off, list, true);
   d.ss_woff = dupload(owned, off);
   d.ss_list = dupload(owned, list);
 }
@@ -605,16 +605,22 @@ void build_solver(dphd_solver* s, double dt) {
     const long long voff_b = static_cast<long long>(fval.size());
     for (const FactorBlock& fb : L.blocks) {
       const long long width = 4 * static_cast<long long>(fb.quads);
-      for (int r = 0; r < fb.rows; r += kRowsPerGroup) {
+      // Narrow blocks: 2^sh lanes per row, kPackedPasses x (64 >> sh) rows per item.
+      int sh = 0;
+      while ((1 << sh) < fb.quads) ++sh;
+      const bool packed = fb.quads <= kPackedQuads;
+      const int per = packed ? kPackedPasses * (kWave >> sh) : kRowsPerGroup;
+      for (int r = 0; r < fb.rows; r += per) {
         RowItem it{};
         it.offset = voff_b + fb.offset + width * r;
         it.col0 = c0 + fb.col0;
         it.quads = fb.quads;
         it.out_row = c0 + fb.row0 + r;
-        it.count = std::min(kRowsPerGroup, fb.rows - r);
+        it.count = std::min(per, fb.rows - r);
         it.body = b;
         it.part = -1;
         it.stride = fb.quads;
+        it.packed = packed ? sh + 1 : 0;
         s_items.push_back(it);
       }
     }
@@ -687,6 +693,23 @@ void build_solver(dphd_solver* s, double dt) {
   for (int e = 0; e < E; ++e)
     for (int k = 0; k < K; ++k)
       if (!free[s->indices[static_cast<size_t>(e) * K + k]]) pinmask[e] |= static_cast<uint8_t>(1u << k);
+  // Local-step staging: the distinct vertices of each element chunk
+  // (ascending) and each corner's position in its chunk's list. A chunk
+  // holds at most 4 kBlock corners, so a position fits 16 bits.
+  std::vector<int> lv_off(1, 0), lv_list;
+  std::vector<uint16_t> lidx(static_cast<size_t>(E) * K);
+  int lv_max = 0;
+  for (const Chunk& c : echunks) {
+    const size_t a0 = static_cast<size_t>(c.begin) * K, a1 = static_cast<size_t>(c.end) * K;
+    std::vector<int> verts(s->indices.begin() + a0, s->indices.begin() + a1);
+    std::sort(verts.begin(), verts.end());
+    verts.erase(std::unique(verts.begin(), verts.end()), verts.end());
+    for (size_t a = a0; a < a1; ++a)
+      lidx[a] = static_cast<uint16_t>(std::lower_bound(verts.begin(), verts.end(), s->indices[a]) - verts.begin());
+    lv_list.insert(lv_list.end(), verts.begin(), verts.end());
+    lv_off.push_back(static_cast<int>(lv_list.size()));
+    lv_max = std::max(lv_max, static_cast<int>(verts.size()));
+  }
   // Sliced-ELL pattern of the adjoint operator: row i holds itself and every
   // vertex that shares an element with it, ascending.
   const int nvc_host = static_cast<int>(vchunks.size());
@@ -776,6 +799,10 @@ void build_solver(dphd_solver* s, double dt) {
   if (!hex) d.poses = dupload(o, s->poses);
   if (hex) d.hexg = dupload(o, hexg);
   d.pinmask = dupload(o, pinmask);
+  d.lv_off = dupload(o, lv_off);
+  d.lv_list = dupload(o, lv_list);
+  d.lidx = dupload(o, lidx);
+  d.lv_max = lv_max;
   d.ebody = dupload(o, ebody);
   d.vbody = dupload(o, vbody);
   d.free = dupload(o, free);
