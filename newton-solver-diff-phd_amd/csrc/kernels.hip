@@ -55,6 +55,28 @@ __device__ __forceinline__ double chunk_sum(const double* parts, int stride, int
   return t;
 }
 
+// chunk_sum of a[stride_a * c] and of b[stride_b * c] at once, each in chunk
+// order, the 16 loads of a round in flight together.
+__device__ __forceinline__ void chunk_sum2(const double* a, int stride_a, const double* b, int stride_b, int c0, int c1,
+                                           double& ta, double& tb) {
+  ta = 0.0;
+  tb = 0.0;
+  for (int c = c0; c < c1; c += 8) {
+    double pa[8], pb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      pa[u] = a[static_cast<size_t>(stride_a) * min(c + u, c1 - 1)];
+      pb[u] = b[static_cast<size_t>(stride_b) * min(c + u, c1 - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (c + u < c1) {
+        ta += pa[u];
+        tb += pb[u];
+      }
+  }
+}
+
 // chunk_sum of components 0 .. N-1 at once (each in chunk order, as
 // chunk_sum adds it), all 8 x N loads of a round in flight together.
 template <int N>
@@ -893,8 +915,17 @@ __global__ __launch_bounds__(kBlock) void k_rows_combine(const RowCombine* __res
     const int w = t >> 5, k = t & 31;
     const RowCombine it = items[w];
     if (!active[it.body] || (k >> 2) >= it.count) continue;
+    // Chunks eight at a time, all their loads in flight, added in column order.
     Acc acc = Acc(0);
-    for (int p = it.part0; p < it.part0 + it.parts; ++p) acc += part_buf[static_cast<size_t>(p) * 32 + k];
+    const int p1 = it.part0 + it.parts;
+    for (int p = it.part0; p < p1; p += 8) {
+      Acc v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = part_buf[static_cast<size_t>(min(p + u, p1 - 1)) * 32 + k];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (p + u < p1) acc += v[u];
+    }
     out[4 * static_cast<size_t>(it.out_row) + k] = static_cast<In>(acc);
   }
 }
@@ -1104,10 +1135,22 @@ __global__ __launch_bounds__(kBlock) void k_factor_combine(const ColCombine* __r
     Acc acc[4][3];
 #pragma unroll
     for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] = Acc(0);
-    for (int p = it.part0; p < it.part0 + it.parts; ++p) {
-      const Acc* src = part_buf + static_cast<size_t>(p) * 12 * kWave + lane;
+    // Slices four at a time, all their loads in flight, added in slice order.
+    const int p1 = it.part0 + it.parts;
+    for (int p = it.part0; p < p1; p += 4) {
+      Acc v[4][12];
 #pragma unroll
-      for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] += src[t * kWave];
+      for (int u = 0; u < 4; ++u) {
+        const Acc* src = part_buf + static_cast<size_t>(min(p + u, p1 - 1)) * 12 * kWave + lane;
+#pragma unroll
+        for (int t = 0; t < 12; ++t) v[u][t] = src[t * kWave];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (p + u < p1) {
+#pragma unroll
+          for (int t = 0; t < 12; ++t) acc[t / 3][t % 3] += v[u][t];
+        }
     }
     cols_store<Acc>(acc, it.c0 + 4 * lane, it.end, gorig, base, out, adj, it.gid, lane);
   }
@@ -1793,8 +1836,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_apply(DeviceState d) {
 __global__ __launch_bounds__(kBlock) void k_adj_check(DeviceState d, StepScalars s) {
   for (int b = threadIdx.x; b < d.B; b += kBlock) {
     if (!d.aactive[b]) continue;
-    const double t0 = chunk_sum(d.conv_part, 4, 0, d.vcb[b], d.vcb[b + 1]);
-    const double sq = chunk_sum(d.clip_part2, 1, 0, d.vcb[b], d.vcb[b + 1]);
+    double t0, sq;
+    chunk_sum2(d.conv_part, 4, d.clip_part2, 1, d.vcb[b], d.vcb[b + 1], t0, sq);
     const double sc = clip_factor(sq);
     if (sc != 1.0) atomicOr(d.status, 16);
     d.fscale[b] = sc;
