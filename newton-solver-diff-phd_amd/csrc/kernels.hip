@@ -606,6 +606,31 @@ __global__ __launch_bounds__(kBlock) void k_local(DeviceState d, const double* t
 // ---------------------------------------------------------------------------
 // Gather + right-hand side (+ projection-change partials).
 
+// Row i of the FP64 constant matrix times x (FP32-factor mode): the row's
+// entries eight at a time (columns and values, then the x rows, all in
+// flight), added in column order.
+__device__ __forceinline__ V3 a_row_times(const DeviceState& d, int i, const double* x) {
+  V3 ax{0, 0, 0};
+  const int q0 = d.a_off[i], q1 = d.a_off[i + 1];
+  for (int q = q0; q < q1; q += 8) {
+    int col[8];
+    double w[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int qq = min(q + u, q1 - 1);
+      col[u] = d.a_col[qq];
+      w[u] = d.a_val[qq];
+    }
+    V3 xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = ld3(x, col[u]);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (q + u < q1) ax = ax + w[u] * xv[u];
+  }
+  return ax;
+}
+
 __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d) {
   __shared__ double smem[2 * kBlock / kWave];
   __shared__ double res[2];
@@ -614,8 +639,21 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d) {
     const int i = c.begin + threadIdx.x;
     double part[2] = {0.0, 0.0};
     if (i < c.end) {
+      // The incident entries eight at a time (entry loads, then their
+      // contributions, all in flight), added in incidence order.
       V3 t{0, 0, 0};
-      for (int a = d.inc_off[i]; a < d.inc_off[i + 1]; ++a) t = t + ld3(d.contrib, d.inc_ent[a]);
+      const int a0 = d.inc_off[i], a1 = d.inc_off[i + 1];
+      for (int a = a0; a < a1; a += 8) {
+        int ent[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) ent[u] = d.inc_ent[min(a + u, a1 - 1)];
+        V3 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = ld3(d.contrib, ent[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (a + u < a1) t = t + v[u];
+      }
       if (d.planar_w) {
         // planar_projection (hex.py:246-260): k (x - min(0, n.x + d) n - x_D).
         const double kw = d.planar_w[i];
@@ -629,8 +667,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d) {
       const V3 old = ld3(d.p, i);
       V3 r = d.inertia[b] * ld3(d.qt, i) + t;
       if (d.defect) {
-        V3 ax{0, 0, 0};
-        for (int q = d.a_off[i]; q < d.a_off[i + 1]; ++q) ax = ax + d.a_val[q] * ld3(d.x, d.a_col[q]);
+        const V3 ax = a_row_times(d, i, d.x);
         r = r - ax;
         if (d.fc_vslot) {
           const int c = d.fc_vslot[i];
@@ -1161,9 +1198,7 @@ __global__ __launch_bounds__(kBlock) void k_factor_combine(const ColCombine* __r
 __global__ __launch_bounds__(kBlock) void k_residual_a(DeviceState d, const double* rhs, const double* x, double* r) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.N) return;
-  V3 ax{0, 0, 0};
-  for (int q = d.a_off[i]; q < d.a_off[i + 1]; ++q) ax = ax + d.a_val[q] * ld3(x, d.a_col[q]);
-  st3(r, i, ld3(rhs, i) - ax);
+  st3(r, i, ld3(rhs, i) - a_row_times(d, i, x));
 }
 
 __global__ __launch_bounds__(kBlock) void k_add3(int n, const double* x, double* y) {
