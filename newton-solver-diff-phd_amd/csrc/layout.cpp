@@ -238,21 +238,30 @@ void spread_owners(const std::vector<double>& cost, const std::vector<int>& body
   for (int p = 0; p < workers; ++p) rank[(p % blocks) * per_block + p / blocks] = p;
   std::vector<std::vector<int>> items(bodies);
   for (int i = 0; i < static_cast<int>(body.size()); ++i) items[body[i]].push_back(i);
-  std::vector<double> load(workers, 0.0);
-  std::vector<int> order(workers);
+  // Workers by (load, rank), least first: a body's round takes the first
+  // ones off the heap and puts them back with their new loads afterwards.
+  using Slot = std::pair<double, int>;  // (load, rank)
+  std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
+  std::vector<int> by_rank(workers);
+  for (int w = 0; w < workers; ++w) {
+    by_rank[rank[w]] = w;
+    heap.push(Slot{0.0, rank[w]});
+  }
+  std::vector<Slot> taken;
   for (int b = 0; b < bodies; ++b) {
     std::vector<int>& it = items[b];
     std::stable_sort(it.begin(), it.end(), [&](int x, int y) { return cost[x] > cost[y]; });
     for (size_t k0 = 0; k0 < it.size(); k0 += static_cast<size_t>(workers)) {
-      for (int w = 0; w < workers; ++w) order[w] = w;
-      std::sort(order.begin(), order.end(),
-                [&](int x, int y) { return load[x] != load[y] ? load[x] < load[y] : rank[x] < rank[y]; });
       const size_t k1 = std::min(it.size(), k0 + static_cast<size_t>(workers));
+      taken.clear();
       for (size_t k = k0; k < k1; ++k) {
-        const int w = order[k - k0];
-        owner[it[k]] = w;
-        load[w] += cost[it[k]];
+        Slot t = heap.top();
+        heap.pop();
+        owner[it[k]] = by_rank[t.second];
+        t.first += cost[it[k]];
+        taken.push_back(t);
       }
+      for (const Slot& t : taken) heap.push(t);
     }
   }
 }
