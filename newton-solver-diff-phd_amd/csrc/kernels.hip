@@ -1407,10 +1407,74 @@ __global__ __launch_bounds__(kBlock) void k_apply_conv(DeviceState d) {
   });
 }
 
+// Prefix sums over the bodies of NC per-body counts at once, one workgroup
+// (every thread calls it): count c of body b is base[c][b + 1] - base[c][b]
+// when flag[b] (LDS) is set, else 0; out[c][b] receives the counts of the
+// bodies before b and out[c][B] their total (a null out[c] is not stored);
+// tot[c] = the total, in every thread. Thread t takes a contiguous range of
+// bodies; its range totals are scanned within the wavefront by shuffles and
+// across the four wavefronts through LDS (integer sums, exact in any order).
+template <int NC>
+__device__ void body_prefixes(const int* flag, const int* const (&base)[NC], int* const (&out)[NC], int B,
+                              int (&tot)[NC]) {
+  __shared__ int wtot[NC][kBlock / kWave];
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const int per = (B + kBlock - 1) / kBlock;
+  const int b0 = min(B, static_cast<int>(threadIdx.x) * per), b1 = min(B, b0 + per);
+  int local[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) local[c] = 0;
+  for (int b = b0; b < b1; ++b)
+    if (flag[b]) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) local[c] += base[c][b + 1] - base[c][b];
+    }
+  int incl[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) incl[c] = local[c];
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int t = __shfl_up(incl[c], off, kWave);
+      if (lane >= off) incl[c] += t;
+    }
+  if (lane == kWave - 1) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) wtot[c][wave] = incl[c];
+  }
+  __syncthreads();
+  int run[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    run[c] = incl[c] - local[c];
+    tot[c] = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) {
+      if (w < wave) run[c] += wtot[c][w];
+      tot[c] += wtot[c][w];
+    }
+  }
+  for (int b = b0; b < b1; ++b) {
+    const bool on = flag[b] != 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (out[c]) out[c][b] = run[c];
+      if (on) run[c] += base[c][b + 1] - base[c][b];
+    }
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (out[c]) out[c][B] = tot[c];
+  }
+}
+
 // Every body's convergence test (one thread per body, one workgroup), then
 // the work-list prefixes of the bodies still active.
 __global__ __launch_bounds__(kBlock) void k_conv_check(DeviceState d, StepScalars s) {
+  extern __shared__ int flag[];  // [B]: still active after this test
   for (int b = threadIdx.x; b < d.B; b += kBlock) {
+    flag[b] = 0;
     if (!d.active[b]) continue;
     const int c0 = d.vcb[b], c1 = d.vcb[b + 1];
     double red[4];
@@ -1424,11 +1488,17 @@ __global__ __launch_bounds__(kBlock) void k_conv_check(DeviceState d, StepScalar
     } else if (k + 1 >= s.max_iterations) {
       d.active[b] = 0;
       atomicOr(d.status, 1);
+    } else {
+      flag[b] = 1;
     }
   }
-  write_prefix(d.active, d.vcb, d.B, d.pref_fv);
-  write_prefix(d.active, d.ecb, d.B, d.pref_fe);
-  write_mode(d.active, d.tcb, d.B, d.t_switch, d.s_switch, d.t_mode);
+  __syncthreads();
+  int tot[3];
+  body_prefixes<3>(flag, {d.vcb, d.ecb, d.tcb}, {d.pref_fv, d.pref_fe, nullptr}, d.B, tot);
+  if (threadIdx.x == 0) {
+    d.t_mode[0] = tot[2] < d.t_switch ? 1 : 0;
+    d.t_mode[2] = tot[2] < d.s_switch ? 1 : 0;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1869,7 +1939,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_apply(DeviceState d) {
 // the loop), and its squared norm for the gate is fscale^2 times the
 // iterate's (exactly the iterate's when no clip applies).
 __global__ __launch_bounds__(kBlock) void k_adj_check(DeviceState d, StepScalars s) {
+  extern __shared__ int flag[];  // [B]: still active after this test
   for (int b = threadIdx.x; b < d.B; b += kBlock) {
+    flag[b] = 0;
     if (!d.aactive[b]) continue;
     double t0, sq;
     chunk_sum2(d.conv_part, 4, d.clip_part2, 1, d.vcb[b], d.vcb[b + 1], t0, sq);
@@ -1885,10 +1957,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_check(DeviceState d, StepScalars
       d.aactive[b] = 0;
       // Woodbury columns are truncated by design; only the backbone reports.
       if (s.report) atomicOr(d.status, 8);
+    } else {
+      flag[b] = 1;
     }
   }
-  write_prefix(d.aactive, d.vcb, d.B, d.pref_av);
-  write_mode(d.aactive, d.tcb, d.B, d.t_switch, d.s_switch, d.t_mode);
+  __syncthreads();
+  int tot[2];
+  body_prefixes<2>(flag, {d.vcb, d.tcb}, {d.pref_av, nullptr}, d.B, tot);
+  if (threadIdx.x == 0) {
+    d.t_mode[0] = tot[1] < d.t_switch ? 1 : 0;
+    d.t_mode[2] = tot[1] < d.s_switch ? 1 : 0;
+  }
 }
 
 // mu = fscale * xr of every vertex, after an adjoint loop (the bodies that
