@@ -90,7 +90,7 @@ typedef struct {
   int factor_precision;  /* 64: FP64 S values; 32: FP32 S values with an FP64 defect correction */
   int integrate_pinned_velocities;
   int jacobian_mode;     /* 0: reference mean weights in K; 1: per-element weights (exact derivative) */
-  int iteration_chunk;   /* iterations per captured graph launch; 0 picks the default */
+  int iteration_chunk;   /* iterations per captured graph launch; 0 picks the defaults (4 forward, 8 adjoint) */
   int slice_rows;        /* rows per slice of the sliced S^T product; 0 = tuned at dphd_build on this solver's
                             factor (the choice is in dphd_info), > 0 fixes it (bitwise repeatable builds) */
 } dphd_config;

@@ -206,7 +206,14 @@ struct dphd_solver {
 namespace dphd {
 namespace {
 
-int chunk_size(const dphd_solver* s) { return s->cfg.iteration_chunk > 0 ? s->cfg.iteration_chunk : 8; }
+// Iterations per captured graph: the configured count, or by default 4 for
+// the forward loop (a few iterations per step, so a long chunk runs mostly
+// predicated off) and 8 for the adjoint loops (where shorter chunks cost
+// more host waits than they save).
+int chunk_size(const dphd_solver* s, int mode) {
+  if (s->cfg.iteration_chunk > 0) return s->cfg.iteration_chunk;
+  return mode == 0 ? 4 : 8;
+}
 
 void build_mesh_contact(dphd_solver* s);
 void build_friction(dphd_solver* s);
@@ -1226,7 +1233,7 @@ hipGraphExec_t capture(dphd_solver* s, int mode) {
   StepScalars sc = scalars(s);
   sc.max_iterations = loop_limit(s, mode);
   sc.report = mode == kWoodbury ? 0 : 1;
-  const int C = chunk_size(s);
+  const int C = chunk_size(s, mode);
   hipGraph_t graph = nullptr;
   HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   for (int c = 0; c < C; ++c) {
@@ -1256,7 +1263,7 @@ int run_loop(dphd_solver* s, int mode) {
     s->key_abs[mode] = s->cfg.abs_tol;
     s->key_rel[mode] = s->cfg.rel_tol;
   }
-  const int C = chunk_size(s);
+  const int C = chunk_size(s, mode);
   const int max_chunks = (loop_limit(s, mode) + C - 1) / C;
   const int last = s->last_iters[mode];
   // First burst: the chunks the previous solve needed (its most iterations of
@@ -1285,7 +1292,7 @@ void run_adjoint_direct(dphd_solver* s, const DeviceState& d) {
   StepScalars sc = scalars(s);
   sc.max_iterations = loop_limit(s, kAdjoint);
   sc.report = 1;
-  const int C = chunk_size(s);
+  const int C = chunk_size(s, kAdjoint);
   const int max_chunks = (sc.max_iterations + C - 1) / C;
   for (int launched = 0; launched < max_chunks; ++launched) {
     for (int c = 0; c < C; ++c) launch_adjoint_iteration(d, sc, s->stream);
