@@ -142,6 +142,8 @@ struct DeviceState {
   // lv_list[lv_off[c] .. lv_off[c + 1]) (ascending); corner k of element e is
   // entry lidx[e*K + k] of its chunk's list; lv_max is the longest list.
   int* lv_off = nullptr;       // [nec + 1]
+  int* pin_chunks = nullptr;   // element chunks with a pinned corner, ascending
+  int npin_chunks = 0;
   int* lv_list = nullptr;
   uint16_t* lidx = nullptr;    // [E*K]
   int lv_max = 0;

@@ -1630,8 +1630,10 @@ __device__ __forceinline__ void coupling_hex8_chunk(const DeviceState& d, int ci
 }
 
 // Dirichlet coupling rows over every element chunk (once per backward).
+// One workgroup per element chunk that holds a pinned corner (the other
+// chunks have no coupling rows).
 __global__ __launch_bounds__(kBlock) void k_adj_coupling(DeviceState d) {
-  const int ci = blockIdx.x;
+  const int ci = d.pin_chunks[blockIdx.x];
   const int b = d.echunks[ci].body;
   if (d.Q == 1)
     coupling_tet4_chunk(d, ci, b, d.sol);
@@ -2446,7 +2448,7 @@ void launch_adjoint_outputs(const DeviceState& d, const StepScalars& s, const do
   hipLaunchKernelGGL(k_adj_state_grads, dim3(d.nvc), dim3(kBlock), 0, stream, d, s.dt, gqd_out, gq_in, gqd_in,
                      gf_in);
   if (any_pinned) {
-    hipLaunchKernelGGL(k_adj_coupling, dim3(d.nec), dim3(kBlock), 0, stream, d);
+    if (d.npin_chunks) hipLaunchKernelGGL(k_adj_coupling, dim3(d.npin_chunks), dim3(kBlock), 0, stream, d);
     hipLaunchKernelGGL(k_adj_dirichlet, dim3(d.nvc), dim3(kBlock), 0, stream, d, s.dt, gq_in, gqd_in, gdir);
   }
   if (gmat) {

@@ -820,6 +820,17 @@ void build_solver(dphd_solver* s, double dt) {
   d.lv_list = dupload(o, lv_list);
   d.lidx = dupload(o, lidx);
   d.lv_max = lv_max;
+  {
+    std::vector<int> pin_chunks;
+    for (int ci = 0; ci < static_cast<int>(echunks.size()); ++ci)
+      for (int e = echunks[ci].begin; e < echunks[ci].end; ++e)
+        if (pinmask[e]) {
+          pin_chunks.push_back(ci);
+          break;
+        }
+    d.pin_chunks = dupload(o, pin_chunks);
+    d.npin_chunks = static_cast<int>(pin_chunks.size());
+  }
   d.ebody = dupload(o, ebody);
   d.vbody = dupload(o, vbody);
   d.free = dupload(o, free);
