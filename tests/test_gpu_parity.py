@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 import torch
 
+from diffphd_amd import meshes
 from oracle import oracle as O
 from scenes import cube_body, golden, tet_model
 
@@ -293,6 +294,50 @@ def test_cube_rollout_matches_oracle(cuda, precision):
             slack = 2
         np.testing.assert_allclose(xs[t], out["x"], rtol=0, atol=atol, err_msg=f"step {t}")
         assert abs(int(iters[t][0]) - out["iterations"]) <= slack, (t, iters[t], out["iterations"])
+        q, qd = out["q"], out["qd"]
+
+
+def test_disconnected_tets_with_scattered_ids_match_oracle(cuda):
+    """The local step stages each 256-tet chunk's distinct vertices in LDS and reads corners by 16-bit
+    positions: tets that share no vertex, under a random vertex numbering, give chunks of 1024 distinct
+    vertices (the largest a chunk can have) spread over the whole id range."""
+    rng = np.random.default_rng(5)
+    n_tets = 600
+    base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float64) * 0.01
+    coords = np.zeros((4 * n_tets, 3))
+    for t in range(n_tets):
+        shape = base + rng.uniform(-0.002, 0.002, size=(4, 3))
+        coords[4 * t:4 * t + 4] = shape + np.array([t % 25, (t // 25) % 25, t // 625], np.float64) * 0.03
+    perm = rng.permutation(4 * n_tets)
+    vertices = np.zeros_like(coords)
+    vertices[perm] = coords
+    tets = perm.reshape(n_tets, 4).astype(np.int32)
+    poses = meshes.tet_poses(vertices, tets)
+    young = np.exp(rng.uniform(np.log(1e5), np.log(1e7), n_tets))
+    mu, lam = meshes.lame(young, np.full(n_tets, 0.4))
+    materials = np.stack([mu, lam, np.zeros_like(mu)], axis=1).astype(np.float32)
+    free = np.ones(4 * n_tets, bool)
+    free[tets[:, 0]] = False  # one pinned corner per tet
+    vertices = vertices.astype(np.float32)
+    dt, steps = 1e-3, 3
+    print("[case] 600 disconnected tets, scattered vertex ids, fp64", flush=True)
+    model = tet_model(cuda, tets, poses, materials, free, vertices, (0, 0, -9.81))
+    solver = _solver(model, density=1000.0, rel_tol=1e-6)
+    xs, iters = _rollout_gpu(solver, model, vertices, steps, dt)
+    assert solver.read_status() == 0
+    body = _oracle_body(vertices, tets, poses, materials, free, dt, rel_tol=1e-6)
+    q, qd = vertices, np.zeros_like(vertices)
+    for t in range(steps):
+        out = body.forward(dt, q, qd)
+        disp = np.abs(out["x"] - vertices).max()
+        err = np.abs(xs[t] - out["x"]).max()
+        print(f"[result] step {t}: iterations {int(iters[t][0])} (oracle {out['iterations']}), position error "
+              f"{err:.3e} m of a {disp:.3e} m displacement", flush=True)
+        # Stiff tets of a fraction of a gram: rounding differences in the iteration reach the gate's
+        # level (rel 1e-6 of |x|), not the 1e-9 of the well-conditioned cubes; a misplaced corner would
+        # be off by the displacement itself.
+        assert err <= 1e-5 * disp + 1e-12, (t, err, disp)
+        assert abs(int(iters[t][0]) - out["iterations"]) <= 1, (t, iters[t], out["iterations"])
         q, qd = out["q"], out["qd"]
 
 
