@@ -1363,6 +1363,24 @@ dphd_solver* woodbury_batch(dphd_solver* s) {
   return b;
 }
 
+// The batch solver with its adjoint operator buffers allocated. Built with
+// the factor of a solver with a frictional boundary (dphd_build), so that no
+// backward pays for it; otherwise at the first backward that selects rows.
+dphd_solver* woodbury_ready(dphd_solver* s) {
+  dphd_solver* b = woodbury_batch(s);
+  DeviceState& bd = b->d;
+  if (!bd.jac) {
+    bd.jac = dalloc<double>(b->owned, static_cast<size_t>(b->E) * b->Q * 81);
+    bd.ke = dalloc<double>(b->owned, static_cast<size_t>(b->E) * b->K * b->K * 9);
+    bd.kval = dalloc<double>(b->owned, static_cast<size_t>(bd.kblocks) * 9);
+    for (int m = kAdjoint; m <= kWoodbury; ++m) {
+      if (b->exec[m]) HIP_CHECK(hipGraphExecDestroy(b->exec[m]));
+      b->exec[m] = nullptr;
+    }
+  }
+  return b;
+}
+
 // Woodbury projection of the adjoint for the parameter gradients; returns z
 // (Adjoint._woodbury, adjoint.py:1272-1354). The selected rows' columns
 // W_T = (A - K)^{-1} J^T are solved in batches, one column per body of the
@@ -1383,22 +1401,13 @@ const double* woodbury(dphd_solver* s, const double* checkpoint, const double* t
     s->wb_capacity = count;
   }
   if (count > 0) {
-    dphd_solver* b = woodbury_batch(s);
+    dphd_solver* b = woodbury_ready(s);
     DeviceState& bd = b->d;
     const int C = b->B;
     // The tolerances may have changed between steps (dphd_set_tolerances).
     b->cfg.max_iterations = s->cfg.max_iterations;
     b->cfg.abs_tol = s->cfg.abs_tol;
     b->cfg.rel_tol = s->cfg.rel_tol;
-    if (!bd.jac) {
-      bd.jac = dalloc<double>(b->owned, static_cast<size_t>(b->E) * b->Q * 81);
-      bd.ke = dalloc<double>(b->owned, static_cast<size_t>(b->E) * b->K * b->K * 9);
-      bd.kval = dalloc<double>(b->owned, static_cast<size_t>(bd.kblocks) * 9);
-      for (int m = kAdjoint; m <= kWoodbury; ++m) {
-        if (b->exec[m]) HIP_CHECK(hipGraphExecDestroy(b->exec[m]));
-        b->exec[m] = nullptr;
-      }
-    }
     // The adjoint operator at the recorded step, once for every copy.
     for (int c = 0; c < C; ++c) {
       HIP_CHECK(hipMemcpyAsync(s->wb_ckpt + static_cast<size_t>(c) * 3 * s->N, checkpoint, vb,
@@ -1829,6 +1838,7 @@ int dphd_build(dphd_solver* s, double dt) {
   DPHD_TRY({
     if (!s) throw std::invalid_argument("null solver");
     dphd::build_solver(s, dt);
+    if (s->has_friction && s->wb.R > 0) dphd::woodbury_ready(s);
   })
 }
 
