@@ -167,10 +167,13 @@ int dphd_contact_state(dphd_solver* s, const double* set_multiplier, double* mul
  * gradients with (StepRecord.contact, DP/adjoint.py:809-841): multipliers
  * [K], omega [K], normals [K,3] of the recorded step, e.g. copies taken with
  * dphd_contact_state after it. Every dphd_step records its own; this
- * replaces it for the backward of an earlier step. Host or device pointers. */
+ * replaces it for the backward of an earlier step. Host or device pointers;
+ * waits for the solver's queued work and synchronizes. */
 int dphd_contact_record(dphd_solver* s, const double* multiplier, const double* omega, const double* normal);
 /* Mesh contact force [N,3] float64 of float64 positions/velocities [N,3]
- * (MeshContact.evaluate, DP/mesh_contact.py:245-272); device pointers. */
+ * (MeshContact.evaluate, DP/mesh_contact.py:245-272); device pointers. Runs
+ * on `stream` after the solver's queued work, and the solver's later work
+ * runs after it. */
 int dphd_mesh_contact_force(dphd_solver* s, const double* position, const double* velocity, double* force,
                             void* stream);
 /* Vertex-face pairs within the contact radius in the last evaluation
@@ -185,7 +188,9 @@ int dphd_mesh_contact_pairs(dphd_solver* s, int* pairs);
 int dphd_mesh_contact_pair_counts(dphd_solver* s, int* counts);
 /* Mutable between steps (DP/solver.py:206-209). */
 int dphd_set_tolerances(dphd_solver* s, int max_iterations, double abs_tol, double rel_tol);
-/* Element weights, assembly and factor per body (DP/solver.py:491-552). */
+/* Element weights, assembly and factor per body (DP/solver.py:491-552);
+ * with a frictional boundary also the solver of the backward's Woodbury
+ * columns. Returns with the device idle. */
 int dphd_build(dphd_solver* s, double dt);
 int dphd_get_info(const dphd_solver* s, dphd_info* info);
 /* nnz(S) of every body after dphd_build (per_body [B]); sizes the byte model. */
@@ -240,7 +245,9 @@ int dphd_iterations(dphd_solver* s, int adjoint, int* per_body);
 int dphd_synchronize(dphd_solver* s);
 
 /* Device-side application of A^{-1} = S^T S of every body to [N,3] float64
- * device arrays (SparseInverse.apply, DP/sparse.py:350-352). */
+ * device arrays (SparseInverse.apply, DP/sparse.py:350-352). Runs on
+ * `stream` after the solver's queued work (it shares the solver's product
+ * buffers), and the solver's later work runs after it. */
 int dphd_apply_inverse(dphd_solver* s, const double* rhs, double* out, void* stream);
 
 /* det(F) at the eight Gauss samples of every Hex8 cell, out [E,8] float64,

@@ -1227,6 +1227,18 @@ void build_friction(dphd_solver* s) {
   HIP_CHECK(hipDeviceSynchronize());
 }
 
+// Work that a caller queues on its own stream and that shares this solver's
+// device buffers: ordered after everything queued on the solver's stream
+// (join_user), and the solver's later work after it (release_user).
+void join_user(dphd_solver* s, hipStream_t user) {
+  HIP_CHECK(hipEventRecord(s->ev_out, s->stream));
+  HIP_CHECK(hipStreamWaitEvent(user, s->ev_out, 0));
+}
+void release_user(dphd_solver* s, hipStream_t user) {
+  HIP_CHECK(hipEventRecord(s->ev_in, user));
+  HIP_CHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+}
+
 StepScalars scalars(const dphd_solver* s) {
   return StepScalars{s->dt, s->cfg.abs_tol, s->cfg.rel_tol, s->cfg.max_iterations};
 }
@@ -1779,9 +1791,13 @@ int dphd_contact_record(dphd_solver* s, const double* multiplier, const double* 
     HIP_CHECK(hipSetDevice(s->device));
     const size_t K = s->fc.K;
     if (K) {
+      // After the solver's queued work (a backward may still read the
+      // recorded state), and complete before its next.
+      HIP_CHECK(hipStreamSynchronize(s->stream));
       HIP_CHECK(hipMemcpy(s->rec_mult, multiplier, sizeof(double) * K, hipMemcpyDefault));
       HIP_CHECK(hipMemcpy(s->rec_omega, omega, sizeof(double) * K, hipMemcpyDefault));
       HIP_CHECK(hipMemcpy(s->rec_normal, normal, sizeof(double) * 3 * K, hipMemcpyDefault));
+      HIP_CHECK(hipDeviceSynchronize());
     }
   })
 }
@@ -1793,10 +1809,12 @@ int dphd_mesh_contact_force(dphd_solver* s, const double* position, const double
       throw std::invalid_argument("dphd_mesh_contact_force: needs a built solver with mesh contact");
     HIP_CHECK(hipSetDevice(s->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
+    dphd::join_user(s, st);
     dphd::launch_mesh_contact(s->mc, nullptr, nullptr, position, velocity, nullptr, nullptr, nullptr, 0.0, st);
     HIP_CHECK(hipMemcpyAsync(force, s->mc.force, sizeof(double) * 3 * static_cast<size_t>(s->N),
                              hipMemcpyDeviceToDevice, st));
     HIP_CHECK(hipGetLastError());
+    dphd::release_user(s, st);
   })
 }
 
@@ -1839,6 +1857,9 @@ int dphd_build(dphd_solver* s, double dt) {
     if (!s) throw std::invalid_argument("null solver");
     dphd::build_solver(s, dt);
     if (s->has_friction && s->wb.R > 0) dphd::woodbury_ready(s);
+    // The build's last launches (the tuner's, the batch solver's) finish
+    // before any caller's work can reach the buffers they use.
+    HIP_CHECK(hipDeviceSynchronize());
   })
 }
 
@@ -2059,8 +2080,10 @@ int dphd_apply_inverse(dphd_solver* s, const double* rhs, double* out, void* str
     if (!s || !s->built || !rhs || !out) throw std::invalid_argument("dphd_apply_inverse: invalid arguments");
     HIP_CHECK(hipSetDevice(s->device));
     hipStream_t user = static_cast<hipStream_t>(stream);
+    dphd::join_user(s, user);
     dphd::launch_apply_inverse(s->d, rhs, out, user);
     HIP_CHECK(hipGetLastError());
+    dphd::release_user(s, user);
   })
 }
 
