@@ -168,6 +168,10 @@ struct dphd_solver {
   double* targets = nullptr;  // device copy of muscle targets [E]
   hipStream_t stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  // A caller stream whose queued work (apply_inverse, mesh_contact_force)
+  // shares this solver's buffers; the solver's next stream work waits for it.
+  hipStream_t user_pending = nullptr;
+  hipEvent_t ev_user = nullptr;
   // Iteration graphs of the forward, the adjoint and the Woodbury columns,
   // keyed on the tolerances they bake in.
   hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
@@ -1010,6 +1014,7 @@ void build_solver(dphd_solver* s, double dt) {
   if (!s->stream) HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   if (!s->ev_in) HIP_CHECK(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming));
   if (!s->ev_out) HIP_CHECK(hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming));
+  if (!s->ev_user) HIP_CHECK(hipEventCreateWithFlags(&s->ev_user, hipEventDisableTiming));
   HIP_CHECK(hipDeviceSynchronize());
   s->dt = dt;
   s->built = true;
@@ -1228,16 +1233,31 @@ void build_friction(dphd_solver* s) {
 }
 
 // Work that a caller queues on its own stream and that shares this solver's
-// device buffers: ordered after everything queued on the solver's stream
-// (join_user), and the solver's later work after it (release_user).
+// device buffers (join_user before it, release_user after it): it runs after
+// everything queued on the solver's stream, and the solver's next work on
+// its stream waits for it (flush_user, at the start of step and backward:
+// the event is recorded then, so consecutive calls on one stream cost no
+// event each). Work pending on another caller stream is ordered first.
+void flush_user(dphd_solver* s) {
+  if (!s->user_pending) return;
+  hipError_t e = hipEventRecord(s->ev_user, s->user_pending);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s->stream, s->ev_user, 0);
+  s->user_pending = nullptr;
+  if (e != hipSuccess) {
+    // The caller's stream is gone: wait for the whole device instead.
+    (void)hipGetLastError();
+    HIP_CHECK(hipDeviceSynchronize());
+  }
+}
 void join_user(dphd_solver* s, hipStream_t user) {
+  if (s->user_pending && s->user_pending != user) flush_user(s);
+  const hipError_t idle = hipStreamQuery(s->stream);
+  if (idle == hipSuccess) return;
+  if (idle != hipErrorNotReady) HIP_CHECK(idle);
   HIP_CHECK(hipEventRecord(s->ev_out, s->stream));
   HIP_CHECK(hipStreamWaitEvent(user, s->ev_out, 0));
 }
-void release_user(dphd_solver* s, hipStream_t user) {
-  HIP_CHECK(hipEventRecord(s->ev_in, user));
-  HIP_CHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
-}
+void release_user(dphd_solver* s, hipStream_t user) { s->user_pending = user; }
 
 StepScalars scalars(const dphd_solver* s) {
   return StepScalars{s->dt, s->cfg.abs_tol, s->cfg.rel_tol, s->cfg.max_iterations};
@@ -1532,16 +1552,20 @@ void dphd_solver::drop_woodbury_batch() {
   b->release();
   if (b->ev_in) (void)hipEventDestroy(b->ev_in);
   if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+  if (b->ev_user) (void)hipEventDestroy(b->ev_user);
   delete b;
 }
 
 void dphd_destroy(dphd_solver* s) {
   if (!s) return;
   (void)hipSetDevice(s->device);
-  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  // Caller-stream work (apply_inverse, mesh_contact_force) may still read
+  // the buffers freed here.
+  (void)hipDeviceSynchronize();
   s->release();
   if (s->ev_in) (void)hipEventDestroy(s->ev_in);
   if (s->ev_out) (void)hipEventDestroy(s->ev_out);
+  if (s->ev_user) (void)hipEventDestroy(s->ev_user);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -1855,6 +1879,10 @@ int dphd_set_tolerances(dphd_solver* s, int max_iterations, double abs_tol, doub
 int dphd_build(dphd_solver* s, double dt) {
   DPHD_TRY({
     if (!s) throw std::invalid_argument("null solver");
+    // Caller-stream work may still read the buffers a rebuild frees.
+    HIP_CHECK(hipSetDevice(s->device));
+    HIP_CHECK(hipDeviceSynchronize());
+    s->user_pending = nullptr;
     dphd::build_solver(s, dt);
     if (s->has_friction && s->wb.R > 0) dphd::woodbury_ready(s);
     // The build's last launches (the tuner's, the batch solver's) finish
@@ -1907,6 +1935,7 @@ int dphd_step(dphd_solver* s, const float* q_in, const float* qd_in, const float
   DPHD_TRY({
     HIP_CHECK(hipSetDevice(s->device));
     hipStream_t user = static_cast<hipStream_t>(stream);
+    dphd::flush_user(s);
     HIP_CHECK(hipEventRecord(s->ev_in, user));
     HIP_CHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
     if (s->d.has_muscles) {
@@ -1970,6 +1999,7 @@ int dphd_backward(dphd_solver* s, const double* checkpoint, const double* muscle
       }
     }
     hipStream_t user = static_cast<hipStream_t>(stream);
+    dphd::flush_user(s);
     HIP_CHECK(hipEventRecord(s->ev_in, user));
     HIP_CHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
     if (s->d.has_muscles && muscle_targets)
@@ -2032,6 +2062,7 @@ int dphd_backward_static(dphd_solver* s, const double* checkpoint, const double*
     d.ke = base + jac_n;
     d.kval = base + jac_n + ke_n;
     hipStream_t user = static_cast<hipStream_t>(stream);
+    dphd::flush_user(s);
     HIP_CHECK(hipEventRecord(s->ev_in, user));
     HIP_CHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
     if (s->d.has_muscles && muscle_targets)
