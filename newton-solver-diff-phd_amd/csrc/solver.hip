@@ -1884,7 +1884,11 @@ int dphd_build(dphd_solver* s, double dt) {
     HIP_CHECK(hipDeviceSynchronize());
     s->user_pending = nullptr;
     dphd::build_solver(s, dt);
-    if (s->has_friction && s->wb.R > 0) dphd::woodbury_ready(s);
+    if (s->has_friction && s->wb.R > 0) {
+      const auto t0 = std::chrono::steady_clock::now();
+      dphd::woodbury_ready(s);
+      s->info.setup_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
     // The build's last launches (the tuner's, the batch solver's) finish
     // before any caller's work can reach the buffers they use.
     HIP_CHECK(hipDeviceSynchronize());
