@@ -1351,8 +1351,8 @@ void run_adjoint_direct(dphd_solver* s, const DeviceState& d) {
 
 // The solver of the Woodbury columns: up to 64 copies of this single body
 // (the reference solves its columns 64 at a time, adjoint.py:662-663), built
-// on the first backward that selects rows and kept until this solver is
-// rebuilt. It shares this solver's stream; its factor is computed once.
+// once (woodbury_ready) and kept until this solver is rebuilt. It shares
+// this solver's stream; its factor is computed once.
 dphd_solver* woodbury_batch(dphd_solver* s) {
   if (s->wb_batch) return s->wb_batch;
   const int C = std::max(1, std::min(64, s->wb.R));
