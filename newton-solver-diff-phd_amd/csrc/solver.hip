@@ -1783,13 +1783,14 @@ int dphd_boundary_parameters(dphd_solver* s, const double* set, double* get) {
   DPHD_TRY({
     if (!s || !s->built || !s->has_friction) throw std::invalid_argument("dphd_boundary_parameters: no frictional boundary");
     HIP_CHECK(hipSetDevice(s->device));
-    HIP_CHECK(hipStreamSynchronize(s->stream));
     if (set) {
       for (int t = 0; t < 4; ++t)
         if (!std::isfinite(set[t])) throw std::invalid_argument("frictional boundary parameters must be four finite values");
-      HIP_CHECK(hipMemcpy(s->fc.params, set, sizeof(double) * 4, hipMemcpyHostToDevice));
+      // On the solver's stream, so the next step reads them after the copy.
+      HIP_CHECK(hipMemcpyAsync(s->fc.params, set, sizeof(double) * 4, hipMemcpyHostToDevice, s->stream));
     }
-    if (get) HIP_CHECK(hipMemcpy(get, s->fc.params, sizeof(double) * 4, hipMemcpyDeviceToHost));
+    if (get) HIP_CHECK(hipMemcpyAsync(get, s->fc.params, sizeof(double) * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
   })
 }
 
@@ -1805,6 +1806,9 @@ int dphd_contact_state(dphd_solver* s, const double* set_multiplier, double* mul
     if (omega && K) HIP_CHECK(hipMemcpy(omega, s->fc.omega, sizeof(double) * K, hipMemcpyDefault));
     if (normal && K) HIP_CHECK(hipMemcpy(normal, s->fc.normal, sizeof(double) * 3 * K, hipMemcpyDefault));
     if (candidate && K) HIP_CHECK(hipMemcpy(candidate, s->fc.candidate, sizeof(int) * K, hipMemcpyDefault));
+    // Device-to-device copies may still be running: finish them before the
+    // solver's stream or the caller uses these arrays.
+    HIP_CHECK(hipDeviceSynchronize());
   })
 }
 
@@ -2087,9 +2091,10 @@ int dphd_read_status(dphd_solver* s, int clear, int* bits) {
   DPHD_TRY({
     if (!s || !bits || !s->built) throw std::invalid_argument("dphd_read_status: invalid arguments");
     HIP_CHECK(hipSetDevice(s->device));
+    HIP_CHECK(hipMemcpyAsync(bits, s->d.status, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    // The clear on the solver's stream, ahead of the next step's kernels.
+    if (clear) HIP_CHECK(hipMemsetAsync(s->d.status, 0, sizeof(int), s->stream));
     HIP_CHECK(hipStreamSynchronize(s->stream));
-    HIP_CHECK(hipMemcpy(bits, s->d.status, sizeof(int), hipMemcpyDeviceToHost));
-    if (clear) HIP_CHECK(hipMemset(s->d.status, 0, sizeof(int)));
   })
 }
 
