@@ -74,6 +74,10 @@ struct RowCombine {
   int pad[3];
 };
 
+// Workgroups (one wavefront each) per vertex chunk of the adjoint operator
+// product.
+constexpr int kRowParts = 4;
+
 // Quads per column chunk of the split S list; the default of the build.
 constexpr int kSplitQuads = 256;
 // Workgroups of the two combine kernels (each strides over its items).
@@ -240,6 +244,7 @@ struct DeviceState {
   // still-active bodies); multiples of 8 once above 8, so that every XCD
   // holds the same number of workgroups.
   int g_vert = 0, g_elem = 0;
+  int g_rows = 0;  // the adjoint operator product's grid (one-wavefront workgroups, kRowParts per chunk)
   double* gram_part = nullptr;  // [nvc*kGramEntries]
   double* conv_part = nullptr;  // [nvc*4]
   double* clip_part2 = nullptr; // [nvc] squared norm of the adjoint's mixed iterate per chunk

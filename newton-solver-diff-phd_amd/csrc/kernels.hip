@@ -331,6 +331,23 @@ __global__ __launch_bounds__(kBlock) void k_active_prefix(const int* active, con
   write_mode(active, tcb, B, t_switch, s_switch, t_mode);
 }
 
+// Loop `fn(item, body, part)` over P parts of each active item listed by the
+// prefix `pref_g` [B + 1] (chunks `base`); block-uniform bounds, as below.
+template <int P, typename Fn>
+__device__ __forceinline__ void for_active_parts(const int* pref_g, const int* base, int B, Fn fn) {
+  extern __shared__ int pref[];
+  const int total = P * pref_g[B];
+  const ItemRange r = xcd_items(total, 1, 0);
+  if (r.begin + r.first >= r.end) return;
+  for (int b = threadIdx.x; b <= B; b += blockDim.x) pref[b] = pref_g[b];
+  __syncthreads();
+  for (int v = r.begin + r.first; v < r.end; v += r.stride) {
+    const int cv = v / P;
+    const int b = find_body(pref, B, cv);
+    fn(base[b] + (cv - pref[b]), b, v % P);
+  }
+}
+
 // Loop `fn(item, body)` over the active items listed by the prefix `pref_g`
 // [B + 1] (chunks `base`); the whole block iterates together (block-uniform
 // bounds), so fn may use block barriers. A workgroup without items reads one
@@ -1800,20 +1817,24 @@ __global__ __launch_bounds__(kBlock) void k_operator_assemble(DeviceState d) {
 }
 
 // work = K mu + rhs (gather with accumulate, adjoint.py:939), or
-// rhs + (K - A) mu in the defect mode; one thread per row. mu is the last
-// iteration's mixed iterate times its safe-clip scale (fscale * xr), which
-// this kernel also stores as the row's mu (sol).
-__global__ __launch_bounds__(kBlock) void k_adj_rhs(DeviceState d) {
-  for_active(d.pref_av, d.vcb, d.B, [&](int ci, int b) {
+// rhs + (K - A) mu in the defect mode; one thread per row, a workgroup of
+// one wavefront per kRowParts-th of a chunk (so a single body's rows spread
+// over that many more CUs). mu is the last iteration's mixed iterate times
+// its safe-clip scale (fscale * xr), which this kernel also stores as the
+// row's mu (sol).
+static_assert(kRowParts * kWave == kBlock, "the parts of a chunk cover its kBlock rows");
+__global__ __launch_bounds__(kWave) void k_adj_rhs(DeviceState d) {
+  for_active_parts<kRowParts>(d.pref_av, d.vcb, d.B, [&](int ci, int b, int part) {
     const Chunk c = d.vchunks[ci];
-    const int i = c.begin + threadIdx.x;
+    const int row = part * kWave + static_cast<int>(threadIdx.x);
+    const int i = c.begin + row;
     if (i >= c.end) return;
     const double sc = d.fscale[b];
     st3(d.sol, i, sc * ld3(d.xr, i));
     const int w = d.kwidth[ci];
     const size_t base = static_cast<size_t>(d.kbase[ci]);
-    const int* col = d.kcol + base + threadIdx.x;
-    const double* val = d.kval + 9 * base + threadIdx.x;
+    const int* col = d.kcol + base + row;
+    const double* val = d.kval + 9 * base + row;
     double t0 = 0.0, t1 = 0.0, t2 = 0.0;
     // Slots in batches of 8 column loads, then 4 slots at a time with all
     // their vertex and block loads issued before their products; padding
@@ -2426,7 +2447,7 @@ void launch_adjoint_setup(const DeviceState& d, const StepScalars& s, const doub
 
 void launch_adjoint_iteration(const DeviceState& d, const StepScalars& s, hipStream_t stream) {
   const size_t lds = prefix_lds(d);
-  hipLaunchKernelGGL(k_adj_rhs, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
+  hipLaunchKernelGGL(k_adj_rhs, dim3(d.g_rows), dim3(kWave), lds, stream, d);
   spmv_dispatch(d, d.xr, d.defect ? d.sol : nullptr, d.aactive, stream, AdjointEpilogue{d.free, d.t_norm, d.status});
   hipLaunchKernelGGL(k_adj_push, dim3(d.g_vert), dim3(kBlock), lds, stream, d);
   hipLaunchKernelGGL(k_adj_apply, dim3(d.g_vert), dim3(kBlock), lds, stream, d);

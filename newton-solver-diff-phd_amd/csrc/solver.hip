@@ -959,6 +959,8 @@ void build_solver(dphd_solver* s, double dt) {
   d.nec = static_cast<int>(echunks.size());
   d.ecb = dupload(o, ecb);
   d.g_vert = grid_for(d.nvc);
+  d.g_rows = std::max(1, std::min(kRowParts * d.nvc, 8192));
+  if (d.g_rows >= 8) d.g_rows &= ~7;
   d.g_elem = grid_for(d.nec);
   d.gram_part = dzeros<double>(o, static_cast<size_t>(d.nvc) * kGramEntries);
   d.conv_part = dzeros<double>(o, static_cast<size_t>(d.nvc) * 4);
