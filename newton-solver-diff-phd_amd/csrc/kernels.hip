@@ -332,7 +332,10 @@ __global__ __launch_bounds__(kBlock) void k_active_prefix(const int* active, con
 }
 
 // Loop `fn(item, body, part)` over P parts of each active item listed by the
-// prefix `pref_g` [B + 1] (chunks `base`); block-uniform bounds, as below.
+// prefix `pref_g` [B + 1] (chunks `base`); the whole block iterates together
+// (block-uniform bounds), so fn may use block barriers. A workgroup without
+// items reads one value; the others copy the prefix to LDS for the body
+// search.
 template <int P, typename Fn>
 __device__ __forceinline__ void for_active_parts(const int* pref_g, const int* base, int B, Fn fn) {
   extern __shared__ int pref[];
@@ -348,22 +351,10 @@ __device__ __forceinline__ void for_active_parts(const int* pref_g, const int* b
   }
 }
 
-// Loop `fn(item, body)` over the active items listed by the prefix `pref_g`
-// [B + 1] (chunks `base`); the whole block iterates together (block-uniform
-// bounds), so fn may use block barriers. A workgroup without items reads one
-// value; the others copy the prefix to LDS for the body search.
+// `fn(item, body)` over the active items, one part each.
 template <typename Fn>
 __device__ __forceinline__ void for_active(const int* pref_g, const int* base, int B, Fn fn) {
-  extern __shared__ int pref[];
-  const int total = pref_g[B];
-  const ItemRange r = xcd_items(total, 1, 0);
-  if (r.begin + r.first >= r.end) return;
-  for (int b = threadIdx.x; b <= B; b += blockDim.x) pref[b] = pref_g[b];
-  __syncthreads();
-  for (int v = r.begin + r.first; v < r.end; v += r.stride) {
-    const int b = find_body(pref, B, v);
-    fn(base[b] + (v - pref[b]), b);
-  }
+  for_active_parts<1>(pref_g, base, B, [&](int item, int b, int) { fn(item, b); });
 }
 
 // Right-hand side of the S product, 4-wide at the vertex's postorder index:
